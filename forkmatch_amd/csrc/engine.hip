@@ -1,0 +1,1538 @@
+// ForkMatch MI355X engine: the persistent subgraph-matching kernel, its
+// device work queues, the bank pool, the paged output table, the wave64
+// intersection microkernel, and the device-side C ABI.
+//
+// Reference counterparts (src/ = the reference tree):
+//  - initPosting/isoInit/isoLoop/isoMatch (src/smsm.cu:395-556,784-942):
+//    one persistent launch; each wave64 repeatedly takes a task (a range of
+//    partial embeddings of one query at one depth) and extends it one depth.
+//  - the dynamic-parallelism fork chain (src/smsm.cu:559-687,947-1139): a
+//    full output bank is not forked as a child grid; its rows are pushed as
+//    tasks onto device queues ordered by remaining depth, so deeper work
+//    runs first and memory stays bounded.
+//  - MemoryPool/forest (src/tbs.cuh:10-136): banks are carved by a device
+//    bump allocator from one arena and recycled through a device free list
+//    with reference counts. Partial embeddings are stored as full rows, not
+//    as (mapping, prev) chains, so a consumer reads a parent with one
+//    coalesced load instead of a dependent pointer walk.
+//  - isoGen/isoCompare/isInj (src/smsm.cu:55-140,187-227,297-345): the
+//    candidates of a batch of up to 64 parents are flattened with a wave
+//    prefix sum over the shortest requirement row of each parent, so every
+//    lane checks one candidate per step whatever the row lengths are;
+//    membership in the other rows is a binary search, injectivity a compare
+//    against the parent row in LDS.
+//  - isoWrite (src/smsm.cu:704-779): final-depth rows are written straight
+//    into lazily allocated output pages, columns in query-vertex order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "fm_internal.hpp"
+
+namespace fm {
+
+constexpr uint32_t WAVE = 64;
+constexpr uint32_t MAXN = FM_MAX_QUERY_VERTICES;
+constexpr uint32_t MAXR = FM_MAX_REQS_PER_DEPTH;
+constexpr uint32_t NLEVELS = MAXN;
+constexpr uint32_t LABEL_ABSENT = 0xFFFFFFu;
+constexpr uint32_t ROOT_BANK = 0xFFFFFFFFu;
+constexpr uint32_t NO_BANK = 0xFFFFFFFFu;
+constexpr uint32_t ALL_VERTICES = 0xFFFFFFFFu;
+constexpr uint32_t PAGE_EMPTY = 0, PAGE_BUSY = 1, PAGE_FAILED = 2, PAGE_BASE = 3;
+
+// Packed requirement: bits 0-5 anchor depth (== own depth: self-loop),
+// bit 7 direction (0 out-row, 1 in-row), bits 8-31 dense label id.
+__host__ __device__ inline uint32_t req_anchor(uint32_t r) { return r & 63u; }
+__host__ __device__ inline uint32_t req_dir(uint32_t r) { return (r >> 7) & 1u; }
+__host__ __device__ inline uint32_t req_label(uint32_t r) { return r >> 8; }
+
+struct DPlan {
+    uint32_t n;
+    uint32_t root_base;  // ALL_VERTICES or offset into the combined root list
+    uint32_t nroots;
+    uint32_t pad;
+    uint32_t order[MAXN];
+    uint32_t req_begin[MAXN + 1];
+    uint32_t nself[MAXN];  // number of self-loop requirements per depth
+    uint32_t req[FM_MAX_REQS_TOTAL];  // per depth: non-self first, then self
+};
+
+struct DQuery {
+    unsigned long long count;
+    unsigned long long rows;       // table rows reserved
+    unsigned long long row_words;  // instrumented
+    unsigned long long lookups;
+    unsigned long long partials[MAXN];
+    unsigned long long max_rows;
+    unsigned long long max_count;
+    unsigned long long max_partials;
+    unsigned long long stored;  // intermediate partial embeddings written
+    uint32_t status;
+    uint32_t flags;
+    uint32_t page_rows;
+    uint32_t max_pages;
+    uint32_t page_table_off;
+    uint32_t root_chunk_begin;
+    uint32_t root_chunks;
+    uint32_t pad;
+};
+
+enum StatIdx {
+    ST_TASKS = 0,
+    ST_ROOTS,
+    ST_BANKS_FRESH,
+    ST_BANKS_REUSED,
+    ST_ALLOC_FAIL,
+    ST_PAGES,
+    ST_REQUEUE,
+    ST_N
+};
+
+// Control block. The first part is cleared before every launch; the queue
+// and pool positions persist across launches (both are empty at the end of
+// a clean launch) and are re-initialised after an aborted one.
+struct Ctl {
+    long long pending;  // tasks pushed or root chunks not yet finished
+    unsigned long long root_cursor;
+    uint32_t abort;
+    uint32_t error;
+    uint32_t page_bump;
+    uint32_t pad0;
+    unsigned long long t0;  // wall clock of the first wave (deadline origin)
+    unsigned long long timeout_ticks;
+    unsigned long long stats[ST_N];
+    // persistent
+    uint32_t q_head[NLEVELS];
+    uint32_t q_tail[NLEVELS];
+    int32_t q_avail[NLEVELS];
+    uint32_t f_head;
+    uint32_t f_tail;
+    int32_t f_avail;
+    uint32_t bank_bump;
+};
+constexpr size_t CTL_RESET_BYTES = offsetof(Ctl, q_head);
+
+struct alignas(32) Slot {
+    uint32_t seq;
+    uint32_t q;
+    uint32_t depth;
+    uint32_t bank;
+    uint32_t begin;  // word offset in the bank, or first root index
+    uint32_t count;
+    uint32_t skip;   // candidates of the first parent already done
+    uint32_t pad;
+};
+
+struct alignas(8) FSlot {
+    uint32_t seq;
+    uint32_t id;
+};
+
+struct DGraph {
+    uint32_t V;
+    uint32_t L;
+    const uint32_t *ptr[2];  // row pointers into col (in-rows offset by E)
+    const uint32_t *col;
+    const uint32_t *roots;
+};
+
+struct KParams {
+    DGraph g;
+    const DPlan *plans;
+    DQuery *qs;
+    Ctl *ctl;
+    Slot *ring;
+    FSlot *free_ring;
+    int32_t *bank_ref;
+    uint32_t *banks;
+    uint32_t *pages;
+    uint32_t *page_table;
+    unsigned long long timeout_ticks;
+    unsigned long long starve_ticks;
+    unsigned long long bank_words;
+    unsigned long long page_words;
+    uint32_t nq;
+    uint32_t total_root_chunks;
+    uint32_t root_chunk;
+    uint32_t task_rows;
+    uint32_t table;
+    uint32_t instrument;
+    uint32_t qcap;
+    uint32_t fcap;
+    uint32_t nbanks;
+    uint32_t npages;
+    uint32_t nlevels;
+    uint32_t pad;
+};
+
+// ---------------------------------------------------------------------------
+// Device helpers.
+
+__device__ __forceinline__ uint32_t ld_acq(const uint32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t ld_rlx(const uint32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int32_t ld_rlx(const int32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ long long ld_rlx(const long long *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long ld_rlx(const unsigned long long *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_rel(uint32_t *p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_rlx(uint32_t *p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ __forceinline__ T atom_add(T *p, T v) {
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ bool cas_u32(uint32_t *p, uint32_t expect, uint32_t desired) {
+    return __hip_atomic_compare_exchange_strong(p, &expect, desired, __ATOMIC_ACQ_REL, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void backoff() { __builtin_amdgcn_s_sleep(2); }
+
+struct Ctl;
+__device__ bool spin_ok(const Ctl *c, uint32_t &iter);
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint32_t bcast(uint32_t v, int src) {
+    return (uint32_t)__shfl((int)v, src, WAVE);
+}
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int o = 1; o < (int)WAVE; o <<= 1) {
+        uint32_t t = (uint32_t)__shfl_up((int)v, o, WAVE);
+        if (lane >= (uint32_t)o) v += t;
+    }
+    return v;
+}
+
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+    uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+    // sum the halves separately with carries folded through 64-bit adds
+    unsigned long long acc = v;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        lo = (uint32_t)acc;
+        hi = (uint32_t)(acc >> 32);
+        uint32_t olo = (uint32_t)__shfl_xor((int)lo, o, WAVE);
+        uint32_t ohi = (uint32_t)__shfl_xor((int)hi, o, WAVE);
+        acc += ((unsigned long long)ohi << 32) | olo;
+    }
+    return acc;
+}
+
+__device__ __forceinline__ bool row_contains(const uint32_t *col, uint32_t st, uint32_t len, uint32_t x) {
+    uint32_t lo = st, hi = st + len;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (col[mid] < x) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo < st + len && col[lo] == x;
+}
+
+__device__ __forceinline__ void row_of(const DGraph &g, uint32_t req, uint32_t v, uint32_t &st, uint32_t &len) {
+    uint32_t lab = req_label(req);
+    if (lab == LABEL_ABSENT) {
+        st = 0;
+        len = 0;
+        return;
+    }
+    const uint32_t *p = g.ptr[req_dir(req)] + ((size_t)lab * g.V + v);
+    st = p[0];
+    len = p[1] - st;
+}
+
+// Every spin-wait calls this between polls: it sleeps briefly and, every
+// 256 polls, gives up (returns false) once the launch is aborting or past
+// its deadline, so no wait can outlive the launch.
+__device__ bool spin_ok(const Ctl *c, uint32_t &iter) {
+    backoff();
+    if ((++iter & 255u) != 0) return true;
+    if (__hip_atomic_load(&c->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
+    unsigned long long t0 = __hip_atomic_load(&c->t0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned long long lim = __hip_atomic_load(&c->timeout_ticks, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (t0 && wall_clock64() - t0 > lim) {
+        uint32_t zero = 0;
+        __hip_atomic_compare_exchange_strong(const_cast<uint32_t *>(&c->error), &zero, (uint32_t)FM_ERR_TIMEOUT,
+                                             __ATOMIC_RELAXED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(const_cast<uint32_t *>(&c->abort), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return false;
+    }
+    return true;
+}
+
+__device__ void raise_error(const KParams &P, uint32_t code) {
+    uint32_t zero = 0;
+    __hip_atomic_compare_exchange_strong(&P.ctl->error, &zero, code, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+    st_rlx(&P.ctl->abort, 1u);
+}
+
+__device__ void query_fail(const KParams &P, uint32_t q, uint32_t code) {
+    uint32_t zero = 0;
+    __hip_atomic_compare_exchange_strong(&P.qs[q].status, &zero, code, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---- bank pool (lane 0 only) ----------------------------------------------
+
+__device__ uint32_t bank_alloc(const KParams &P, unsigned long long *stats) {
+    Ctl *c = P.ctl;
+    if (ld_rlx(&c->f_avail) > 0) {
+        int32_t a = atom_add(&c->f_avail, -1);
+        if (a > 0) {
+            uint32_t pos = atom_add(&c->f_head, 1u);
+            FSlot *s = &P.free_ring[pos & (P.fcap - 1)];
+            uint32_t it = 0;
+            while (ld_acq(&s->seq) != pos + 1)
+                if (!spin_ok(c, it)) return NO_BANK;
+            uint32_t id = s->id;
+            st_rel(&s->seq, pos + P.fcap);
+            __hip_atomic_store(&P.bank_ref[id], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            stats[ST_BANKS_REUSED]++;
+            return id;
+        }
+        atom_add(&c->f_avail, 1);
+    }
+    if (ld_rlx(&c->bank_bump) < P.nbanks) {
+        uint32_t id = atom_add(&c->bank_bump, 1u);
+        if (id < P.nbanks) {
+            __hip_atomic_store(&P.bank_ref[id], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            stats[ST_BANKS_FRESH]++;
+            return id;
+        }
+    }
+    return NO_BANK;
+}
+
+__device__ void bank_release(const KParams &P, uint32_t id) {
+    if (atom_add(&P.bank_ref[id], -1) == 1) {
+        Ctl *c = P.ctl;
+        uint32_t pos = atom_add(&c->f_tail, 1u);
+        FSlot *s = &P.free_ring[pos & (P.fcap - 1)];
+        uint32_t it = 0;
+        while (ld_acq(&s->seq) != pos)
+            if (!spin_ok(c, it)) return;
+        s->id = id;
+        st_rel(&s->seq, pos + 1);
+        atom_add(&c->f_avail, 1);
+    }
+}
+
+// ---- task queues ------------------------------------------------------------
+
+struct TaskDesc {
+    uint32_t q, depth, bank, begin, count, skip;
+};
+
+// Pushes `m` tasks (lane i < m describes task i) at priority level lvl.
+// All lanes call. Row data the tasks reference must be written before.
+__device__ void push_tasks(const KParams &P, uint32_t lvl, uint32_t m, const TaskDesc &t, uint32_t bank,
+                           unsigned long long *stats) {
+    if (m == 0) return;
+    const uint32_t lane = lane_id();
+    Ctl *c = P.ctl;
+    uint32_t pos0 = 0, full = 0;
+    if (lane == 0) {
+        atom_add(&c->pending, (long long)m);
+        if (bank != ROOT_BANK) atom_add(&P.bank_ref[bank], (int32_t)m);
+        pos0 = atom_add(&c->q_tail[lvl], m);
+        uint32_t head = ld_rlx(&c->q_head[lvl]);
+        if (pos0 + m - head > P.qcap) full = 1;
+        stats[ST_TASKS] += m;
+    }
+    pos0 = bcast(pos0, 0);
+    full = bcast(full, 0);
+    if (full) {
+        if (lane == 0) raise_error(P, FM_ERR_QUEUE_FULL);
+        return;
+    }
+    // make the rows written by every lane visible before publication
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if (lane < m) {
+        uint32_t pos = pos0 + lane;
+        Slot *s = &P.ring[(size_t)lvl * P.qcap + (pos & (P.qcap - 1))];
+        uint32_t it = 0;
+        bool ok = true;
+        while (ok && ld_acq(&s->seq) != pos) ok = spin_ok(c, it);
+        if (ok) {
+        s->q = t.q;
+        s->depth = t.depth;
+        s->bank = t.bank;
+        s->begin = t.begin;
+        s->count = t.count;
+        s->skip = t.skip;
+        st_rel(&s->seq, pos + 1);
+        }
+    }
+    if (lane == 0) atom_add(&c->q_avail[lvl], (int32_t)m);
+}
+
+// Lane 0 only: takes the task of the lowest level (fewest remaining depths).
+__device__ bool pop_task(const KParams &P, TaskDesc &t) {
+    Ctl *c = P.ctl;
+    for (uint32_t lvl = 0; lvl < P.nlevels; lvl++) {
+        if (ld_rlx(&c->q_avail[lvl]) <= 0) continue;
+        if (atom_add(&c->q_avail[lvl], -1) <= 0) {
+            atom_add(&c->q_avail[lvl], 1);
+            continue;
+        }
+        uint32_t pos = atom_add(&c->q_head[lvl], 1u);
+        Slot *s = &P.ring[(size_t)lvl * P.qcap + (pos & (P.qcap - 1))];
+        uint32_t it = 0;
+        while (ld_acq(&s->seq) != pos + 1)
+            if (!spin_ok(c, it)) return false;
+        t.q = s->q;
+        t.depth = s->depth;
+        t.bank = s->bank;
+        t.begin = s->begin;
+        t.count = s->count;
+        t.skip = s->skip;
+        st_rel(&s->seq, pos + P.qcap);
+        return true;
+    }
+    return false;
+}
+
+// Lane 0 only: claims the next chunk of root vertices.
+__device__ bool claim_root(const KParams &P, TaskDesc &t, unsigned long long *stats) {
+    Ctl *c = P.ctl;
+    if (ld_rlx(&c->root_cursor) >= P.total_root_chunks) return false;
+    unsigned long long k = atom_add(&c->root_cursor, 1ull);
+    if (k >= P.total_root_chunks) return false;
+    // the query whose chunk range holds k
+    uint32_t lo = 0, hi = P.nq - 1;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi + 1) >> 1;
+        if (P.qs[mid].root_chunk_begin <= k) lo = mid;
+        else hi = mid - 1;
+    }
+    const DPlan &pl = P.plans[lo];
+    uint32_t local = (uint32_t)k - P.qs[lo].root_chunk_begin;
+    uint32_t first = local * P.root_chunk;
+    t.q = lo;
+    t.depth = 0;
+    t.bank = ROOT_BANK;
+    t.begin = first;
+    t.count = min(P.root_chunk, pl.nroots - first);
+    t.skip = 0;
+    stats[ST_ROOTS]++;
+    return true;
+}
+
+// ---- output pages (lane 0 only) ---------------------------------------------
+
+__device__ uint32_t ensure_page(const KParams &P, const DQuery &Q, unsigned long long pg,
+                                unsigned long long *stats) {
+    if (pg >= Q.max_pages) return UINT32_MAX;
+    uint32_t *e = &P.page_table[Q.page_table_off + pg];
+    uint32_t v = ld_acq(e);
+    uint32_t it = 0;
+    while (true) {
+        if (v >= PAGE_BASE) return v - PAGE_BASE;
+        if (v == PAGE_FAILED) return UINT32_MAX;
+        if (v == PAGE_EMPTY && cas_u32(e, PAGE_EMPTY, PAGE_BUSY)) {
+            uint32_t id = UINT32_MAX;
+            if (ld_rlx(&P.ctl->page_bump) < P.npages) id = atom_add(&P.ctl->page_bump, 1u);
+            if (id >= P.npages) {
+                st_rel(e, PAGE_FAILED);
+                return UINT32_MAX;
+            }
+            st_rel(e, id + PAGE_BASE);
+            stats[ST_PAGES]++;
+            return id;
+        }
+        if (!spin_ok(P.ctl, it)) return UINT32_MAX;
+        v = ld_acq(e);
+    }
+}
+
+// ---- the matcher --------------------------------------------------------------
+
+struct WaveOut {
+    uint32_t bank;       // current output bank or NO_BANK
+    uint32_t used;       // words used in the bank
+    uint32_t run_begin;  // word offset of the rows not yet pushed
+};
+
+struct Lds {
+    uint32_t par[WAVE * MAXN];          // parent rows of the batch
+    uint32_t rows_st[WAVE * MAXR];      // requirement rows: start
+    uint32_t rows_len[WAVE * MAXR];     // requirement rows: length
+    uint32_t dstart[WAVE];              // driver row start
+    uint32_t cum[WAVE];                 // inclusive scan of driver lengths
+    uint32_t didx[WAVE];                // driver requirement index
+    uint32_t pad[WAVE];
+};
+
+// Pushes the rows written since run_begin as tasks of at most task_rows
+// parents each (depth D, row width D + 1).
+__device__ void flush_run(const KParams &P, WaveOut &o, uint32_t q, uint32_t D, uint32_t lvl,
+                          unsigned long long *stats) {
+    const uint32_t W = D + 1;
+    uint32_t rows = (o.used - o.run_begin) / W;
+    if (rows == 0) return;
+    if (lane_id() == 0) {
+        DQuery &Q = P.qs[q];
+        unsigned long long stored = atom_add(&Q.stored, (unsigned long long)rows) + rows;
+        if (Q.max_partials && stored > Q.max_partials) query_fail(P, q, FM_ERR_COUNT_LIMIT);
+    }
+    uint32_t per = P.task_rows;
+    uint32_t ntasks = (rows + per - 1) / per;
+    const uint32_t lane = lane_id();
+    for (uint32_t t0 = 0; t0 < ntasks; t0 += WAVE) {
+        uint32_t m = min(WAVE, ntasks - t0);
+        TaskDesc td;
+        uint32_t ti = t0 + lane;
+        td.q = q;
+        td.depth = D;
+        td.bank = o.bank;
+        td.begin = o.run_begin + ti * per * W;
+        td.count = min(per, rows - min(rows, ti * per));
+        td.skip = 0;
+        push_tasks(P, lvl, m, td, o.bank, stats);
+    }
+    o.run_begin = o.used;
+}
+
+
+// Extends one task by one depth. Returns after pushing children tasks; the
+// caller completes the task (bank reference and pending count).
+__device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOut &o,
+                             unsigned long long *stats, unsigned long long &starve_since,
+                             unsigned long long deadline) {
+    const uint32_t lane = lane_id();
+    const DPlan &pl = P.plans[t.q];
+    DQuery &Q = P.qs[t.q];
+    if (ld_rlx(&Q.status) != 0) return;
+    const uint32_t n = pl.n;
+    const uint32_t d = t.depth;
+    const uint32_t W = d + 1;  // parent width
+    const bool root = t.bank == ROOT_BANK;
+    const DGraph &g = P.g;
+
+    // ---- single-vertex query: every root is an embedding -----------------
+    if (n == 1) {
+        unsigned long long cnt_acc = 0;
+        for (uint32_t b0 = 0; b0 < t.count; b0 += WAVE) {
+            uint32_t nb = min(WAVE, t.count - b0);
+            bool ok = lane < nb;
+            uint32_t v = t.begin + b0 + lane;
+            if (ok && pl.root_base != ALL_VERTICES) v = g.roots[pl.root_base + t.begin + b0 + lane];
+            for (uint32_t i = 0; i < pl.nself[0] && ok; i++) {
+                uint32_t st, len;
+                row_of(g, pl.req[pl.req_begin[0] + i], v, st, len);
+                ok = row_contains(g.col, st, len, v);
+            }
+            unsigned long long mask = __ballot(ok);
+            uint32_t cnt = __popcll(mask);
+            uint32_t rank = __popcll(mask & ((1ull << lane) - 1ull));
+            cnt_acc += cnt;
+            if (P.instrument && lane == 0) atom_add(&Q.partials[0], (unsigned long long)cnt);
+            if (P.table && cnt) {
+                unsigned long long base = 0;
+                uint32_t p0 = UINT32_MAX, p1 = UINT32_MAX;
+                if (lane == 0) {
+                    base = atom_add(&Q.rows, (unsigned long long)cnt);
+                    if (base < Q.max_rows) {
+                        unsigned long long last = min(base + cnt - 1, Q.max_rows - 1);
+                        p0 = ensure_page(P, Q, base / Q.page_rows, stats);
+                        p1 = ensure_page(P, Q, last / Q.page_rows, stats);
+                    }
+                    if (base + cnt > Q.max_rows || p0 == UINT32_MAX || p1 == UINT32_MAX)
+                        __hip_atomic_fetch_or(&Q.flags, FM_RESULT_TRUNCATED, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+                }
+                base = ((unsigned long long)bcast((uint32_t)(base >> 32), 0) << 32) | bcast((uint32_t)base, 0);
+                p0 = bcast(p0, 0);
+                p1 = bcast(p1, 0);
+                unsigned long long r = base + rank;
+                if (ok && r < Q.max_rows) {
+                    unsigned long long pg = r / Q.page_rows;
+                    uint32_t id = pg == base / Q.page_rows ? p0 : p1;
+                    if (id != UINT32_MAX) P.pages[(size_t)id * P.page_words + (r % Q.page_rows)] = v;
+                }
+            }
+        }
+        if (lane == 0) atom_add(&Q.count, cnt_acc);
+        return;
+    }
+
+    const uint32_t D = d + 1;  // depth of the children
+    const bool final_level = D == n - 1;
+    const uint32_t lvl_children = final_level ? 0 : n - 2 - D;
+    const uint32_t lvl_self = n - 2 - d;
+    const uint32_t rb = pl.req_begin[D];
+    const uint32_t nself = pl.nself[D];
+    const uint32_t k = pl.req_begin[D + 1] - rb - nself;  // anchored requirements
+
+    unsigned long long cnt_acc = 0;
+    for (uint32_t b0 = 0; b0 < t.count; b0 += WAVE) {
+        // periodic checks
+        uint32_t stop = 0;
+        if (lane == 0) stop = (ld_rlx(&P.ctl->abort) || ld_rlx(&Q.status)) ? 1u : 0u;
+        if (bcast(stop, 0)) break;
+        if (wall_clock64() > deadline) {
+            if (lane == 0) raise_error(P, FM_ERR_TIMEOUT);
+            break;
+        }
+        const uint32_t nb = min(WAVE, t.count - b0);
+        // ---- parents into LDS ----
+        if (root) {
+            if (lane < nb) {
+                uint32_t idx = t.begin + b0 + lane;
+                s.par[lane] = pl.root_base == ALL_VERTICES ? idx : g.roots[pl.root_base + idx];
+            }
+        } else {
+            const uint32_t *src = P.banks + (size_t)t.bank * P.bank_words + t.begin + (size_t)b0 * W;
+            for (uint32_t w = lane; w < nb * W; w += WAVE) s.par[w] = src[w];
+        }
+        __syncthreads();
+        // ---- requirement rows per parent; driver = shortest ----
+        uint32_t dlen = 0, dst_ = 0, didx = 0;
+        unsigned long long words = 0;
+        bool alive = lane < nb;
+        if (alive && root && pl.nself[0]) {
+            uint32_t v = s.par[lane];
+            for (uint32_t i = 0; i < pl.nself[0] && alive; i++) {
+                uint32_t st, len;
+                row_of(g, pl.req[pl.req_begin[0] + i], v, st, len);
+                alive = row_contains(g.col, st, len, v);
+            }
+        }
+        if (lane < nb) {
+            uint32_t best = 0xFFFFFFFFu;
+            for (uint32_t i = 0; i < k; i++) {
+                uint32_t r = pl.req[rb + i];
+                uint32_t st, len;
+                row_of(g, r, s.par[lane * W + req_anchor(r)], st, len);
+                s.rows_st[lane * MAXR + i] = st;
+                s.rows_len[lane * MAXR + i] = len;
+                words += len;
+                if (len < best) {
+                    best = len;
+                    dst_ = st;
+                    didx = i;
+                }
+            }
+            dlen = alive ? best : 0;
+            if (b0 == 0 && lane == 0 && t.skip) {
+                uint32_t sk = min(t.skip, dlen);
+                dst_ += sk;
+                dlen -= sk;
+            }
+        }
+        s.dstart[lane] = dst_;
+        s.didx[lane] = didx;
+        const uint32_t incl = wave_incl_scan(dlen);
+        s.cum[lane] = incl;
+        const uint32_t T = bcast(incl, WAVE - 1);
+        if (P.instrument) {
+            unsigned long long wsum = wave_sum_u64(alive ? words : 0ull);
+            uint32_t nalive = __popcll(__ballot(alive));
+            if (lane == 0) {
+                atom_add(&Q.row_words, wsum);
+                atom_add(&Q.lookups, (unsigned long long)nalive * k);
+                if (root) atom_add(&Q.partials[0], (unsigned long long)nalive);
+            }
+        }
+        __syncthreads();
+        // ---- candidates, 64 per step ----
+        for (uint32_t base = 0; base < T; base += WAVE) {
+            if (!final_level) {
+                const uint32_t need = WAVE * (D + 1);
+                bool yield = false;
+                if (o.bank != NO_BANK && o.used + need > P.bank_words) {
+                    // The output bank is full: hand its rows over as tasks and
+                    // yield, so the deeper tasks just pushed run before this
+                    // one produces more (bounded, depth-first memory use).
+                    flush_run(P, o, t.q, D, lvl_children, stats);
+                    if (lane == 0) bank_release(P, o.bank);
+                    o.bank = NO_BANK;
+                    yield = true;
+                } else if (o.bank == NO_BANK) {
+                    uint32_t nbk = NO_BANK;
+                    if (lane == 0) nbk = bank_alloc(P, stats);
+                    nbk = bcast(nbk, 0);
+                    if (nbk == NO_BANK) {
+                        // No bank: requeue and take deeper (bank-freeing) work.
+                        if (lane == 0) {
+                            stats[ST_ALLOC_FAIL]++;
+                            unsigned long long now = wall_clock64();
+                            if (starve_since == 0) starve_since = now;
+                            else if (now - starve_since > P.starve_ticks) raise_error(P, FM_ERR_POOL_EXHAUSTED);
+                        }
+                        yield = true;
+                    } else {
+                        o.bank = nbk;
+                        o.used = 0;
+                        o.run_begin = 0;
+                        if (lane == 0) starve_since = 0;
+                    }
+                }
+                if (yield) {
+                    // remainder: from the parent of candidate `base`
+                    uint32_t lo = 0, hi = nb - 1;
+                    while (lo < hi) {
+                        uint32_t mid = (lo + hi) >> 1;
+                        if (s.cum[mid] > base) hi = mid;
+                        else lo = mid + 1;
+                    }
+                    const uint32_t excl = lo ? s.cum[lo - 1] : 0;
+                    TaskDesc r;
+                    r.q = t.q;
+                    r.depth = d;
+                    r.bank = t.bank;
+                    r.begin = root ? t.begin + b0 + lo : t.begin + (b0 + lo) * W;
+                    r.count = t.count - b0 - lo;
+                    r.skip = (base - excl) + ((b0 == 0 && lo == 0) ? t.skip : 0);
+                    if (lane == 0) stats[ST_REQUEUE]++;
+                    push_tasks(P, lvl_self, 1, r, t.bank, stats);
+                    return;
+                }
+            }
+            const uint32_t tt = base + lane;
+            bool ok = false;
+            uint32_t x = 0, pi = 0;
+            if (tt < T) {
+                uint32_t lo = 0, hi = nb - 1;
+                while (lo < hi) {
+                    uint32_t mid = (lo + hi) >> 1;
+                    if (s.cum[mid] > tt) hi = mid;
+                    else lo = mid + 1;
+                }
+                pi = lo;
+                uint32_t excl = pi ? s.cum[pi - 1] : 0;
+                x = g.col[s.dstart[pi] + (tt - excl)];
+                ok = true;
+                const uint32_t *prow = &s.par[pi * W];
+                for (uint32_t a = 0; a < W; a++) ok &= prow[a] != x;
+                const uint32_t dx = s.didx[pi];
+                for (uint32_t i = 0; i < k && ok; i++)
+                    if (i != dx) ok = row_contains(g.col, s.rows_st[pi * MAXR + i], s.rows_len[pi * MAXR + i], x);
+                for (uint32_t i = 0; i < nself && ok; i++) {
+                    uint32_t st, len;
+                    row_of(g, pl.req[rb + k + i], x, st, len);
+                    ok = row_contains(g.col, st, len, x);
+                }
+            }
+            const unsigned long long mask = __ballot(ok);
+            const uint32_t cnt = __popcll(mask);
+            if (cnt == 0) continue;
+            const uint32_t rank = __popcll(mask & ((1ull << lane) - 1ull));
+            if (P.instrument && lane == 0) atom_add(&Q.partials[D], (unsigned long long)cnt);
+            if (final_level) {
+                cnt_acc += cnt;
+                if (P.table) {
+                    unsigned long long base_row = 0;
+                    uint32_t p0 = UINT32_MAX, p1 = UINT32_MAX;
+                    if (lane == 0) {
+                        base_row = atom_add(&Q.rows, (unsigned long long)cnt);
+                        if (base_row < Q.max_rows) {
+                            unsigned long long last = min(base_row + cnt - 1, Q.max_rows - 1);
+                            p0 = ensure_page(P, Q, base_row / Q.page_rows, stats);
+                            p1 = ensure_page(P, Q, last / Q.page_rows, stats);
+                        }
+                        if (base_row + cnt > Q.max_rows || p0 == UINT32_MAX || p1 == UINT32_MAX)
+                            __hip_atomic_fetch_or(&Q.flags, FM_RESULT_TRUNCATED, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                    base_row = ((unsigned long long)bcast((uint32_t)(base_row >> 32), 0) << 32) |
+                               bcast((uint32_t)base_row, 0);
+                    p0 = bcast(p0, 0);
+                    p1 = bcast(p1, 0);
+                    unsigned long long r = base_row + rank;
+                    if (ok && r < Q.max_rows) {
+                        uint32_t id = (r / Q.page_rows) == (base_row / Q.page_rows) ? p0 : p1;
+                        if (id != UINT32_MAX) {
+                            uint32_t *dst = P.pages + (size_t)id * P.page_words + (r % Q.page_rows) * n;
+                            const uint32_t *prow = &s.par[pi * W];
+                            for (uint32_t j = 0; j < W; j++) dst[pl.order[j]] = prow[j];
+                            dst[pl.order[D]] = x;
+                        }
+                    }
+                }
+            } else {
+                const uint32_t CW = D + 1;
+                if (ok) {
+                    uint32_t *dst = P.banks + (size_t)o.bank * P.bank_words + o.used + rank * CW;
+                    const uint32_t *prow = &s.par[pi * W];
+                    for (uint32_t j = 0; j < W; j++) dst[j] = prow[j];
+                    dst[W] = x;
+                }
+                o.used += cnt * CW;
+                if ((o.used - o.run_begin) / CW >= P.task_rows) flush_run(P, o, t.q, D, lvl_children, stats);
+            }
+        }
+        if (final_level && lane == 0 && cnt_acc >= (1ull << 20)) {
+            unsigned long long now_cnt = atom_add(&Q.count, cnt_acc) + cnt_acc;
+            cnt_acc = 0;
+            if (Q.max_count && now_cnt > Q.max_count) query_fail(P, t.q, FM_ERR_COUNT_LIMIT);
+        }
+        __syncthreads();
+    }
+    if (!final_level && o.bank != NO_BANK) flush_run(P, o, t.q, D, lvl_children, stats);
+    if (lane == 0 && cnt_acc) {
+        unsigned long long now_cnt = atom_add(&Q.count, cnt_acc) + cnt_acc;
+        if (Q.max_count && now_cnt > Q.max_count) query_fail(P, t.q, FM_ERR_COUNT_LIMIT);
+    }
+}
+
+__global__ __launch_bounds__(WAVE) void fm_match_kernel(KParams P) {
+    __shared__ Lds s;
+    __shared__ TaskDesc s_task;
+    __shared__ uint32_t s_flag;
+    const uint32_t lane = lane_id();
+    WaveOut o{NO_BANK, 0, 0};
+    unsigned long long stats[ST_N] = {0, 0, 0, 0, 0, 0, 0};
+    unsigned long long starve_since = 0;
+    unsigned long long deadline = 0;
+    if (lane == 0) {
+        unsigned long long now = wall_clock64(), zero = 0;
+        __hip_atomic_compare_exchange_strong(&P.ctl->t0, &zero, now, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+        deadline = ld_rlx(&P.ctl->t0) + P.timeout_ticks;
+    }
+    deadline = ((unsigned long long)bcast((uint32_t)(deadline >> 32), 0) << 32) | bcast((uint32_t)deadline, 0);
+    while (true) {
+        // 0: work, 1: idle, 2: exit
+        if (lane == 0) {
+            uint32_t flag = 1;
+            if (ld_rlx(&P.ctl->abort)) {
+                flag = 2;
+            } else if (wall_clock64() > deadline) {
+                raise_error(P, FM_ERR_TIMEOUT);
+                flag = 2;
+            } else {
+                TaskDesc t;
+                if (pop_task(P, t) || claim_root(P, t, stats)) {
+                    s_task = t;
+                    flag = 0;
+                } else if (ld_rlx(&P.ctl->pending) <= 0) {
+                    flag = 2;
+                }
+            }
+            s_flag = flag;
+        }
+        __syncthreads();
+        const uint32_t flag = s_flag;
+        if (flag == 2) break;
+        if (flag == 1) {
+            backoff();
+            continue;
+        }
+        const TaskDesc t = s_task;
+        __syncthreads();
+        process_task(P, t, s, o, stats, starve_since, deadline);
+        __syncthreads();
+        if (lane == 0) {
+            if (t.bank != ROOT_BANK) bank_release(P, t.bank);
+            atom_add(&P.ctl->pending, -1ll);
+        }
+    }
+    if (lane == 0) {
+        if (o.bank != NO_BANK) bank_release(P, o.bank);
+        for (int i = 0; i < ST_N; i++)
+            if (stats[i]) atom_add(&P.ctl->stats[i], stats[i]);
+    }
+}
+
+__global__ void fm_init_slots_kernel(Slot *ring, uint32_t qcap, uint32_t levels, FSlot *fring, uint32_t fcap) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t nq = (size_t)qcap * levels;
+    if (i < nq) {
+        Slot s{};
+        s.seq = (uint32_t)(i & (qcap - 1));
+        ring[i] = s;
+    }
+    if (i < fcap) {
+        FSlot f{(uint32_t)i, 0};
+        fring[i] = f;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Wave64 intersection microkernel: one wave per pair, grid-stride. The
+// shorter list is staged in LDS in tiles; the longer list's slice that
+// overlaps the tile's value range is streamed with coalesced loads and each
+// lane binary-searches its element in the LDS tile. Matches are compacted
+// in order with a ballot, so the output is ascending.
+
+constexpr uint32_t ITILE = 2048;
+
+__global__ __launch_bounds__(WAVE) void fm_intersect_kernel(const uint32_t *__restrict__ data,
+                                                            const uint64_t *a_off, const uint32_t *a_len,
+                                                            const uint64_t *b_off, const uint32_t *b_len,
+                                                            uint32_t npairs, uint32_t *counts,
+                                                            const uint64_t *out_off, uint32_t *out) {
+    __shared__ uint32_t tile[ITILE];
+    const uint32_t lane = lane_id();
+    for (uint32_t p = blockIdx.x; p < npairs; p += gridDim.x) {
+        const uint32_t *A = data + a_off[p];
+        const uint32_t *B = data + b_off[p];
+        uint32_t na = a_len[p], nbl = b_len[p];
+        if (na > nbl) {
+            const uint32_t *tp = A;
+            A = B;
+            B = tp;
+            uint32_t tn = na;
+            na = nbl;
+            nbl = tn;
+        }
+        uint32_t found = 0;
+        uint32_t *o = out_off ? out + out_off[p] : nullptr;
+        uint32_t bpos = 0;  // B elements below the current tile are skipped
+        for (uint32_t t0 = 0; t0 < na && bpos < nbl; t0 += ITILE) {
+            const uint32_t tn = min(ITILE, na - t0);
+            __syncthreads();
+            for (uint32_t i = lane; i < tn; i += WAVE) tile[i] = A[t0 + i];
+            __syncthreads();
+            const uint32_t lo_v = tile[0], hi_v = tile[tn - 1];
+            // first B index >= lo_v (from bpos), and first > hi_v
+            uint32_t lo = bpos, hi = nbl;
+            while (lo < hi) {
+                uint32_t mid = (lo + hi) >> 1;
+                if (B[mid] < lo_v) lo = mid + 1;
+                else hi = mid;
+            }
+            uint32_t b_begin = lo;
+            hi = nbl;
+            while (lo < hi) {
+                uint32_t mid = (lo + hi) >> 1;
+                if (B[mid] <= hi_v) lo = mid + 1;
+                else hi = mid;
+            }
+            const uint32_t b_end = lo;
+            for (uint32_t j0 = b_begin; j0 < b_end; j0 += WAVE) {
+                const uint32_t j = j0 + lane;
+                bool hit = false;
+                uint32_t x = 0;
+                if (j < b_end) {
+                    x = B[j];
+                    uint32_t l2 = 0, h2 = tn;
+                    while (l2 < h2) {
+                        uint32_t mid = (l2 + h2) >> 1;
+                        if (tile[mid] < x) l2 = mid + 1;
+                        else h2 = mid;
+                    }
+                    hit = l2 < tn && tile[l2] == x;
+                }
+                const unsigned long long m = __ballot(hit);
+                if (o && hit) o[found + __popcll(m & ((1ull << lane) - 1ull))] = x;
+                found += __popcll(m);
+            }
+            bpos = b_end;
+        }
+        if (lane == 0) counts[p] = found;
+    }
+}
+
+}  // namespace fm
+
+// ===========================================================================
+// Host side.
+using namespace fm;
+
+#define HIP_TRY(expr)                                                                       \
+    do {                                                                                    \
+        hipError_t _e = (expr);                                                             \
+        if (_e != hipSuccess)                                                               \
+            return fail(_e == hipErrorOutOfMemory ? FM_ERR_OUT_OF_MEMORY : FM_ERR_DEVICE,   \
+                        "%s failed: %s", #expr, hipGetErrorString(_e));                     \
+    } while (0)
+
+struct fm_engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int cu_count = 0;
+    int waves_per_cu = 0;
+    uint32_t grid_waves = 0;
+    double wall_hz = 100e6;
+    char name[64] = {0};
+    // arenas
+    uint32_t *banks = nullptr;
+    uint64_t bank_words = 0;
+    uint32_t nbanks = 0;
+    int32_t *bank_ref = nullptr;
+    FSlot *free_ring = nullptr;
+    uint32_t fcap = 0;
+    Slot *ring = nullptr;
+    uint32_t qcap = 0;
+    uint32_t *pages = nullptr;
+    uint64_t page_words = 0;
+    uint32_t npages = 0;
+    Ctl *ctl = nullptr;
+    // per-launch buffers (grow only)
+    uint32_t *page_table = nullptr;
+    size_t page_table_cap = 0;
+    DPlan *d_plans = nullptr;
+    DQuery *d_qs = nullptr;
+    size_t q_cap = 0;
+    uint32_t max_queries = 0;
+    uint64_t epoch = 0;
+    bool dirty = true;  // queues/pool need a full re-initialisation
+};
+
+struct fm_result {
+    fm_engine *e = nullptr;
+    uint64_t epoch = 0;
+    std::vector<DQuery> qs;
+    std::vector<uint32_t> n;
+    fm_launch_stats stats{};
+};
+
+static int full_reset(fm_engine *e) {
+    HIP_TRY(hipSetDevice(e->device));
+    HIP_TRY(hipMemsetAsync(e->ctl, 0, sizeof(Ctl), e->stream));
+    size_t total = std::max<size_t>((size_t)e->qcap * NLEVELS, e->fcap);
+    unsigned blocks = (unsigned)((total + 255) / 256);
+    hipLaunchKernelGGL(fm_init_slots_kernel, dim3(blocks), dim3(256), 0, e->stream, e->ring, e->qcap,
+                       NLEVELS, e->free_ring, e->fcap);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    e->dirty = false;
+    return FM_OK;
+}
+
+static uint32_t pow2_at_least(uint64_t x) {
+    uint32_t p = 1;
+    while (p < x && p < (1u << 31)) p <<= 1;
+    return p;
+}
+
+extern "C" {
+
+int fm_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int fm_graph_upload(fm_graph *g) {
+    if (!g) return fail(FM_ERR_INVALID_ARG, "null graph");
+    if (g->device >= 0) return FM_OK;
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    const HostGraph &h = g->h;
+    uint64_t E = h.E();
+    if (!h.symmetric && 2 * E >= (1ull << 32))
+        return fail(FM_ERR_NOT_SUPPORTED, "directed graph with %llu edges exceeds the u32 row index",
+                    (unsigned long long)E);
+    uint64_t rows = (uint64_t)h.L() * h.V + 1;
+    uint64_t bytes = 0;
+    auto dalloc = [&](uint32_t **p, uint64_t n) -> int {
+        HIP_TRY(hipMalloc((void **)p, std::max<uint64_t>(n, 1) * 4));
+        bytes += n * 4;
+        return FM_OK;
+    };
+    int rc;
+    if ((rc = dalloc(&g->d_ptr[0], rows))) return rc;
+    HIP_TRY(hipMemcpy(g->d_ptr[0], h.out.ptr.data(), rows * 4, hipMemcpyHostToDevice));
+    uint64_t ncol = h.symmetric ? E : 2 * E;
+    if ((rc = dalloc(&g->d_col[0], ncol))) return rc;
+    HIP_TRY(hipMemcpy(g->d_col[0], h.out.col.data(), E * 4, hipMemcpyHostToDevice));
+    uint64_t nroots = h.out.roots.size() + (h.symmetric ? 0 : h.in.roots.size());
+    if ((rc = dalloc(&g->d_roots[0], nroots))) return rc;
+    HIP_TRY(hipMemcpy(g->d_roots[0], h.out.roots.data(), h.out.roots.size() * 4, hipMemcpyHostToDevice));
+    if (h.symmetric) {
+        g->d_ptr[1] = g->d_ptr[0];
+        g->d_col[1] = g->d_col[0];
+        g->d_roots[1] = g->d_roots[0];
+    } else {
+        std::vector<uint32_t> shifted(h.in.ptr);
+        for (auto &v : shifted) v += (uint32_t)E;
+        if ((rc = dalloc(&g->d_ptr[1], rows))) return rc;
+        HIP_TRY(hipMemcpy(g->d_ptr[1], shifted.data(), rows * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(g->d_col[0] + E, h.in.col.data(), E * 4, hipMemcpyHostToDevice));
+        g->d_col[1] = g->d_col[0];
+        HIP_TRY(hipMemcpy(g->d_roots[0] + h.out.roots.size(), h.in.roots.data(), h.in.roots.size() * 4,
+                          hipMemcpyHostToDevice));
+        g->d_roots[1] = g->d_roots[0];
+    }
+    g->device = dev;
+    g->device_bytes = bytes;
+    return FM_OK;
+}
+
+int fm_graph_load_g(const char *path, fm_graph **out) {
+    int rc = fm_graph_parse_g(path, out);
+    if (rc) return rc;
+    rc = fm_graph_upload(*out);
+    if (rc) {
+        fm_graph_destroy(*out);
+        *out = nullptr;
+    }
+    return rc;
+}
+
+int fm_graph_copy_device_csr(const fm_graph *g, int dir, uint32_t *ptr, uint32_t *col) {
+    if (!g || g->device < 0 || (dir != 0 && dir != 1)) return fail(FM_ERR_INVALID_ARG, "graph not uploaded");
+    const HostGraph &h = g->h;
+    uint64_t rows = (uint64_t)h.L() * h.V + 1, E = h.E();
+    bool in = dir == 1 && !h.symmetric;
+    if (ptr) {
+        HIP_TRY(hipMemcpy(ptr, g->d_ptr[dir], rows * 4, hipMemcpyDeviceToHost));
+        if (in)
+            for (uint64_t i = 0; i < rows; i++) ptr[i] -= (uint32_t)E;
+    }
+    if (col) HIP_TRY(hipMemcpy(col, g->d_col[0] + (in ? E : 0), E * 4, hipMemcpyDeviceToHost));
+    return FM_OK;
+}
+
+void fm_graph_destroy(fm_graph *g) {
+    if (!g) return;
+    if (g->device >= 0) {
+        (void)hipFree(g->d_ptr[0]);
+        if (g->d_ptr[1] != g->d_ptr[0]) (void)hipFree(g->d_ptr[1]);
+        (void)hipFree(g->d_col[0]);
+        (void)hipFree(g->d_roots[0]);
+    }
+    delete g;
+}
+
+int fm_engine_create(const fm_engine_config *cfg, fm_engine **out) {
+    if (!out) return fail(FM_ERR_INVALID_ARG, "null argument");
+    fm_engine_config c{};
+    if (cfg) c = *cfg;
+    if (!c.soln_prealloc) c.soln_prealloc = SOLN_PREALLOC;
+    if (!c.output_prealloc) c.output_prealloc = OUTPUT_PREALLOC;
+    if (!c.bank_size) c.bank_size = CurBankSize;
+    if (!c.output_page_bytes) c.output_page_bytes = OUTPUT_PAGE_BYTES;
+    if (!c.queue_capacity) c.queue_capacity = 1u << 20;
+    if (!c.max_queries) c.max_queries = 1u << 16;
+    int ndev = 0;
+    HIP_TRY(hipGetDeviceCount(&ndev));
+    if (ndev == 0) return fail(FM_ERR_DEVICE, "no HIP device");
+    if (c.device < 0 || c.device >= ndev) return fail(FM_ERR_INVALID_ARG, "device %d out of range", c.device);
+    auto *e = new fm_engine();
+    e->device = c.device;
+    e->max_queries = c.max_queries;
+    auto cleanup_fail = [&](int rc) {
+        fm_engine_destroy(e);
+        return rc;
+    };
+    if (hipSetDevice(e->device) != hipSuccess) return cleanup_fail(fail(FM_ERR_DEVICE, "hipSetDevice failed"));
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, e->device) != hipSuccess)
+        return cleanup_fail(fail(FM_ERR_DEVICE, "hipGetDeviceProperties failed"));
+    e->cu_count = prop.multiProcessorCount;
+    snprintf(e->name, sizeof e->name, "%s", prop.gcnArchName);
+    int wall_khz = 0;
+    if (hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, e->device) == hipSuccess && wall_khz > 0)
+        e->wall_hz = wall_khz * 1000.0;
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fm_match_kernel, WAVE, 0) != hipSuccess || blocks <= 0)
+        blocks = 8;
+    e->waves_per_cu = blocks;
+    e->grid_waves = (uint32_t)(blocks * e->cu_count);
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess)
+        return cleanup_fail(fail(FM_ERR_DEVICE, "stream/event creation failed"));
+    if (c.bank_size < WAVE * MAXN / 2)
+        return cleanup_fail(fail(FM_ERR_INVALID_ARG, "bank_size %u is below the minimum %u (one step of rows)",
+                                 c.bank_size, WAVE * MAXN / 2));
+    e->bank_words = (uint64_t)c.bank_size * 2;  // 8-byte units -> u32 words
+    e->nbanks = (uint32_t)std::min<uint64_t>(c.soln_prealloc / (e->bank_words * 4), 1u << 30);
+    if (e->nbanks == 0) return cleanup_fail(fail(FM_ERR_INVALID_ARG, "SOLN_PREALLOC smaller than one bank"));
+    e->fcap = pow2_at_least(e->nbanks);
+    e->qcap = pow2_at_least(c.queue_capacity);
+    e->page_words = c.output_page_bytes / 4;
+    e->npages = (uint32_t)std::min<uint64_t>(c.output_prealloc / c.output_page_bytes, 1u << 30);
+    if (e->npages == 0) return cleanup_fail(fail(FM_ERR_INVALID_ARG, "OUTPUT_PREALLOC smaller than one page"));
+    auto alloc = [&](void **p, size_t bytes) { return hipMalloc(p, std::max<size_t>(bytes, 4)) == hipSuccess; };
+    if (!alloc((void **)&e->banks, (size_t)e->nbanks * e->bank_words * 4) ||
+        !alloc((void **)&e->bank_ref, (size_t)e->nbanks * 4) ||
+        !alloc((void **)&e->free_ring, (size_t)e->fcap * sizeof(FSlot)) ||
+        !alloc((void **)&e->ring, (size_t)e->qcap * NLEVELS * sizeof(Slot)) ||
+        !alloc((void **)&e->pages, (size_t)e->npages * e->page_words * 4) ||
+        !alloc((void **)&e->ctl, sizeof(Ctl)))
+        return cleanup_fail(fail(FM_ERR_OUT_OF_MEMORY, "engine arenas do not fit in device memory"));
+    int rc = full_reset(e);
+    if (rc) return cleanup_fail(rc);
+    *out = e;
+    return FM_OK;
+}
+
+void fm_engine_destroy(fm_engine *e) {
+    if (!e) return;
+    (void)hipSetDevice(e->device);
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    (void)hipFree(e->banks);
+    (void)hipFree(e->bank_ref);
+    (void)hipFree(e->free_ring);
+    (void)hipFree(e->ring);
+    (void)hipFree(e->pages);
+    (void)hipFree(e->ctl);
+    (void)hipFree(e->page_table);
+    (void)hipFree(e->d_plans);
+    (void)hipFree(e->d_qs);
+    if (e->ev0) (void)hipEventDestroy(e->ev0);
+    if (e->ev1) (void)hipEventDestroy(e->ev1);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+}
+
+int fm_engine_info_get(fm_engine *e, fm_engine_info *out) {
+    if (!e || !out) return fail(FM_ERR_INVALID_ARG, "null argument");
+    out->compute_units = (uint32_t)e->cu_count;
+    out->waves_per_cu = (uint32_t)e->waves_per_cu;
+    out->grid_waves = e->grid_waves;
+    out->banks = e->nbanks;
+    out->bank_bytes = e->bank_words * 4;
+    out->output_pages = e->npages;
+    out->output_page_bytes = (uint32_t)(e->page_words * 4);
+    snprintf(out->device_name, sizeof out->device_name, "%s", e->name);
+    return FM_OK;
+}
+
+// Device plan of a query against a graph: dense label ids, symmetric
+// de-duplication, root list selection.
+static int make_dplan(const fm_graph *g, const HostQuery &q, bool all_roots, DPlan &dp) {
+    const HostGraph &h = g->h;
+    const Plan &p = q.plan;
+    std::memset(&dp, 0, sizeof dp);
+    dp.n = p.n;
+    uint32_t nr = 0;
+    for (uint32_t d = 0; d < p.n; d++) {
+        dp.order[d] = p.order[d];
+        dp.req_begin[d] = nr;
+        std::vector<uint32_t> anchored, self;
+        for (uint32_t i = p.req_begin[d]; i < p.req_begin[d + 1]; i++) {
+            const Req &r = p.reqs[i];
+            uint32_t lid = h.label_id(r.label);
+            if (lid == UINT32_MAX) lid = LABEL_ABSENT;
+            uint32_t dir = h.symmetric ? 0 : r.dir;
+            uint32_t packed = (r.anchor & 63u) | (dir << 7) | (lid << 8);
+            auto &dst = r.anchor == d ? self : anchored;
+            if (std::find(dst.begin(), dst.end(), packed) == dst.end()) dst.push_back(packed);
+        }
+        for (auto v : anchored) dp.req[nr++] = v;
+        for (auto v : self) dp.req[nr++] = v;
+        dp.nself[d] = (uint32_t)self.size();
+    }
+    dp.req_begin[p.n] = nr;
+    if (p.n == 1 || all_roots) {
+        dp.root_base = ALL_VERTICES;
+        dp.nroots = h.V;
+        return FM_OK;
+    }
+    // smallest root list among the depth-1 requirements (all anchored at 0)
+    uint64_t best = UINT64_MAX;
+    uint32_t base = 0;
+    uint64_t out_roots = h.out.roots.size();
+    for (uint32_t i = dp.req_begin[1]; i < dp.req_begin[1] + (dp.req_begin[2] - dp.req_begin[1] - dp.nself[1]); i++) {
+        uint32_t r = dp.req[i];
+        uint32_t lid = req_label(r), dir = req_dir(r);
+        uint64_t cnt = 0, off = 0;
+        if (lid != LABEL_ABSENT) {
+            const Csr &c = h.dir(dir);
+            cnt = c.root_ptr[lid + 1] - c.root_ptr[lid];
+            off = c.root_ptr[lid] + ((dir == 1 && !h.symmetric) ? out_roots : 0);
+        }
+        if (cnt < best) {
+            best = cnt;
+            base = (uint32_t)off;
+        }
+    }
+    dp.root_base = base;
+    dp.nroots = (uint32_t)(best == UINT64_MAX ? 0 : best);
+    return FM_OK;
+}
+
+int fm_match_batch(fm_engine *e, const fm_graph *g, const fm_query *const *qs, uint32_t nq,
+                   const fm_match_opts *opts, fm_result **out) {
+    auto t_start = std::chrono::steady_clock::now();
+    if (!e || !g || !out || (nq && !qs)) return fail(FM_ERR_INVALID_ARG, "null argument");
+    if (g->device != e->device) return fail(FM_ERR_INVALID_ARG, "graph is not uploaded to the engine's device");
+    if (nq == 0 || nq > e->max_queries) return fail(FM_ERR_INVALID_ARG, "query count %u out of range", nq);
+    fm_match_opts o{};
+    if (opts) o = *opts;
+    if (o.timeout_ms == 0) o.timeout_ms = 120000;
+    if (o.task_rows == 0) o.task_rows = SubMaxMulSpill;
+    if (o.root_chunk == 0) o.root_chunk = TooLargeThreshold;
+    HIP_TRY(hipSetDevice(e->device));
+    if (e->dirty) {
+        int rc = full_reset(e);
+        if (rc) return rc;
+    }
+    e->epoch++;
+    // ---- host-side plans and per-query state ----
+    std::vector<DPlan> plans(nq);
+    std::vector<DQuery> qh(nq);
+    uint64_t chunks = 0, pt_entries = 0;
+    uint32_t maxn = 1;
+    for (uint32_t i = 0; i < nq; i++) {
+        if (!qs[i]) return fail(FM_ERR_INVALID_ARG, "null query %u", i);
+        int rc = make_dplan(g, qs[i]->h, o.instrument != 0, plans[i]);
+        if (rc) return rc;
+        DQuery &Q = qh[i];
+        std::memset(&Q, 0, sizeof Q);
+        uint32_t n = plans[i].n;
+        maxn = std::max(maxn, n);
+        Q.page_rows = (uint32_t)(e->page_words / n);
+        uint64_t cap_rows = (uint64_t)e->npages * Q.page_rows;
+        Q.max_rows = (o.mode == FM_MODE_TABLE) ? (o.max_rows ? std::min<uint64_t>(o.max_rows, cap_rows) : cap_rows) : 0;
+        Q.max_pages = (o.mode == FM_MODE_TABLE) ? (uint32_t)((Q.max_rows + Q.page_rows - 1) / Q.page_rows) : 0;
+        Q.page_table_off = (uint32_t)pt_entries;
+        pt_entries += Q.max_pages;
+        Q.max_count = o.max_count;
+        Q.max_partials = o.max_partials;
+        Q.root_chunk_begin = (uint32_t)chunks;
+        Q.root_chunks = (plans[i].nroots + o.root_chunk - 1) / o.root_chunk;
+        chunks += Q.root_chunks;
+    }
+    if (chunks >= (1ull << 32)) return fail(FM_ERR_INVALID_ARG, "too many root chunks");
+    if (pt_entries >= (1ull << 32)) return fail(FM_ERR_INVALID_ARG, "page table too large");
+    // ---- device buffers ----
+    if (nq > e->q_cap) {
+        (void)hipFree(e->d_plans);
+        (void)hipFree(e->d_qs);
+        e->d_plans = nullptr;
+        e->d_qs = nullptr;
+        size_t cap = std::max<size_t>(nq, 64);
+        HIP_TRY(hipMalloc((void **)&e->d_plans, cap * sizeof(DPlan)));
+        HIP_TRY(hipMalloc((void **)&e->d_qs, cap * sizeof(DQuery)));
+        e->q_cap = cap;
+    }
+    if (pt_entries > e->page_table_cap) {
+        (void)hipFree(e->page_table);
+        e->page_table = nullptr;
+        size_t cap = std::max<size_t>(pt_entries, 1024);
+        HIP_TRY(hipMalloc((void **)&e->page_table, cap * 4));
+        e->page_table_cap = cap;
+    }
+    HIP_TRY(hipMemcpyAsync(e->d_plans, plans.data(), nq * sizeof(DPlan), hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipMemcpyAsync(e->d_qs, qh.data(), nq * sizeof(DQuery), hipMemcpyHostToDevice, e->stream));
+    if (pt_entries) HIP_TRY(hipMemsetAsync(e->page_table, 0, pt_entries * 4, e->stream));
+    Ctl reset{};
+    reset.pending = (long long)chunks;
+    reset.timeout_ticks = (unsigned long long)((double)o.timeout_ms * (e->wall_hz / 1000.0));
+    HIP_TRY(hipMemcpyAsync(e->ctl, &reset, CTL_RESET_BYTES, hipMemcpyHostToDevice, e->stream));
+    // ---- launch ----
+    KParams P{};
+    P.g.V = g->h.V;
+    P.g.L = g->h.L();
+    P.g.ptr[0] = g->d_ptr[0];
+    P.g.ptr[1] = g->d_ptr[1];
+    P.g.col = g->d_col[0];
+    P.g.roots = g->d_roots[0];
+    P.plans = e->d_plans;
+    P.qs = e->d_qs;
+    P.ctl = e->ctl;
+    P.ring = e->ring;
+    P.free_ring = e->free_ring;
+    P.bank_ref = e->bank_ref;
+    P.banks = e->banks;
+    P.pages = e->pages;
+    P.page_table = e->page_table;
+    P.bank_words = e->bank_words;
+    P.page_words = e->page_words;
+    P.nq = nq;
+    P.total_root_chunks = (uint32_t)chunks;
+    P.root_chunk = o.root_chunk;
+    P.task_rows = o.task_rows;
+    P.table = o.mode == FM_MODE_TABLE ? 1 : 0;
+    P.instrument = o.instrument ? 1 : 0;
+    P.qcap = e->qcap;
+    P.fcap = e->fcap;
+    P.nbanks = e->nbanks;
+    P.npages = e->npages;
+    P.nlevels = std::max<uint32_t>(1, maxn - 1);
+    P.starve_ticks = (unsigned long long)(e->wall_hz * 2.0);  // 2 s without a bank
+    uint32_t grid = o.grid_waves ? std::min(o.grid_waves, e->grid_waves) : e->grid_waves;
+    P.timeout_ticks = (unsigned long long)((double)o.timeout_ms * (e->wall_hz / 1000.0));
+    fm_launch_stats st{};
+    st.grid_waves = grid;
+    HIP_TRY(hipEventRecord(e->ev0, e->stream));
+    hipLaunchKernelGGL(fm_match_kernel, dim3(grid), dim3(WAVE), 0, e->stream, P);
+    hipError_t lerr = hipGetLastError();
+    HIP_TRY(hipEventRecord(e->ev1, e->stream));
+    std::vector<DQuery> back(nq);
+    Ctl ctl_back{};
+    HIP_TRY(hipMemcpyAsync(back.data(), e->d_qs, nq * sizeof(DQuery), hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipMemcpyAsync(&ctl_back, e->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, e->stream));
+    hipError_t serr = hipStreamSynchronize(e->stream);
+    if (lerr != hipSuccess || serr != hipSuccess) {
+        e->dirty = true;
+        return fail(FM_ERR_DEVICE, "match kernel failed: %s",
+                    hipGetErrorString(lerr != hipSuccess ? lerr : serr));
+    }
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+    auto *r = new fm_result();
+    r->e = e;
+    r->epoch = e->epoch;
+    r->qs = back;
+    r->n.resize(nq);
+    for (uint32_t i = 0; i < nq; i++) r->n[i] = plans[i].n;
+    st.kernel_ms = ms;
+    st.tasks_pushed = ctl_back.stats[ST_TASKS];
+    st.root_claims = ctl_back.stats[ST_ROOTS];
+    st.banks_allocated = ctl_back.stats[ST_BANKS_FRESH];
+    st.banks_reused = ctl_back.stats[ST_BANKS_REUSED];
+    st.alloc_failures = ctl_back.stats[ST_ALLOC_FAIL];
+    st.pages_allocated = ctl_back.stats[ST_PAGES];
+    st.status = ctl_back.error;
+    if (ctl_back.error) {
+        // an aborted launch leaves tasks and bank references behind
+        e->dirty = true;
+        for (auto &Q : r->qs) {
+            if (!Q.status) Q.status = ctl_back.error;
+            Q.flags |= FM_RESULT_ABORTED;
+        }
+    }
+    for (auto &Q : r->qs)
+        if (Q.status == FM_ERR_COUNT_LIMIT) Q.flags |= FM_RESULT_ABORTED;
+    st.total_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    r->stats = st;
+    *out = r;
+    if (ctl_back.error)
+        return fail((int)ctl_back.error, "match launch aborted with status %u", ctl_back.error);
+    return FM_OK;
+}
+
+int fm_match(fm_engine *e, const fm_graph *g, const fm_query *q, const fm_match_opts *opts, fm_result **out) {
+    const fm_query *qs[1] = {q};
+    if (!q) return fail(FM_ERR_INVALID_ARG, "null query");
+    return fm_match_batch(e, g, qs, 1, opts, out);
+}
+
+uint32_t fm_result_queries(const fm_result *r) { return r ? (uint32_t)r->qs.size() : 0; }
+
+static uint64_t stored_rows(const DQuery &Q) {
+    uint64_t rows = std::min<uint64_t>(Q.rows, Q.max_rows);
+    return rows;
+}
+
+int fm_result_query(const fm_result *r, uint32_t qi, fm_query_result *out) {
+    if (!r || !out || qi >= r->qs.size()) return fail(FM_ERR_INVALID_ARG, "bad result index");
+    const DQuery &Q = r->qs[qi];
+    std::memset(out, 0, sizeof *out);
+    out->count = Q.count;
+    out->rows = stored_rows(Q);
+    out->status = Q.status;
+    out->flags = Q.flags;
+    out->vertices = r->n[qi];
+    out->alg_bytes = Q.row_words * 4;
+    out->row_lookups = Q.lookups;
+    for (uint32_t d = 0; d < MAXN; d++) out->partials[d] = Q.partials[d];
+    return FM_OK;
+}
+
+int fm_result_stats(const fm_result *r, fm_launch_stats *out) {
+    if (!r || !out) return fail(FM_ERR_INVALID_ARG, "null argument");
+    *out = r->stats;
+    return FM_OK;
+}
+
+int fm_result_copy_table(const fm_result *r, uint32_t qi, uint32_t *dst, uint64_t max_rows, int sorted,
+                         uint64_t *copied) {
+    if (!r || qi >= r->qs.size()) return fail(FM_ERR_INVALID_ARG, "bad result index");
+    fm_engine *e = r->e;
+    if (r->epoch != e->epoch) return fail(FM_ERR_STALE_RESULT, "the engine has run again; tables are gone");
+    const DQuery &Q = r->qs[qi];
+    const uint32_t n = r->n[qi];
+    uint64_t rows = std::min<uint64_t>(stored_rows(Q), max_rows);
+    if (copied) *copied = 0;
+    if (rows == 0) return FM_OK;
+    if (!dst) return fail(FM_ERR_INVALID_ARG, "null destination");
+    HIP_TRY(hipSetDevice(e->device));
+    uint64_t npg = (rows + Q.page_rows - 1) / Q.page_rows;
+    std::vector<uint32_t> pt(npg);
+    HIP_TRY(hipMemcpy(pt.data(), e->page_table + Q.page_table_off, npg * 4, hipMemcpyDeviceToHost));
+    uint64_t done = 0;
+    for (uint64_t p = 0; p < npg && done < rows; p++) {
+        if (pt[p] < PAGE_BASE) break;  // a page that could not be allocated ends the table
+        uint64_t take = std::min<uint64_t>(Q.page_rows, rows - done);
+        HIP_TRY(hipMemcpy(dst + done * n, e->pages + (size_t)(pt[p] - PAGE_BASE) * e->page_words,
+                          take * n * 4, hipMemcpyDeviceToHost));
+        done += take;
+    }
+    if (sorted && done > 1) {
+        std::vector<uint64_t> idx(done);
+        std::iota(idx.begin(), idx.end(), 0);
+        std::sort(idx.begin(), idx.end(), [&](uint64_t a, uint64_t b) {
+            const uint32_t *x = dst + a * n, *y = dst + b * n;
+            for (uint32_t c = 0; c < n; c++)
+                if (x[c] != y[c]) return x[c] < y[c];
+            return false;
+        });
+        std::vector<uint32_t> tmp(done * n);
+        for (uint64_t i = 0; i < done; i++) std::memcpy(&tmp[i * n], dst + idx[i] * n, n * 4);
+        std::memcpy(dst, tmp.data(), done * n * 4);
+    }
+    if (copied) *copied = done;
+    return FM_OK;
+}
+
+void fm_result_destroy(fm_result *r) { delete r; }
+
+int fm_intersect_pairs(fm_engine *e, const uint32_t *data, const uint64_t *a_off, const uint32_t *a_len,
+                       const uint64_t *b_off, const uint32_t *b_len, uint32_t npairs, uint32_t *counts,
+                       const uint64_t *out_off, uint32_t *out, double *kernel_ms) {
+    if (!e || !data || !a_off || !a_len || !b_off || !b_len || !counts)
+        return fail(FM_ERR_INVALID_ARG, "null argument");
+    if (out_off && !out) return fail(FM_ERR_INVALID_ARG, "out_off without out");
+    if (npairs == 0) return FM_OK;
+    HIP_TRY(hipSetDevice(e->device));
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fm_intersect_kernel, WAVE, 0) != hipSuccess ||
+        blocks <= 0)
+        blocks = 8;
+    uint32_t grid = std::min<uint32_t>(npairs, (uint32_t)(blocks * e->cu_count));
+    HIP_TRY(hipEventRecord(e->ev0, e->stream));
+    hipLaunchKernelGGL(fm_intersect_kernel, dim3(grid), dim3(WAVE), 0, e->stream, data, a_off, a_len, b_off, b_len,
+                       npairs, counts, out_off, out);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(e->ev1, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+    if (kernel_ms) *kernel_ms = ms;
+    return FM_OK;
+}
+
+int fm_device_alloc(uint64_t bytes, void **out) {
+    if (!out) return fail(FM_ERR_INVALID_ARG, "null argument");
+    HIP_TRY(hipMalloc(out, std::max<uint64_t>(bytes, 4)));
+    return FM_OK;
+}
+
+int fm_device_free(void *p) {
+    HIP_TRY(hipFree(p));
+    return FM_OK;
+}
+
+int fm_copy_to_device(void *dst, const void *src, uint64_t bytes) {
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return FM_OK;
+}
+
+int fm_copy_to_host(void *dst, const void *src, uint64_t bytes) {
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return FM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,104 @@
+// Internal declarations shared by the host-side C++ (graph/query/plan/io)
+// and the HIP engine. Nothing here crosses the C ABI.
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "environment.h"
+#include "forkmatch.h"
+
+namespace fm {
+
+// Records a failure for fm_last_error() and returns `code`.
+int fail(int code, const char *fmt, ...);
+void clear_error();
+
+// One label-partitioned adjacency direction. Row (l, v) is
+// col[ptr[l*V + v] .. ptr[l*V + v + 1]), destinations ascending. roots holds,
+// per label, the ascending list of vertices whose row is non-empty
+// (roots[root_ptr[l] .. root_ptr[l+1])): the candidate roots of a query
+// whose first requirement uses label l.
+struct Csr {
+    std::vector<uint32_t> ptr;
+    std::vector<uint32_t> col;
+    std::vector<uint32_t> root_ptr;
+    std::vector<uint32_t> roots;
+};
+
+struct HostGraph {
+    uint32_t V = 0;
+    uint32_t header_edges = 0;
+    std::vector<uint32_t> label_values;  // dense label id -> label value
+    Csr out;                             // rows by source
+    Csr in;                              // rows by destination (empty if symmetric)
+    bool symmetric = false;
+    uint32_t L() const { return (uint32_t)label_values.size(); }
+    uint64_t E() const { return out.col.size(); }
+    // dense id of a label value, or UINT32_MAX when the graph has no edge
+    // with that label
+    uint32_t label_id(uint32_t value) const;
+    const Csr &dir(int d) const { return (d == 0 || symmetric) ? out : in; }
+};
+
+// Builds a HostGraph from directed (src, dst, label) lines. Duplicate lines
+// are collapsed. Returns FM_OK or a status.
+int build_graph(uint32_t V, uint64_t n, const uint32_t *src, const uint32_t *dst,
+                const uint32_t *label, HostGraph &g);
+
+struct Req {
+    uint32_t anchor;  // depth of the earlier query vertex (== own depth: self-loop)
+    uint32_t label;   // label value
+    uint32_t dir;     // 0: candidate in anchor's out-row, 1: in anchor's in-row
+};
+
+struct Plan {
+    uint32_t n = 0;
+    std::vector<uint32_t> order;      // depth -> query vertex
+    std::vector<uint32_t> depth_of;   // query vertex -> depth
+    std::vector<uint32_t> req_begin;  // n + 1
+    std::vector<Req> reqs;
+};
+
+struct HostQuery {
+    uint32_t n = 0;
+    uint32_t header_edges = 0;
+    std::vector<std::array<uint32_t, 3>> edges;  // unique (src, dst, label) lines
+    Plan plan;
+};
+
+int build_query(uint32_t n, uint32_t m, const uint32_t *src, const uint32_t *dst,
+                const uint32_t *label, HostQuery &q);
+int make_plan(const HostQuery &q, Plan &p);
+
+// Reads a whole file. Returns false if it cannot be opened.
+bool read_file(const std::string &path, std::string &buf);
+
+// Deterministic generator: xoshiro256** seeded through splitmix64.
+struct Rng {
+    uint64_t s[4];
+    explicit Rng(uint64_t seed);
+    uint64_t next();
+    double uniform();                     // [0, 1)
+    uint64_t below(uint64_t n);           // [0, n)
+};
+
+}  // namespace fm
+
+// Opaque handle bodies (declared in forkmatch.h).
+struct fm_graph {
+    fm::HostGraph h;
+    int device = -1;  // device holding the upload, -1 if host only
+    // device CSR (dir 1 aliases dir 0 when the graph is symmetric)
+    uint32_t *d_ptr[2] = {nullptr, nullptr};
+    uint32_t *d_col[2] = {nullptr, nullptr};
+    uint32_t *d_root_ptr[2] = {nullptr, nullptr};
+    uint32_t *d_roots[2] = {nullptr, nullptr};
+    uint64_t device_bytes = 0;
+};
+
+struct fm_query {
+    fm::HostQuery h;
+};

@@ -1,0 +1,59 @@
+/*
+ * Build-time defaults for the ForkMatch engine knobs.
+ *
+ * The names are the reference's compile-time knobs
+ * (src/environment.cuh:4-20, README.md:37-49) so that a user who tuned them
+ * there finds them here. Unlike the reference they are only defaults: every
+ * one of them can be overridden at run time through fm_engine_config
+ * (include/forkmatch.h) or, for the CLI, through an environment variable of
+ * the same name (e.g. `CurBankSize=131072 ForkMatch q.g d.g`).
+ */
+#pragma once
+
+/* Total device memory the CLI reserves for one match (src/environment.cuh:4).
+ * Here it is the sum of the two arenas below plus the graph; it is not one
+ * slab, each arena is allocated on its own. */
+#define PREALLOC 45600000000ull
+
+/* Bytes of the partial-embedding bank arena (src/environment.cuh:5). Banks of
+ * CurBankSize * 8 bytes are carved from it by a device bump allocator and
+ * recycled through a device free list. */
+#define SOLN_PREALLOC 24000000000ull
+
+/* Bytes of the output-table arena (src/environment.cuh:6). Tables are stored
+ * in lazily allocated pages of OUTPUT_PAGE_BYTES. */
+#define OUTPUT_PREALLOC 17300000000ull
+
+/* Size of one memory bank in 8-byte units, the width of the reference's
+ * FuncPair node (src/environment.cuh:19). The reference default is 1 << 20
+ * (8 MiB banks). Here a wave yields its task whenever its output bank
+ * fills, so the bank is also the unit of depth-first progress; the default
+ * is 1 << 15 (256 KiB banks, 96k banks in SOLN_PREALLOC). The minimum is
+ * 1024 (one 64-row step of 32-vertex rows). */
+#define CurBankSize (1u << 15)
+
+/* Threads per workgroup of the reference's main kernel
+ * (src/environment.cuh:11). The MI355X engine runs one wave64 per
+ * workgroup; the name is kept for the launch-shape report only. */
+#define CurBlockSize 64
+
+/* Maximum parents carried by one queued task. This replaces the reference's
+ * soft fork threshold (outputMax - BlockSize * SubMaxMulSpill,
+ * src/smsm.cu:451-454): a bank segment larger than this is split into
+ * several tasks so idle waves can take part of it. */
+#define SubMaxMulSpill 1024
+
+/* Number of root vertices claimed by one wave at a time (the reference gives
+ * each block a fixed root range, src/smsm.cu:803-811). */
+#define TooLargeThreshold 1024
+
+/* Bytes of one output-table page. */
+#define OUTPUT_PAGE_BYTES (1u << 20)
+
+/* Limits of the device plan. The reference supports queries of at most 19
+ * vertices and 3 backward edges per vertex (src/ccsr.cuh:381-385,
+ * src/smsm.cu:887-899); here the limits are 32 vertices and 16 backward
+ * edges per vertex. */
+#define FM_MAX_QUERY_VERTICES 32
+#define FM_MAX_REQS_PER_DEPTH 16
+#define FM_MAX_REQS_TOTAL 256

@@ -1,0 +1,310 @@
+/*
+ * ForkMatch for MI355X: the C ABI of libforkmatch.so.
+ *
+ * The reference exposes no library API: its surface is the CLI
+ * `ForkMatch <query.g> <data.g>` (src/smsm.cu:1351-1385), the `.g` grammar
+ * (README.md:7-25) and the knobs of src/environment.cuh. Its internal
+ * host->device boundary is one kernel launch,
+ *   initPosting<256,sym><<<G,256>>>(MemoryPool, FuncPair* forest,
+ *       GPUGraph* data, Requirements reqs, unsigned* output)
+ * (src/smsm.cu:1305-1326), preceded by the host loaders parseGraph
+ * (src/trie.cuh:1051-1222), fileParse (src/hostparser.cuh:286-509) and
+ * preProcessQuery (src/ccsr.cuh:485-728). Each entry point below names the
+ * reference function it replaces. Plain pointers and sizes only: no C++ or
+ * torch types cross this boundary.
+ *
+ * Error convention: every function returns an int status (FM_OK = 0). The
+ * reference calls exit(code); the numeric codes of its parse errors are kept
+ * so the CLI can exit with the same values. fm_last_error() returns a text
+ * description of the last failure on the calling thread.
+ *
+ * Threading: one host thread per fm_engine. Graphs are immutable after
+ * construction and may be shared by several engines.
+ */
+#ifndef FORKMATCH_H
+#define FORKMATCH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+enum fm_status {
+    FM_OK = 0,
+    /* Disconnected query, plan check failure, arena overflow
+     * (src/ccsr.cuh:258, src/ccsr.cuh:719-725, src/trie.cuh:1044). */
+    FM_ERR_GENERIC = 1,
+    /* Data graph file missing (src/trie.cuh:1057-1062). */
+    FM_ERR_DATA_MISSING_FILE = 12,
+    /* Data graph does not start with "t # 0\n" (src/trie.cuh:1075-1078). */
+    FM_ERR_DATA_BAD_START = 76,
+    /* Data graph does not end with "t # -1" (src/trie.cuh:1080-1083). */
+    FM_ERR_DATA_BAD_END = 77,
+    /* Malformed data graph header (src/trie.cuh:1088-1092). */
+    FM_ERR_DATA_BAD_HEADER = 78,
+    /* Malformed data graph body line or vertex id out of range. The
+     * reference traps on the device here (src/trie.cuh:845-856). */
+    FM_ERR_DATA_BAD_BODY = 79,
+    /* Wrong argument count on the CLI (src/smsm.cu:1378-1385). */
+    FM_ERR_ARGV = 99,
+    /* Query file missing / invalid / short (src/hostparser.cuh:15-17). */
+    FM_ERR_QUERY_MISSING_FILE = 232,
+    FM_ERR_QUERY_INVALID = 233,
+    FM_ERR_QUERY_MISSING_DATA = 234,
+    /* Statuses with no reference counterpart (the reference __trap()s). */
+    FM_ERR_INVALID_ARG = 300,
+    FM_ERR_DEVICE = 301,          /* HIP runtime error */
+    FM_ERR_OUT_OF_MEMORY = 302,   /* hipMalloc failed */
+    FM_ERR_POOL_EXHAUSTED = 303,  /* no bank could be allocated for too long */
+    FM_ERR_QUEUE_FULL = 304,      /* a device work queue overflowed */
+    FM_ERR_TIMEOUT = 305,         /* the launch passed its deadline */
+    FM_ERR_COUNT_LIMIT = 306,     /* a query passed opts.max_count */
+    FM_ERR_QUERY_TOO_LARGE = 307, /* query beyond the device plan limits */
+    FM_ERR_STALE_RESULT = 308,    /* the engine ran again; tables are gone */
+    FM_ERR_NOT_SUPPORTED = 309
+};
+
+/* Query size limit of the device plan (environment.h FM_MAX_QUERY_VERTICES). */
+#define FM_MAX_QUERY_VERTICES_ABI 32
+
+/* Per-query result flags (fm_query_result.flags). */
+#define FM_RESULT_TRUNCATED 1u   /* table capacity reached; count is exact */
+#define FM_RESULT_ABORTED 2u     /* the launch aborted; count is partial */
+
+/* ---- opaque handles --------------------------------------------------- */
+typedef struct fm_graph fm_graph;   /* device-resident data graph */
+typedef struct fm_query fm_query;   /* parsed query + matching plan */
+typedef struct fm_engine fm_engine; /* device context: pools, queues, stream */
+typedef struct fm_result fm_result; /* results of one match launch */
+
+/* ---- library ----------------------------------------------------------- */
+const char *fm_last_error(void);
+const char *fm_version(void);
+/* Number of visible HIP devices (0 without a GPU; never fails). */
+int fm_device_count(void);
+
+/* ---- data graph ---------------------------------------------------------
+ * Replaces parseGraph + parseCSR + csrToTrie (src/trie.cuh:800-951,
+ * 1051-1222): the data graph becomes a label-partitioned CSR in HBM.
+ * Duplicate edge lines are collapsed; vertex labels are read and ignored
+ * (the reference stores them write-only, src/trie.cuh:872). */
+typedef struct fm_graph_info {
+    uint32_t vertices;      /* V */
+    uint32_t labels;        /* distinct edge labels L */
+    uint64_t edges;         /* directed edges after de-duplication */
+    uint32_t symmetric;     /* 1 if every (u,v,l) has its (v,u,l) */
+    uint32_t header_edges;  /* edge count stated in the .g header */
+    uint64_t device_bytes;  /* HBM held by the graph */
+} fm_graph_info;
+
+/* Host-side graph (parse + CSR build only, no device). Used by tools that
+ * convert or inspect graphs, and by the tests on machines without a GPU. */
+int fm_graph_parse_g(const char *path, fm_graph **out);
+int fm_graph_from_edges(uint32_t vertices, uint64_t nedges,
+                        const uint32_t *src, const uint32_t *dst,
+                        const uint32_t *label, fm_graph **out);
+/* Uploads the CSR to the current HIP device (idempotent). */
+int fm_graph_upload(fm_graph *g);
+/* Parse + upload in one call (the reference's parseGraph). */
+int fm_graph_load_g(const char *path, fm_graph **out);
+int fm_graph_info_get(const fm_graph *g, fm_graph_info *out);
+/* Copies the host CSR. ptr has labels*vertices+1 entries (row (l,v) is
+ * col[ptr[l*V+v] .. ptr[l*V+v+1])); col has edges entries. label_values
+ * maps dense label id -> label value as written in the file. Any pointer
+ * may be NULL. dir 0 = out-rows, 1 = in-rows. */
+int fm_graph_copy_csr(const fm_graph *g, int dir, uint32_t *ptr,
+                      uint32_t *col, uint32_t *label_values);
+/* Copies the device CSR back (parity check of the upload). */
+int fm_graph_copy_device_csr(const fm_graph *g, int dir, uint32_t *ptr,
+                             uint32_t *col);
+/* Binary graph cache: the .g text of the large configurations exceeds the
+ * reference's 2 GiB int limit (src/trie.cuh:1115). */
+int fm_graph_save_bin(const fm_graph *g, const char *path);
+int fm_graph_load_bin(const char *path, fm_graph **out);
+/* Writes the graph in the reference's .g grammar (paired edge lines as
+ * stored; the file ends with "t # -1" and no newline, src/trie.cuh:1080). */
+int fm_graph_write_g(const fm_graph *g, const char *path);
+void fm_graph_destroy(fm_graph *g);
+
+/* ---- synthetic data graphs (seeded, deterministic) ----------------------
+ * Undirected simple graphs written as paired directed edges with the same
+ * label on both lines. label_dist: 0 = uniform over [0, nlabels),
+ * 1 = Zipf P(k) ~ (k+1)^-zipf_s over [0, nlabels). Vertex labels uniform
+ * over [0, vlabels). */
+int fm_gen_erdos_renyi(uint32_t vertices, uint64_t undirected_edges,
+                       uint32_t nlabels, int label_dist, double zipf_s,
+                       uint32_t vlabels, uint64_t seed, fm_graph **out);
+int fm_gen_rmat(uint32_t scale, uint64_t undirected_edges, double a, double b,
+                double c, uint32_t nlabels, int label_dist, double zipf_s,
+                uint32_t vlabels, uint64_t seed, fm_graph **out);
+
+/* Query extraction by seeded random walk on a symmetric data graph (the
+ * walk crosses edges of every label). shape 0: tree (the edge that first
+ * reached each vertex); 1: every traversed edge, with at least min_cycles
+ * independent cycles; 2: the whole induced subgraph of the walked vertices.
+ * Edge labels are copied from the data, so the query has at least one
+ * embedding. */
+int fm_gen_query_walk(const fm_graph *g, uint32_t vertices, int shape,
+                      uint32_t min_cycles, uint64_t seed, fm_query **out);
+
+/* ---- queries ------------------------------------------------------------
+ * Replaces fileParse (src/hostparser.cuh:286-509) and preProcessQuery
+ * (src/ccsr.cuh:485-728). Query edge lines are directed; an embedding m
+ * must map every line (u, w, l) to a data edge (m(u), m(w)) of label l,
+ * with m injective. Vertex labels are ignored. */
+int fm_query_load_g(const char *path, fm_query **out);
+int fm_query_from_edges(uint32_t vertices, uint32_t nedges,
+                        const uint32_t *src, const uint32_t *dst,
+                        const uint32_t *label, fm_query **out);
+int fm_query_write_g(const fm_query *q, const char *path);
+int fm_query_info(const fm_query *q, uint32_t *vertices, uint32_t *nedges);
+/* Copies the unique edge lines (nedges entries each). */
+int fm_query_edges(const fm_query *q, uint32_t *src, uint32_t *dst,
+                   uint32_t *label);
+/* Matching plan, following the reference's order (src/ccsr.cuh:229-249):
+ * vertices ranked by (out-degree desc, id asc); depth 0 = rank 0; then the
+ * lowest-ranked unmapped neighbour of the mapped set.
+ * order[d] = query vertex matched at depth d. For each depth d,
+ * req_begin[d] .. req_begin[d+1] index the requirements; requirement i is
+ * (req_anchor[i] = depth of the earlier vertex, or d itself for a
+ * self-loop; req_label[i] = label value; req_dir[i] = 0 if the candidate
+ * must be in the anchor's out-row (query line anchor -> u), 1 if in its
+ * in-row (query line u -> anchor)). Arrays must hold vertices entries
+ * (order), vertices + 1 (req_begin) and nreqs entries. */
+typedef struct fm_plan_info {
+    uint32_t vertices;
+    uint32_t nreqs;
+    uint32_t max_reqs_per_depth;
+} fm_plan_info;
+int fm_query_plan(const fm_query *q, fm_plan_info *info, uint32_t *order,
+                  uint32_t *req_begin, uint32_t *req_anchor,
+                  uint32_t *req_label, uint32_t *req_dir);
+void fm_query_destroy(fm_query *q);
+
+/* ---- engine -------------------------------------------------------------
+ * Replaces prealloc()/populatePool and the MemoryPool/forest of
+ * src/tbs.cuh:10-136 and src/smsm.cu:1155-1180,1230-1262. Zero fields take
+ * the defaults of include/environment.h. */
+typedef struct fm_engine_config {
+    uint64_t soln_prealloc;   /* bank arena bytes (SOLN_PREALLOC) */
+    uint64_t output_prealloc; /* table arena bytes (OUTPUT_PREALLOC) */
+    uint32_t bank_size;       /* bank size in 8-byte units (CurBankSize) */
+    uint32_t output_page_bytes;
+    uint32_t queue_capacity;  /* task slots per priority level (pow2) */
+    uint32_t max_queries;     /* queries per launch */
+    int32_t device;           /* HIP device id (default 0) */
+    uint32_t reserved[7];
+} fm_engine_config;
+int fm_engine_create(const fm_engine_config *cfg, fm_engine **out);
+void fm_engine_destroy(fm_engine *e);
+/* Device and launch-shape facts: compute units, resident waves of the
+ * matcher kernel, arena sizes. */
+typedef struct fm_engine_info {
+    uint32_t compute_units;
+    uint32_t waves_per_cu;
+    uint32_t grid_waves;
+    uint32_t banks;
+    uint64_t bank_bytes;
+    uint32_t output_pages;
+    uint32_t output_page_bytes;
+    char device_name[64];
+} fm_engine_info;
+int fm_engine_info_get(fm_engine *e, fm_engine_info *out);
+
+/* ---- matching -------------------------------------------------------------
+ * Replaces match() + initPosting + the dynamic-parallelism fork chain
+ * (src/smsm.cu:784-1139,1182-1349): one persistent kernel launch whose
+ * waves pull partial-embedding tasks from device work queues. */
+#define FM_MODE_COUNT 0u  /* count only; the last level is never stored */
+#define FM_MODE_TABLE 1u  /* count and store every embedding */
+
+typedef struct fm_match_opts {
+    uint32_t mode;            /* FM_MODE_COUNT or FM_MODE_TABLE */
+    uint32_t instrument;      /* 1: gather algorithmic-byte counters */
+    uint64_t max_rows;        /* table capacity per query (0 = arena) */
+    uint64_t max_count;       /* abort a query past this count (0 = none) */
+    uint32_t timeout_ms;      /* launch deadline (0 = 120000) */
+    uint32_t grid_waves;      /* persistent waves (0 = full residency) */
+    uint32_t task_rows;       /* parents per queued task (SubMaxMulSpill) */
+    uint32_t root_chunk;      /* roots per claim (TooLargeThreshold) */
+    uint32_t reserved0;
+    /* abort a query once it has stored more than this many intermediate
+     * partial embeddings (0 = none); its status becomes COUNT_LIMIT */
+    uint64_t max_partials;
+    uint32_t reserved[4];
+} fm_match_opts;
+
+/* One query in one launch (the reference's match()). */
+int fm_match(fm_engine *e, const fm_graph *g, const fm_query *q,
+             const fm_match_opts *opts, fm_result **out);
+/* nq queries in one persistent launch (no reference counterpart: the
+ * reference matches one query per process). */
+int fm_match_batch(fm_engine *e, const fm_graph *g, const fm_query *const *qs,
+                   uint32_t nq, const fm_match_opts *opts, fm_result **out);
+
+typedef struct fm_query_result {
+    uint64_t count;       /* number of embeddings (exact unless ABORTED) */
+    uint64_t rows;        /* rows held in the table (<= count) */
+    uint32_t status;      /* fm_status of this query */
+    uint32_t flags;       /* FM_RESULT_* */
+    uint32_t vertices;    /* row width */
+    uint32_t reserved;
+    /* Instrumented launches only: algorithmic bytes (neighbour words read
+     * once per requirement, 8-byte partial records, the final table) and
+     * the number of row lookups. */
+    uint64_t alg_bytes;
+    uint64_t row_lookups;
+    uint64_t partials[FM_MAX_QUERY_VERTICES_ABI];
+} fm_query_result;
+
+typedef struct fm_launch_stats {
+    double kernel_ms;        /* persistent kernel, HIP events on its stream */
+    double total_ms;         /* host wall time of the call */
+    uint64_t tasks_pushed;   /* queued tasks (forks) */
+    uint64_t root_claims;
+    uint64_t banks_allocated;/* fresh banks from the bump allocator */
+    uint64_t banks_reused;   /* banks taken from the free list */
+    uint64_t alloc_failures; /* requeues after a failed bank allocation */
+    uint64_t pages_allocated;
+    uint32_t grid_waves;
+    uint32_t status;
+} fm_launch_stats;
+
+uint32_t fm_result_queries(const fm_result *r);
+int fm_result_query(const fm_result *r, uint32_t qi, fm_query_result *out);
+int fm_result_stats(const fm_result *r, fm_launch_stats *out);
+/* Copies up to max_rows rows (row-major, `vertices` u32 columns in query
+ * vertex id order) of query qi into dst. Returns the rows copied through
+ * *copied. The order of rows is unspecified; sorted=1 sorts them
+ * lexicographically on the host. */
+int fm_result_copy_table(const fm_result *r, uint32_t qi, uint32_t *dst,
+                         uint64_t max_rows, int sorted, uint64_t *copied);
+void fm_result_destroy(fm_result *r);
+
+/* ---- intersection microkernel -------------------------------------------
+ * Replaces the warp-cooperative intersect<CoopSize,Decomp> family
+ * (src/intersect.cuh:31-191). pairs of sorted u32 lists inside one device
+ * array `data` (device pointers): list A_i = data[a_off[i] .. +a_len[i]],
+ * B_i likewise. counts[i] = |A_i n B_i|. If out_off is not NULL, the
+ * intersection of pair i is written ascending at out[out_off[i] ..]. */
+int fm_intersect_pairs(fm_engine *e, const uint32_t *data,
+                       const uint64_t *a_off, const uint32_t *a_len,
+                       const uint64_t *b_off, const uint32_t *b_len,
+                       uint32_t npairs, uint32_t *counts,
+                       const uint64_t *out_off, uint32_t *out,
+                       double *kernel_ms);
+
+/* Device buffers for the microkernel and its tests (hipMalloc/hipMemcpy). */
+int fm_device_alloc(uint64_t bytes, void **out);
+int fm_device_free(void *p);
+int fm_copy_to_device(void *dst, const void *src, uint64_t bytes);
+int fm_copy_to_host(void *dst, const void *src, uint64_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FORKMATCH_H */

@@ -1,0 +1,260 @@
+"""GPU parity: the HIP engine (through libforkmatch.so) against the CPU oracle.
+
+Bit-exact bar: the same count and the same lexicographically sorted
+embedding table (columns in query vertex order) on the same inputs.
+"""
+import numpy as np
+import pytest
+
+import forkmatch_amd as fm
+import oracle as orc
+from conftest import FIXTURE_NAMES, fixture_path
+
+pytestmark = pytest.mark.gpu
+
+TIMEOUT_MS = 30000
+
+
+def _oracle_graph(g):
+    s, d, l = g.edge_list()
+    return orc.OracleGraph(g.vertices, s, d, l)
+
+
+def _check(engine, g, og, q, label=""):
+    n = q.vertices
+    edges = q.edges()
+    ref = orc.match(og, n, edges)
+    r = engine.match(g, q, timeout_ms=TIMEOUT_MS)
+    out = r.query(0)
+    assert out.status == 0, (label, out)
+    assert out.count == ref.count, (label, out.count, ref.count)
+    assert out.rows == ref.count, (label, out.rows)
+    tab = r.table(0)
+    assert tab.shape == ref.table.shape, label
+    assert np.array_equal(tab, ref.table), label
+    return ref.count
+
+
+@pytest.fixture(scope="module")
+def er(er_graph):
+    er_graph.upload()
+    return er_graph, _oracle_graph(er_graph)
+
+
+def test_device_csr_matches_host(er):
+    g, _ = er
+    for direction in (0, 1):
+        hp, hc, _ = g.csr(direction)
+        dp, dc, _ = g.csr(direction, device=True)
+        assert np.array_equal(hp, dp)
+        assert np.array_equal(hc, dc)
+
+
+@pytest.mark.parametrize("name", FIXTURE_NAMES)
+def test_reference_fixture_queries(engine, er, name):
+    g, og = er
+    q = fm.Query.load_g(str(fixture_path(name)))
+    _check(engine, g, og, q, name)
+
+
+@pytest.mark.parametrize("name", FIXTURE_NAMES)
+def test_fixture_queries_single_label(engine, name):
+    # every data edge labelled 1, like the fixtures: counts are the closed
+    # forms of the unlabelled queries
+    g = fm.Graph.erdos_renyi(300, 1500, 1, seed=7)
+    s, d, _ = g.edge_list()
+    g = fm.Graph.from_edges(300, s, d, np.ones_like(s)).upload()
+    og = orc.OracleGraph(300, s, d, np.ones_like(s))
+    q = fm.Query.load_g(str(fixture_path(name)))
+    _check(engine, g, og, q, name)
+
+
+def test_random_walk_queries(engine, er):
+    g, og = er
+    total = 0
+    for seed in range(24):
+        n = 3 + seed % 5
+        shape = seed % 3
+        q = fm.Query.random_walk(g, n, shape=shape, min_cycles=1 if shape == 1 else 0, seed=seed)
+        total += _check(engine, g, og, q, f"walk seed={seed} n={n} shape={shape}")
+    assert total > 0
+
+
+def test_rmat_zipf_queries(engine):
+    # config-2 shape at scale 12: R-MAT, Zipf(1.2) labels over 0..86
+    g = fm.Graph.rmat(12, 16 * 4096, 87, zipf_s=1.2, seed=0xF0F2).upload()
+    og = _oracle_graph(g)
+    for seed in range(8):
+        n = 4 + seed % 3
+        q = fm.Query.random_walk(g, n, shape=seed % 3, min_cycles=2 if seed % 3 == 1 else 0, seed=100 + seed)
+        ref = orc.match(og, n, q.edges(), table=False)
+        if ref.count > 2_000_000:
+            continue
+        _check(engine, g, og, q, f"rmat seed={seed}")
+
+
+def test_count_mode_equals_table_mode(engine, er):
+    g, og = er
+    q = fm.Query.load_g(str(fixture_path("_4_square")))
+    a = engine.match(g, q, table=False, timeout_ms=TIMEOUT_MS).query(0)
+    b = engine.match(g, q, table=True, timeout_ms=TIMEOUT_MS).query(0)
+    assert a.count == b.count
+    assert a.rows == 0
+
+
+def test_batch_equals_alone(engine, er):
+    g, _ = er
+    qs = [fm.Query.random_walk(g, 3 + i % 5, shape=i % 3, min_cycles=1 if i % 3 == 1 else 0, seed=500 + i)
+          for i in range(16)]
+    alone = []
+    for q in qs:
+        r = engine.match(g, q, timeout_ms=TIMEOUT_MS)
+        alone.append((r.count(0), r.table(0)))
+    rb = engine.match_batch(g, qs, timeout_ms=TIMEOUT_MS)
+    for i, q in enumerate(qs):
+        assert rb.query(i).status == 0
+        assert rb.count(i) == alone[i][0]
+        assert np.array_equal(rb.table(i), alone[i][1])
+
+
+def test_directed_graph(engine):
+    rng = np.random.default_rng(3)
+    V = 200
+    s = rng.integers(0, V, 1500, dtype=np.uint32)
+    d = rng.integers(0, V, 1500, dtype=np.uint32)
+    keep = s != d
+    s, d = s[keep], d[keep]
+    lab = rng.integers(0, 3, len(s), dtype=np.uint32)
+    g = fm.Graph.from_edges(V, s, d, lab).upload()
+    assert not g.info().symmetric
+    og = orc.OracleGraph(V, s, d, lab)
+    # directed path, directed triangle, 2-cycle with distinct labels, fan-in
+    queries = [
+        (3, [(0, 1, 0), (1, 2, 1)]),
+        (3, [(0, 1, 0), (1, 2, 0), (2, 0, 0)]),
+        (2, [(0, 1, 1), (1, 0, 2)]),
+        (4, [(1, 0, 0), (2, 0, 1), (3, 0, 2)]),
+        (4, [(0, 1, 0), (0, 2, 1), (1, 3, 2), (2, 3, 0)]),
+    ]
+    for n, e in queries:
+        _check(engine, g, og, fm.Query.from_edges(n, e), str(e))
+
+
+def test_absent_label_has_no_match(engine, er):
+    g, _ = er
+    q = fm.Query.from_edges(2, [(0, 1, 1000), (1, 0, 1000)])
+    r = engine.match(g, q, timeout_ms=TIMEOUT_MS)
+    assert r.count(0) == 0 and r.query(0).status == 0
+
+
+def test_single_vertex_query(engine, er):
+    g, og = er
+    q = fm.Query.from_edges(1, [])
+    r = engine.match(g, q, timeout_ms=TIMEOUT_MS)
+    assert r.count(0) == g.vertices
+    assert np.array_equal(r.table(0)[:, 0], np.arange(g.vertices, dtype=np.uint32))
+
+
+def test_self_loops(engine):
+    rng = np.random.default_rng(5)
+    V = 100
+    s = rng.integers(0, V, 600, dtype=np.uint32)
+    d = rng.integers(0, V, 600, dtype=np.uint32)
+    loops = rng.integers(0, V, 30, dtype=np.uint32)
+    s = np.concatenate([s, d, loops])
+    d = np.concatenate([d, s[:600], loops])
+    lab = np.zeros(len(s), np.uint32)
+    g = fm.Graph.from_edges(V, s, d, lab).upload()
+    og = orc.OracleGraph(V, s, d, lab)
+    for n, e in [(1, [(0, 0, 0)]), (2, [(0, 1, 0), (1, 0, 0), (0, 0, 0)]),
+                 (3, [(0, 1, 0), (1, 0, 0), (1, 2, 0), (2, 1, 0), (2, 2, 0)])]:
+        _check(engine, g, og, fm.Query.from_edges(n, e), str(e))
+
+
+def test_instrumented_counters_match_oracle(engine, er):
+    g, og = er
+    for name in ["_3_triangle", "_4_square", "_5_road5"]:
+        q = fm.Query.load_g(str(fixture_path(name)))
+        ref = orc.match(og, q.vertices, q.edges(), table=False, instrument=True)
+        out = engine.match(g, q, table=False, instrument=True, timeout_ms=TIMEOUT_MS).query(0)
+        assert out.count == ref.count
+        assert out.alg_row_bytes == 4 * ref.row_words, name
+        assert out.partials == ref.partials, name
+
+
+def test_table_truncation_keeps_exact_count(engine, er):
+    g, og = er
+    q = fm.Query.load_g(str(fixture_path("_4_roadY")))
+    ref = orc.match(og, q.vertices, q.edges())
+    assert ref.count > 100
+    r = engine.match(g, q, max_rows=100, timeout_ms=TIMEOUT_MS)
+    out = r.query(0)
+    assert out.count == ref.count
+    assert out.rows == 100 and out.truncated
+    tab = r.table(0, sorted=True)
+    # every stored row is a real embedding
+    ref_rows = {tuple(x) for x in ref.table.tolist()}
+    assert all(tuple(x) in ref_rows for x in tab.tolist())
+    assert len({tuple(x) for x in tab.tolist()}) == 100
+
+
+def test_count_limit_stops_query(engine, er):
+    g, _ = er
+    q = fm.Query.load_g(str(fixture_path("_5_plus")))
+    r = engine.match(g, q, table=False, max_count=10, timeout_ms=TIMEOUT_MS, allow_errors=True)
+    out = r.query(0)
+    assert out.status == 306  # FM_ERR_COUNT_LIMIT
+    assert out.count > 10
+
+
+def test_small_bank_pool_recycles_banks(er):
+    g, og = er
+    # 64 banks of 8 KiB: the frontier must be recycled through the free list
+    with fm.Engine(soln_prealloc=64 * 8192, output_prealloc=1 << 26, bank_size=1024,
+                   output_page_bytes=1 << 16) as e:
+        for name in ["_4_road4", "_5_road5", "_4_square"]:
+            q = fm.Query.load_g(str(fixture_path(name)))
+            ref = orc.match(og, q.vertices, q.edges())
+            r = e.match(g, q, timeout_ms=TIMEOUT_MS, grid_waves=16, task_rows=32)
+            st = r.stats()
+            assert r.count(0) == ref.count, name
+            assert np.array_equal(r.table(0), ref.table), name
+            assert st.banks_reused > 0 or st.banks_allocated <= 64
+
+
+def test_exhausted_bank_pool_fails_cleanly():
+    # one 8 KiB bank: the depth-1 rows of a 5-path on 8000 same-label edges
+    # fill it, and expanding them needs a second bank that never comes
+    g0 = fm.Graph.erdos_renyi(1000, 4000, 1, seed=9)
+    s, d, _ = g0.edge_list()
+    g = fm.Graph.from_edges(1000, s, d, np.ones_like(s)).upload()
+    q = fm.Query.load_g(str(fixture_path("_5_road5")))
+    with fm.Engine(soln_prealloc=8192, output_prealloc=1 << 24, bank_size=1024, output_page_bytes=1 << 16) as e:
+        r = e.match(g, q, table=False, timeout_ms=TIMEOUT_MS, grid_waves=4, allow_errors=True)
+        assert r.query(0).status == 303  # FM_ERR_POOL_EXHAUSTED
+        # the engine recovers for the next launch
+        q2 = fm.Query.load_g(str(fixture_path("_2_road2")))
+        assert e.match(g, q2, table=False, timeout_ms=TIMEOUT_MS).count(0) > 0
+
+
+def test_intersection_microkernel(engine):
+    rng = np.random.default_rng(11)
+    lists, a_off, a_len, b_off, b_len = [], [], [], [], []
+    pos = 0
+    expected = []
+    sizes = [(0, 5), (1, 1), (3, 3000), (2500, 2600), (5000, 70), (64, 64), (4096, 10000), (1, 0)]
+    for na, nb in sizes:
+        a = np.unique(rng.integers(0, 20000, na, dtype=np.uint32))
+        b = np.unique(rng.integers(0, 20000, nb, dtype=np.uint32))
+        a_off.append(pos); a_len.append(len(a)); lists.append(a); pos += len(a)
+        b_off.append(pos); b_len.append(len(b)); lists.append(b); pos += len(b)
+        expected.append(np.intersect1d(a, b))
+    data = np.concatenate(lists).astype(np.uint32)
+    out_off = np.cumsum([0] + [min(x, y) for x, y in zip(a_len, b_len)]).astype(np.uint64)
+    res = fm.intersect_pairs(engine, data, np.array(a_off, np.uint64), np.array(a_len, np.uint32),
+                             np.array(b_off, np.uint64), np.array(b_len, np.uint32), out_off[:-1],
+                             int(out_off[-1]))
+    counts, out = res["counts"], res["out"]
+    for i, ex in enumerate(expected):
+        assert counts[i] == len(ex)
+        assert np.array_equal(out[out_off[i]: out_off[i] + counts[i]], ex)
