@@ -148,6 +148,45 @@ def cpu_baseline(g, queries, target_s: float, threads: int):
     return total_cnt, total_s, stride
 
 
+def intersect_bench(engine, pairs: int = 2048, mean_len: int = 32768, reps: int = 5, seed: int = 7):
+    """The wave64 intersection microkernel on synthetic sorted lists.
+    Bytes per pair = 4 (|A| + |B|) + 4 |A n B| (lists read once, the
+    intersection written once)."""
+    rng = np.random.default_rng(seed)
+    la = rng.integers(mean_len // 2, 3 * mean_len // 2, pairs)
+    lb = rng.integers(mean_len // 2, 3 * mean_len // 2, pairs)
+    lens = np.empty(2 * pairs, np.int64)
+    lens[0::2], lens[1::2] = la, lb
+    gaps = rng.integers(1, 5, int(lens.sum())).astype(np.uint32)
+    offs = np.concatenate([[0], np.cumsum(lens)])
+    data = np.cumsum(gaps, dtype=np.uint64)
+    # restart every list at a random base so each list is sorted on its own
+    starts = np.repeat(data[offs[:-1]] - rng.integers(0, 1000, 2 * pairs).astype(np.uint64), lens)
+    data = (data - starts).astype(np.uint32)
+    a_off, b_off = offs[0:-1:2].astype(np.uint64), offs[1::2].astype(np.uint64)
+    out_off = np.concatenate([[0], np.cumsum(np.minimum(la, lb))]).astype(np.uint64)
+    res = fm.intersect_pairs(engine, data, a_off, la.astype(np.uint32), b_off, lb.astype(np.uint32),
+                             out_off[:-1], int(out_off[-1]))
+    bufs = res["buffers"]
+    inter = int(res["counts"].astype(np.int64).sum())
+    # spot-check 8 pairs against numpy
+    ok = True
+    for i in rng.integers(0, pairs, 8):
+        A = data[a_off[i]:a_off[i] + la[i]]
+        B = data[b_off[i]:b_off[i] + lb[i]]
+        ok = ok and int(res["counts"][i]) == len(np.intersect1d(A, B))
+    ms = []
+    for _ in range(reps):
+        r = fm.intersect_pairs(engine, None, None, la.astype(np.uint32), None, None, out_off[:-1],
+                               int(out_off[-1]), buffers=bufs)
+        ms.append(r["kernel_ms"])
+    nbytes = 4 * (int(la.sum()) + int(lb.sum())) + 4 * inter
+    gbs = nbytes / (np.mean(ms) / 1e3) / 1e9
+    return {"kernel": "fm_intersect_kernel", "pairs": pairs, "elements": int(lens.sum()), "bytes": nbytes,
+            "kernel_ms": float(np.mean(ms)), "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": gbs / HBM_PEAK_GBS, "checked": bool(ok)}
+
+
 def b_alg(o, n: int, table: bool) -> int:
     """Algorithmic bytes: neighbour words of every requirement row of every
     partial embedding, an 8-byte record per partial, the final table."""
@@ -165,6 +204,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--queries", type=int, default=0)
+    ap.add_argument("--save-queries", default="", help="write the selected queries + manifest here")
+    ap.add_argument("--load-queries", default="", help="reuse a saved query set (no screening launches)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -186,7 +227,13 @@ def main():
 
     nq = args.queries or (32 if args.config == 2 else 1024)
     t1 = time.time()
-    if args.config == 2:
+    manifest = None
+    if args.load_queries:
+        manifest = json.loads((Path(args.load_queries) / "manifest.json").read_text())
+        chosen = [(m["shape"], m["n"], fm.Query.load_g(str(Path(args.load_queries) / m["file"])))
+                  for m in manifest["queries"]]
+        counts = [m["count"] for m in manifest["queries"]]
+    elif args.config == 2:
         cands = make_candidates(g, 2, args.seed, 4 * nq)
         chosen, counts = screen(engine, g, cands, args.budget, nq, per_shape={0: nq // 2, 1: nq - nq // 2})
         if len(chosen) < nq:
@@ -204,9 +251,25 @@ def main():
     ns = [c[1] for c in chosen]
 
     # algorithmic bytes (instrumented, untimed)
-    ri = engine.match_batch(g, queries, table=False, instrument=True, timeout_ms=300000)
-    balg = [b_alg(ri.query(i), ns[i], table=True) for i in range(len(queries))]
-    rows_bytes = sum(ri.query(i).alg_row_bytes for i in range(len(queries)))
+    if manifest is not None:
+        balg = [m["balg"] for m in manifest["queries"]]
+        rows_bytes = sum(m["row_bytes"] for m in manifest["queries"])
+    else:
+        ri = engine.match_batch(g, queries, table=False, instrument=True, timeout_ms=300000)
+        balg = [b_alg(ri.query(i), ns[i], table=True) for i in range(len(queries))]
+        row_b = [ri.query(i).alg_row_bytes for i in range(len(queries))]
+        rows_bytes = sum(row_b)
+        ri.close()
+        if args.save_queries:
+            d = Path(args.save_queries)
+            d.mkdir(parents=True, exist_ok=True)
+            items = []
+            for i, (shape, n, q) in enumerate(chosen):
+                q.write_g(str(d / f"q{i:04d}.g"))
+                items.append({"file": f"q{i:04d}.g", "shape": shape, "n": n, "count": counts[i], "balg": balg[i],
+                              "row_bytes": row_b[i]})
+            (d / "manifest.json").write_text(json.dumps({"config": args.config, "seed": args.seed,
+                                                         "budget": args.budget, "queries": items}))
 
     result = {}
     if args.config == 2:
@@ -230,9 +293,10 @@ def main():
         kernel_ms /= args.steps
         embeddings = sum(o.count for o in last)
         ok = all(o.status == 0 and o.count == counts[i] and o.rows == o.count for i, o in enumerate(last))
-        # table validity on a sample (untimed)
+        # table validity on a sample (untimed; skipped for a loaded query set
+        # so that a profiled run holds only the timed launches)
         rng = np.random.default_rng(1)
-        for i in range(0, len(queries), 4):
+        for i in range(0, len(queries) if manifest is None else 0, 4):
             r = engine.match(g, queries[i], timeout_ms=300000)
             ok = ok and validate_sample(g, queries[i], r.table(0, sorted=False, max_rows=1 << 16), rng)
             r.close()
@@ -298,6 +362,9 @@ def main():
                          "kernel": "fm_match_kernel", "kernel_ms_per_step": dom_ms},
         }
         workload = "rmat20_batched_1024q"
+
+    if not args.load_queries:
+        result["intersect_roofline"] = intersect_bench(engine)
 
     cpu = None
     if not args.no_cpu:

@@ -36,7 +36,7 @@ STATUS_NAMES = {
     300: "FM_ERR_INVALID_ARG", 301: "FM_ERR_DEVICE", 302: "FM_ERR_OUT_OF_MEMORY",
     303: "FM_ERR_POOL_EXHAUSTED", 304: "FM_ERR_QUEUE_FULL", 305: "FM_ERR_TIMEOUT",
     306: "FM_ERR_COUNT_LIMIT", 307: "FM_ERR_QUERY_TOO_LARGE", 308: "FM_ERR_STALE_RESULT",
-    309: "FM_ERR_NOT_SUPPORTED",
+    309: "FM_ERR_NOT_SUPPORTED", 310: "FM_ERR_INTERNAL",
 }
 
 
@@ -268,6 +268,8 @@ class QueryOutcome:
     alg_row_bytes: int
     row_lookups: int
     partials: list = field(default_factory=list)
+    candidates: int = 0
+    probes: int = 0
 
     @property
     def truncated(self) -> bool:
@@ -287,7 +289,8 @@ class Result:
         qr = L.QueryResult()
         _check(L.lib().fm_result_query(self._h, i, C.byref(qr)))
         return QueryOutcome(int(qr.count), int(qr.rows), int(qr.status), int(qr.flags), int(qr.vertices),
-                            int(qr.alg_bytes), int(qr.row_lookups), [int(x) for x in qr.partials[: qr.vertices]])
+                            int(qr.alg_bytes), int(qr.row_lookups), [int(x) for x in qr.partials[: qr.vertices]],
+                            int(qr.candidates), int(qr.probes))
 
     def count(self, i: int = 0) -> int:
         return self.query(i).count

@@ -47,13 +47,15 @@ class MatchOpts(C.Structure):
 
 class QueryResult(C.Structure):
     _fields_ = [("count", u64), ("rows", u64), ("status", u32), ("flags", u32), ("vertices", u32),
-                ("reserved", u32), ("alg_bytes", u64), ("row_lookups", u64), ("partials", u64 * MAXN)]
+                ("reserved", u32), ("alg_bytes", u64), ("row_lookups", u64), ("partials", u64 * MAXN),
+                ("candidates", u64), ("probes", u64)]
 
 
 class LaunchStats(C.Structure):
     _fields_ = [("kernel_ms", f64), ("total_ms", f64), ("tasks_pushed", u64), ("root_claims", u64),
                 ("banks_allocated", u64), ("banks_reused", u64), ("alloc_failures", u64),
-                ("pages_allocated", u64), ("grid_waves", u32), ("status", u32)]
+                ("pages_allocated", u64), ("tasks_split", u64), ("requeues", u64), ("grid_waves", u32),
+                ("status", u32)]
 
 
 # name -> (restype, argtypes); every function declared in include/forkmatch.h

@@ -44,7 +44,7 @@ constexpr uint32_t LABEL_ABSENT = 0xFFFFFFu;
 constexpr uint32_t ROOT_BANK = 0xFFFFFFFFu;
 constexpr uint32_t NO_BANK = 0xFFFFFFFFu;
 constexpr uint32_t ALL_VERTICES = 0xFFFFFFFFu;
-constexpr uint32_t PAGE_EMPTY = 0, PAGE_BUSY = 1, PAGE_FAILED = 2, PAGE_BASE = 3;
+constexpr uint32_t NO_PAGE = 0xFFFFFFFFu;
 
 // Packed requirement: bits 0-5 anchor depth (== own depth: self-loop),
 // bit 7 direction (0 out-row, 1 in-row), bits 8-31 dense label id.
@@ -65,19 +65,22 @@ struct DPlan {
 
 struct DQuery {
     unsigned long long count;
-    unsigned long long rows;       // table rows reserved
+    unsigned long long rows;       // table rows stored
     unsigned long long row_words;  // instrumented
     unsigned long long lookups;
+    unsigned long long candidates;  // instrumented
+    unsigned long long probes;      // instrumented
     unsigned long long partials[MAXN];
     unsigned long long max_rows;
     unsigned long long max_count;
     unsigned long long max_partials;
     unsigned long long stored;  // intermediate partial embeddings written
+    unsigned long long spare;  // partly filled page handed between waves: (id + 1) << 32 | fill
     uint32_t status;
     uint32_t flags;
     uint32_t page_rows;
     uint32_t max_pages;
-    uint32_t page_table_off;
+    uint32_t npages;  // pages claimed
     uint32_t root_chunk_begin;
     uint32_t root_chunks;
     uint32_t pad;
@@ -91,32 +94,38 @@ enum StatIdx {
     ST_ALLOC_FAIL,
     ST_PAGES,
     ST_REQUEUE,
+    ST_SPLIT,
     ST_N
 };
 
 // Control block. The first part is cleared before every launch; the queue
 // and pool positions persist across launches (both are empty at the end of
-// a clean launch) and are re-initialised after an aborted one.
+// a clean launch) and are re-initialised after an aborted one. Words that
+// different waves hammer sit on separate 128-byte lines.
+struct LevelQ {
+    alignas(128) uint32_t head;
+    alignas(128) uint32_t tail;
+    alignas(128) int32_t avail;
+};
+
 struct Ctl {
-    long long pending;  // tasks pushed or root chunks not yet finished
-    unsigned long long root_cursor;
-    uint32_t abort;
+    alignas(128) long long pending;  // tasks pushed or root chunks not yet finished
+    alignas(128) unsigned long long root_cursor;
+    alignas(128) int32_t hungry;  // waves that found no work
+    alignas(128) uint32_t abort;
     uint32_t error;
-    uint32_t page_bump;
-    uint32_t pad0;
     unsigned long long t0;  // wall clock of the first wave (deadline origin)
     unsigned long long timeout_ticks;
-    unsigned long long stats[ST_N];
+    alignas(128) uint32_t page_bump;
+    alignas(128) unsigned long long stats[ST_N];
     // persistent
-    uint32_t q_head[NLEVELS];
-    uint32_t q_tail[NLEVELS];
-    int32_t q_avail[NLEVELS];
-    uint32_t f_head;
-    uint32_t f_tail;
-    int32_t f_avail;
-    uint32_t bank_bump;
+    alignas(128) LevelQ lq[NLEVELS];
+    alignas(128) uint32_t f_head;
+    alignas(128) uint32_t f_tail;
+    alignas(128) int32_t f_avail;
+    alignas(128) uint32_t bank_bump;
 };
-constexpr size_t CTL_RESET_BYTES = offsetof(Ctl, q_head);
+constexpr size_t CTL_RESET_BYTES = offsetof(Ctl, lq);
 
 struct alignas(32) Slot {
     uint32_t seq;
@@ -125,8 +134,8 @@ struct alignas(32) Slot {
     uint32_t bank;
     uint32_t begin;  // word offset in the bank, or first root index
     uint32_t count;
-    uint32_t skip;   // candidates of the first parent already done
-    uint32_t pad;
+    uint32_t skip;      // driver-row offset where the first parent starts
+    uint32_t last_end;  // driver-row offset where the last parent stops
 };
 
 struct alignas(8) FSlot {
@@ -152,7 +161,8 @@ struct KParams {
     int32_t *bank_ref;
     uint32_t *banks;
     uint32_t *pages;
-    uint32_t *page_table;
+    uint32_t *page_owner;  // query of each claimed output page
+    uint32_t *page_fill;   // rows held by each output page
     unsigned long long timeout_ticks;
     unsigned long long starve_ticks;
     unsigned long long bank_words;
@@ -340,8 +350,12 @@ __device__ void bank_release(const KParams &P, uint32_t id) {
 
 // ---- task queues ------------------------------------------------------------
 
+// A range of parents of one query at one depth. The first parent starts at
+// candidate `skip` of its driver row and the last one stops at `last_end`
+// (ALL_CANDIDATES: its whole row), which lets a task hold part of a row.
+constexpr uint32_t ALL_CANDIDATES = 0xFFFFFFFFu;
 struct TaskDesc {
-    uint32_t q, depth, bank, begin, count, skip;
+    uint32_t q, depth, bank, begin, count, skip, last_end;
 };
 
 // Pushes `m` tasks (lane i < m describes task i) at priority level lvl.
@@ -355,9 +369,10 @@ __device__ void push_tasks(const KParams &P, uint32_t lvl, uint32_t m, const Tas
     if (lane == 0) {
         atom_add(&c->pending, (long long)m);
         if (bank != ROOT_BANK) atom_add(&P.bank_ref[bank], (int32_t)m);
-        pos0 = atom_add(&c->q_tail[lvl], m);
-        uint32_t head = ld_rlx(&c->q_head[lvl]);
-        if (pos0 + m - head > P.qcap) full = 1;
+        pos0 = atom_add(&c->lq[lvl].tail, m);
+        uint32_t head = ld_rlx(&c->lq[lvl].head);
+        // signed: consumers may already have claimed positions past ours
+        if ((int32_t)(pos0 + m - head) > (int32_t)P.qcap) full = 1;
         stats[ST_TASKS] += m;
     }
     pos0 = bcast(pos0, 0);
@@ -382,22 +397,23 @@ __device__ void push_tasks(const KParams &P, uint32_t lvl, uint32_t m, const Tas
         s->begin = t.begin;
         s->count = t.count;
         s->skip = t.skip;
+        s->last_end = t.last_end;
         st_rel(&s->seq, pos + 1);
         }
     }
-    if (lane == 0) atom_add(&c->q_avail[lvl], (int32_t)m);
+    if (lane == 0) atom_add(&c->lq[lvl].avail, (int32_t)m);
 }
 
 // Lane 0 only: takes the task of the lowest level (fewest remaining depths).
 __device__ bool pop_task(const KParams &P, TaskDesc &t) {
     Ctl *c = P.ctl;
     for (uint32_t lvl = 0; lvl < P.nlevels; lvl++) {
-        if (ld_rlx(&c->q_avail[lvl]) <= 0) continue;
-        if (atom_add(&c->q_avail[lvl], -1) <= 0) {
-            atom_add(&c->q_avail[lvl], 1);
+        if (ld_rlx(&c->lq[lvl].avail) <= 0) continue;
+        if (atom_add(&c->lq[lvl].avail, -1) <= 0) {
+            atom_add(&c->lq[lvl].avail, 1);
             continue;
         }
-        uint32_t pos = atom_add(&c->q_head[lvl], 1u);
+        uint32_t pos = atom_add(&c->lq[lvl].head, 1u);
         Slot *s = &P.ring[(size_t)lvl * P.qcap + (pos & (P.qcap - 1))];
         uint32_t it = 0;
         while (ld_acq(&s->seq) != pos + 1)
@@ -408,6 +424,7 @@ __device__ bool pop_task(const KParams &P, TaskDesc &t) {
         t.begin = s->begin;
         t.count = s->count;
         t.skip = s->skip;
+        t.last_end = s->last_end;
         st_rel(&s->seq, pos + P.qcap);
         return true;
     }
@@ -436,34 +453,101 @@ __device__ bool claim_root(const KParams &P, TaskDesc &t, unsigned long long *st
     t.begin = first;
     t.count = min(P.root_chunk, pl.nroots - first);
     t.skip = 0;
+    t.last_end = ALL_CANDIDATES;
     stats[ST_ROOTS]++;
     return true;
 }
 
-// ---- output pages (lane 0 only) ---------------------------------------------
+// ---- output pages ------------------------------------------------------------
+// Each wave owns one output page of its current query and appends rows
+// without atomics; a page it stops using while partly filled is handed to
+// the next wave of the same query through DQuery::spare. The table of a
+// query is the union of its pages (page_owner / page_fill), row order
+// unspecified.
 
-__device__ uint32_t ensure_page(const KParams &P, const DQuery &Q, unsigned long long pg,
-                                unsigned long long *stats) {
-    if (pg >= Q.max_pages) return UINT32_MAX;
-    uint32_t *e = &P.page_table[Q.page_table_off + pg];
-    uint32_t v = ld_acq(e);
-    uint32_t it = 0;
-    while (true) {
-        if (v >= PAGE_BASE) return v - PAGE_BASE;
-        if (v == PAGE_FAILED) return UINT32_MAX;
-        if (v == PAGE_EMPTY && cas_u32(e, PAGE_EMPTY, PAGE_BUSY)) {
-            uint32_t id = UINT32_MAX;
-            if (ld_rlx(&P.ctl->page_bump) < P.npages) id = atom_add(&P.ctl->page_bump, 1u);
-            if (id >= P.npages) {
-                st_rel(e, PAGE_FAILED);
-                return UINT32_MAX;
-            }
-            st_rel(e, id + PAGE_BASE);
-            stats[ST_PAGES]++;
-            return id;
+struct WaveTab {
+    uint32_t q, page, fill, fill0, cap;
+};
+
+// Lane 0 only.
+__device__ void tab_release(const KParams &P, WaveTab &w) {
+    if (w.page == NO_PAGE) return;
+    DQuery &Q = P.qs[w.q];
+    st_rlx(&P.page_fill[w.page], w.fill);
+    if (w.fill > w.fill0) atom_add(&Q.rows, (unsigned long long)(w.fill - w.fill0));
+    if (w.fill < w.cap) {
+        unsigned long long v = ((unsigned long long)(w.page + 1) << 32) | w.fill, zero = 0;
+        __hip_atomic_compare_exchange_strong(&Q.spare, &zero, v, __ATOMIC_RELEASE, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+    }
+    w.page = NO_PAGE;
+}
+
+// Lane 0 only. Takes the query's spare page (if allowed) or claims a fresh
+// one; false when the query's page budget or the arena is exhausted.
+__device__ bool tab_acquire(const KParams &P, WaveTab &w, uint32_t q, uint32_t cap, bool allow_spare,
+                            unsigned long long *stats) {
+    DQuery &Q = P.qs[q];
+    w.q = q;
+    w.cap = cap;
+    w.page = NO_PAGE;
+    if (allow_spare && ld_rlx(&Q.spare) != 0) {
+        unsigned long long v = __hip_atomic_exchange(&Q.spare, 0ull, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        if (v) {
+            w.page = (uint32_t)(v >> 32) - 1;
+            w.fill = w.fill0 = (uint32_t)v;
+            return true;
         }
-        if (!spin_ok(P.ctl, it)) return UINT32_MAX;
-        v = ld_acq(e);
+    }
+    if (ld_rlx(&Q.npages) >= Q.max_pages || ld_rlx(&P.ctl->page_bump) >= P.npages) return false;
+    if (atom_add(&Q.npages, 1u) >= Q.max_pages) return false;
+    uint32_t id = atom_add(&P.ctl->page_bump, 1u);
+    if (id >= P.npages) return false;
+    st_rlx(&P.page_owner[id], q);
+    w.page = id;
+    w.fill = w.fill0 = 0;
+    stats[ST_PAGES]++;
+    return true;
+}
+
+// All lanes. Appends the rows of the lanes with ok set (rank = their order)
+// to query q's table; write_row(dst) stores one row of n words at dst.
+template <class F>
+__device__ void emit_rows(const KParams &P, WaveTab &wt, uint32_t q, uint32_t n, bool ok, uint32_t rank, uint32_t cnt,
+                          unsigned long long *stats, F write_row) {
+    const uint32_t lane = lane_id();
+    uint32_t pA = NO_PAGE, fA = 0, nA = 0, pB = NO_PAGE, fB = 0;
+    if (lane == 0) {
+        const uint32_t cap = (uint32_t)(P.page_words / n);
+        if (wt.page != NO_PAGE && wt.q != q) tab_release(P, wt);
+        if (wt.page == NO_PAGE) tab_acquire(P, wt, q, cap, true, stats);
+        if (wt.page != NO_PAGE) {
+            pA = wt.page;
+            fA = wt.fill;
+            nA = min(cnt, wt.cap - wt.fill);
+            wt.fill += nA;
+            if (nA < cnt) {
+                tab_release(P, wt);
+                if (tab_acquire(P, wt, q, cap, false, stats)) {
+                    pB = wt.page;
+                    fB = wt.fill;
+                    wt.fill += cnt - nA;
+                }
+            }
+        }
+        if (pA == NO_PAGE || (nA < cnt && pB == NO_PAGE))
+            __hip_atomic_fetch_or(&P.qs[q].flags, FM_RESULT_TRUNCATED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    pA = bcast(pA, 0);
+    fA = bcast(fA, 0);
+    nA = bcast(nA, 0);
+    pB = bcast(pB, 0);
+    fB = bcast(fB, 0);
+    if (ok) {
+        uint32_t *dst = nullptr;
+        if (rank < nA) dst = P.pages + (size_t)pA * P.page_words + (size_t)(fA + rank) * n;
+        else if (pB != NO_PAGE) dst = P.pages + (size_t)pB * P.page_words + (size_t)(fB + rank - nA) * n;
+        if (dst) write_row(dst);
     }
 }
 
@@ -510,6 +594,7 @@ __device__ void flush_run(const KParams &P, WaveOut &o, uint32_t q, uint32_t D, 
         td.begin = o.run_begin + ti * per * W;
         td.count = min(per, rows - min(rows, ti * per));
         td.skip = 0;
+        td.last_end = ALL_CANDIDATES;
         push_tasks(P, lvl, m, td, o.bank, stats);
     }
     o.run_begin = o.used;
@@ -518,7 +603,7 @@ __device__ void flush_run(const KParams &P, WaveOut &o, uint32_t q, uint32_t D, 
 
 // Extends one task by one depth. Returns after pushing children tasks; the
 // caller completes the task (bank reference and pending count).
-__device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOut &o,
+__device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOut &o, WaveTab &wt,
                              unsigned long long *stats, unsigned long long &starve_since,
                              unsigned long long deadline) {
     const uint32_t lane = lane_id();
@@ -530,6 +615,17 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
     const uint32_t W = d + 1;  // parent width
     const bool root = t.bank == ROOT_BANK;
     const DGraph &g = P.g;
+    // A malformed task (an engine bug) must end the launch with a status,
+    // never read out of bounds.
+    {
+        bool bad = t.count == 0 || d + 1 >= max(n, 2u) || (t.count == 1 && t.skip > t.last_end);
+        if (root) bad = bad || d != 0 || (unsigned long long)t.begin + t.count > pl.nroots;
+        else bad = bad || t.bank >= P.nbanks || (unsigned long long)t.begin + (unsigned long long)t.count * W > P.bank_words;
+        if (bad) {
+            if (lane == 0) raise_error(P, FM_ERR_INTERNAL);
+            return;
+        }
+    }
 
     // ---- single-vertex query: every root is an embedding -----------------
     if (n == 1) {
@@ -549,30 +645,8 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
             uint32_t rank = __popcll(mask & ((1ull << lane) - 1ull));
             cnt_acc += cnt;
             if (P.instrument && lane == 0) atom_add(&Q.partials[0], (unsigned long long)cnt);
-            if (P.table && cnt) {
-                unsigned long long base = 0;
-                uint32_t p0 = UINT32_MAX, p1 = UINT32_MAX;
-                if (lane == 0) {
-                    base = atom_add(&Q.rows, (unsigned long long)cnt);
-                    if (base < Q.max_rows) {
-                        unsigned long long last = min(base + cnt - 1, Q.max_rows - 1);
-                        p0 = ensure_page(P, Q, base / Q.page_rows, stats);
-                        p1 = ensure_page(P, Q, last / Q.page_rows, stats);
-                    }
-                    if (base + cnt > Q.max_rows || p0 == UINT32_MAX || p1 == UINT32_MAX)
-                        __hip_atomic_fetch_or(&Q.flags, FM_RESULT_TRUNCATED, __ATOMIC_RELAXED,
-                                              __HIP_MEMORY_SCOPE_AGENT);
-                }
-                base = ((unsigned long long)bcast((uint32_t)(base >> 32), 0) << 32) | bcast((uint32_t)base, 0);
-                p0 = bcast(p0, 0);
-                p1 = bcast(p1, 0);
-                unsigned long long r = base + rank;
-                if (ok && r < Q.max_rows) {
-                    unsigned long long pg = r / Q.page_rows;
-                    uint32_t id = pg == base / Q.page_rows ? p0 : p1;
-                    if (id != UINT32_MAX) P.pages[(size_t)id * P.page_words + (r % Q.page_rows)] = v;
-                }
-            }
+            if (P.table && cnt)
+                emit_rows(P, wt, t.q, 1, ok, rank, cnt, stats, [&](uint32_t *dst) { dst[0] = v; });
         }
         if (lane == 0) atom_add(&Q.count, cnt_acc);
         return;
@@ -587,7 +661,9 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
     const uint32_t k = pl.req_begin[D + 1] - rb - nself;  // anchored requirements
 
     unsigned long long cnt_acc = 0;
-    for (uint32_t b0 = 0; b0 < t.count; b0 += WAVE) {
+    uint32_t cnt_local = t.count;       // shrinks when the upper parents are donated
+    uint32_t last_end = t.last_end;     // end of the last parent's driver range
+    for (uint32_t b0 = 0; b0 < cnt_local; b0 += WAVE) {
         // periodic checks
         uint32_t stop = 0;
         if (lane == 0) stop = (ld_rlx(&P.ctl->abort) || ld_rlx(&Q.status)) ? 1u : 0u;
@@ -596,7 +672,7 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
             if (lane == 0) raise_error(P, FM_ERR_TIMEOUT);
             break;
         }
-        const uint32_t nb = min(WAVE, t.count - b0);
+        const uint32_t nb = min(WAVE, cnt_local - b0);
         // ---- parents into LDS ----
         if (root) {
             if (lane < nb) {
@@ -635,18 +711,17 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
                     didx = i;
                 }
             }
-            dlen = alive ? best : 0;
-            if (b0 == 0 && lane == 0 && t.skip) {
-                uint32_t sk = min(t.skip, dlen);
-                dst_ += sk;
-                dlen -= sk;
-            }
+            uint32_t endv = best;
+            if (b0 + lane == cnt_local - 1) endv = min(endv, last_end);
+            const uint32_t startv = (b0 == 0 && lane == 0) ? min(t.skip, endv) : 0;
+            dst_ += startv;
+            dlen = alive ? endv - startv : 0;
         }
         s.dstart[lane] = dst_;
         s.didx[lane] = didx;
         const uint32_t incl = wave_incl_scan(dlen);
         s.cum[lane] = incl;
-        const uint32_t T = bcast(incl, WAVE - 1);
+        uint32_t T = bcast(incl, WAVE - 1);
         if (P.instrument) {
             unsigned long long wsum = wave_sum_u64(alive ? words : 0ull);
             uint32_t nalive = __popcll(__ballot(alive));
@@ -654,11 +729,52 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
                 atom_add(&Q.row_words, wsum);
                 atom_add(&Q.lookups, (unsigned long long)nalive * k);
                 if (root) atom_add(&Q.partials[0], (unsigned long long)nalive);
+                atom_add(&Q.candidates, (unsigned long long)T);
             }
         }
         __syncthreads();
         // ---- candidates, 64 per step ----
-        for (uint32_t base = 0; base < T; base += WAVE) {
+        for (uint32_t base = 0, step = 0; base < T; base += WAVE, step++) {
+            // Work splitting: when waves are idle, donate half of what is
+            // left, either the upper half of the remaining parents or, in a
+            // task's last batch, the upper half of its remaining candidates
+            // (a hub parent's long driver row is split across waves).
+            if ((step & 3) == 0) {
+                uint32_t want = 0;
+                if (lane == 0) want = ld_rlx(&P.ctl->hungry) > 0 ? 1u : 0u;
+                if (bcast(want, 0)) {
+                    // a candidate split may already have ended the task inside this batch
+                    const uint32_t rem_par = cnt_local > b0 + nb ? cnt_local - (b0 + nb) : 0;
+                    if (rem_par >= 2 * WAVE) {
+                        const uint32_t keep = rem_par / 2, first = b0 + nb + keep;
+                        TaskDesc r{t.q, d, t.bank, root ? t.begin + first : t.begin + first * W, rem_par - keep, 0,
+                                   last_end};
+                        push_tasks(P, lvl_self, 1, r, t.bank, stats);
+                        cnt_local = first;
+                        last_end = ALL_CANDIDATES;
+                        if (lane == 0) stats[ST_SPLIT]++;
+                    } else if (rem_par == 0 && T - base >= 8 * WAVE) {
+                        const uint32_t mid = base + ((T - base) / (2 * WAVE)) * WAVE;
+                        uint32_t lo = 0, hi = nb - 1;
+                        while (lo < hi) {
+                            uint32_t m2 = (lo + hi) >> 1;
+                            if (s.cum[m2] > mid) hi = m2;
+                            else lo = m2 + 1;
+                        }
+                        const uint32_t excl = lo ? s.cum[lo - 1] : 0;
+                        // driver-row offset of candidate `mid` in parent lo
+                        const uint32_t abs_mid = (mid - excl) + ((b0 == 0 && lo == 0) ? t.skip : 0);
+                        TaskDesc r{t.q, d, t.bank, root ? t.begin + b0 + lo : t.begin + (b0 + lo) * W,
+                                   cnt_local - b0 - lo, abs_mid, last_end};
+                        push_tasks(P, lvl_self, 1, r, t.bank, stats);
+                        // this task now ends inside parent lo
+                        cnt_local = b0 + lo + 1;
+                        last_end = abs_mid;
+                        T = mid;
+                        if (lane == 0) stats[ST_SPLIT]++;
+                    }
+                }
+            }
             if (!final_level) {
                 const uint32_t need = WAVE * (D + 1);
                 bool yield = false;
@@ -704,14 +820,16 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
                     r.depth = d;
                     r.bank = t.bank;
                     r.begin = root ? t.begin + b0 + lo : t.begin + (b0 + lo) * W;
-                    r.count = t.count - b0 - lo;
+                    r.count = cnt_local - b0 - lo;
                     r.skip = (base - excl) + ((b0 == 0 && lo == 0) ? t.skip : 0);
+                    r.last_end = last_end;
                     if (lane == 0) stats[ST_REQUEUE]++;
                     push_tasks(P, lvl_self, 1, r, t.bank, stats);
                     return;
                 }
             }
             const uint32_t tt = base + lane;
+            uint32_t nprobe = 0;
             bool ok = false;
             uint32_t x = 0, pi = 0;
             if (tt < T) {
@@ -729,12 +847,19 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
                 for (uint32_t a = 0; a < W; a++) ok &= prow[a] != x;
                 const uint32_t dx = s.didx[pi];
                 for (uint32_t i = 0; i < k && ok; i++)
-                    if (i != dx) ok = row_contains(g.col, s.rows_st[pi * MAXR + i], s.rows_len[pi * MAXR + i], x);
+                    if (i != dx) {
+                        ok = row_contains(g.col, s.rows_st[pi * MAXR + i], s.rows_len[pi * MAXR + i], x);
+                        nprobe++;
+                    }
                 for (uint32_t i = 0; i < nself && ok; i++) {
                     uint32_t st, len;
                     row_of(g, pl.req[rb + k + i], x, st, len);
                     ok = row_contains(g.col, st, len, x);
                 }
+            }
+            if (P.instrument) {
+                unsigned long long np = wave_sum_u64(nprobe);
+                if (lane == 0) atom_add(&Q.probes, np);
             }
             const unsigned long long mask = __ballot(ok);
             const uint32_t cnt = __popcll(mask);
@@ -744,33 +869,11 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
             if (final_level) {
                 cnt_acc += cnt;
                 if (P.table) {
-                    unsigned long long base_row = 0;
-                    uint32_t p0 = UINT32_MAX, p1 = UINT32_MAX;
-                    if (lane == 0) {
-                        base_row = atom_add(&Q.rows, (unsigned long long)cnt);
-                        if (base_row < Q.max_rows) {
-                            unsigned long long last = min(base_row + cnt - 1, Q.max_rows - 1);
-                            p0 = ensure_page(P, Q, base_row / Q.page_rows, stats);
-                            p1 = ensure_page(P, Q, last / Q.page_rows, stats);
-                        }
-                        if (base_row + cnt > Q.max_rows || p0 == UINT32_MAX || p1 == UINT32_MAX)
-                            __hip_atomic_fetch_or(&Q.flags, FM_RESULT_TRUNCATED, __ATOMIC_RELAXED,
-                                                  __HIP_MEMORY_SCOPE_AGENT);
-                    }
-                    base_row = ((unsigned long long)bcast((uint32_t)(base_row >> 32), 0) << 32) |
-                               bcast((uint32_t)base_row, 0);
-                    p0 = bcast(p0, 0);
-                    p1 = bcast(p1, 0);
-                    unsigned long long r = base_row + rank;
-                    if (ok && r < Q.max_rows) {
-                        uint32_t id = (r / Q.page_rows) == (base_row / Q.page_rows) ? p0 : p1;
-                        if (id != UINT32_MAX) {
-                            uint32_t *dst = P.pages + (size_t)id * P.page_words + (r % Q.page_rows) * n;
-                            const uint32_t *prow = &s.par[pi * W];
-                            for (uint32_t j = 0; j < W; j++) dst[pl.order[j]] = prow[j];
-                            dst[pl.order[D]] = x;
-                        }
-                    }
+                    const uint32_t *prow = &s.par[pi * W];
+                    emit_rows(P, wt, t.q, n, ok, rank, cnt, stats, [&](uint32_t *dst) {
+                        for (uint32_t j = 0; j < W; j++) dst[pl.order[j]] = prow[j];
+                        dst[pl.order[D]] = x;
+                    });
                 }
             } else {
                 const uint32_t CW = D + 1;
@@ -802,9 +905,12 @@ __global__ __launch_bounds__(WAVE) void fm_match_kernel(KParams P) {
     __shared__ Lds s;
     __shared__ TaskDesc s_task;
     __shared__ uint32_t s_flag;
+    __shared__ WaveTab wt;  // lane 0 owns it
     const uint32_t lane = lane_id();
     WaveOut o{NO_BANK, 0, 0};
-    unsigned long long stats[ST_N] = {0, 0, 0, 0, 0, 0, 0};
+    uint32_t idle = 0;
+    if (lane == 0) wt = WaveTab{0, NO_PAGE, 0, 0, 0};
+    unsigned long long stats[ST_N] = {0, 0, 0, 0, 0, 0, 0, 0};
     unsigned long long starve_since = 0;
     unsigned long long deadline = 0;
     if (lane == 0) {
@@ -838,12 +944,17 @@ __global__ __launch_bounds__(WAVE) void fm_match_kernel(KParams P) {
         const uint32_t flag = s_flag;
         if (flag == 2) break;
         if (flag == 1) {
-            backoff();
+            if (idle == 0 && lane == 0) atom_add(&P.ctl->hungry, 1);
+            // exponential back-off keeps idle waves off the queue counters
+            for (uint32_t i = 0, lim = 1u << min(idle, 5u); i < lim; i++) __builtin_amdgcn_s_sleep(8);
+            idle++;
             continue;
         }
+        if (idle && lane == 0) atom_add(&P.ctl->hungry, -1);
+        idle = 0;
         const TaskDesc t = s_task;
         __syncthreads();
-        process_task(P, t, s, o, stats, starve_since, deadline);
+        process_task(P, t, s, o, wt, stats, starve_since, deadline);
         __syncthreads();
         if (lane == 0) {
             if (t.bank != ROOT_BANK) bank_release(P, t.bank);
@@ -852,6 +963,7 @@ __global__ __launch_bounds__(WAVE) void fm_match_kernel(KParams P) {
     }
     if (lane == 0) {
         if (o.bank != NO_BANK) bank_release(P, o.bank);
+        tab_release(P, wt);
         for (int i = 0; i < ST_N; i++)
             if (stats[i]) atom_add(&P.ctl->stats[i], stats[i]);
     }
@@ -982,10 +1094,10 @@ struct fm_engine {
     uint32_t *pages = nullptr;
     uint64_t page_words = 0;
     uint32_t npages = 0;
+    uint32_t *page_owner = nullptr;
+    uint32_t *page_fill = nullptr;
     Ctl *ctl = nullptr;
     // per-launch buffers (grow only)
-    uint32_t *page_table = nullptr;
-    size_t page_table_cap = 0;
     DPlan *d_plans = nullptr;
     DQuery *d_qs = nullptr;
     size_t q_cap = 0;
@@ -1000,6 +1112,8 @@ struct fm_result {
     std::vector<DQuery> qs;
     std::vector<uint32_t> n;
     fm_launch_stats stats{};
+    uint32_t pages_used = 0;
+    std::vector<uint32_t> page_owner, page_fill;  // fetched on the first table copy
 };
 
 static int full_reset(fm_engine *e) {
@@ -1160,12 +1274,15 @@ int fm_engine_create(const fm_engine_config *cfg, fm_engine **out) {
     e->page_words = c.output_page_bytes / 4;
     e->npages = (uint32_t)std::min<uint64_t>(c.output_prealloc / c.output_page_bytes, 1u << 30);
     if (e->npages == 0) return cleanup_fail(fail(FM_ERR_INVALID_ARG, "OUTPUT_PREALLOC smaller than one page"));
+    if (e->page_words < WAVE * MAXN)
+        return cleanup_fail(fail(FM_ERR_INVALID_ARG, "output pages must hold at least %u words", WAVE * MAXN));
     auto alloc = [&](void **p, size_t bytes) { return hipMalloc(p, std::max<size_t>(bytes, 4)) == hipSuccess; };
     if (!alloc((void **)&e->banks, (size_t)e->nbanks * e->bank_words * 4) ||
         !alloc((void **)&e->bank_ref, (size_t)e->nbanks * 4) ||
         !alloc((void **)&e->free_ring, (size_t)e->fcap * sizeof(FSlot)) ||
         !alloc((void **)&e->ring, (size_t)e->qcap * NLEVELS * sizeof(Slot)) ||
         !alloc((void **)&e->pages, (size_t)e->npages * e->page_words * 4) ||
+        !alloc((void **)&e->page_owner, (size_t)e->npages * 4) || !alloc((void **)&e->page_fill, (size_t)e->npages * 4) ||
         !alloc((void **)&e->ctl, sizeof(Ctl)))
         return cleanup_fail(fail(FM_ERR_OUT_OF_MEMORY, "engine arenas do not fit in device memory"));
     int rc = full_reset(e);
@@ -1184,7 +1301,8 @@ void fm_engine_destroy(fm_engine *e) {
     (void)hipFree(e->ring);
     (void)hipFree(e->pages);
     (void)hipFree(e->ctl);
-    (void)hipFree(e->page_table);
+    (void)hipFree(e->page_owner);
+    (void)hipFree(e->page_fill);
     (void)hipFree(e->d_plans);
     (void)hipFree(e->d_qs);
     if (e->ev0) (void)hipEventDestroy(e->ev0);
@@ -1280,7 +1398,7 @@ int fm_match_batch(fm_engine *e, const fm_graph *g, const fm_query *const *qs, u
     // ---- host-side plans and per-query state ----
     std::vector<DPlan> plans(nq);
     std::vector<DQuery> qh(nq);
-    uint64_t chunks = 0, pt_entries = 0;
+    uint64_t chunks = 0;
     uint32_t maxn = 1;
     for (uint32_t i = 0; i < nq; i++) {
         if (!qs[i]) return fail(FM_ERR_INVALID_ARG, "null query %u", i);
@@ -1293,9 +1411,10 @@ int fm_match_batch(fm_engine *e, const fm_graph *g, const fm_query *const *qs, u
         Q.page_rows = (uint32_t)(e->page_words / n);
         uint64_t cap_rows = (uint64_t)e->npages * Q.page_rows;
         Q.max_rows = (o.mode == FM_MODE_TABLE) ? (o.max_rows ? std::min<uint64_t>(o.max_rows, cap_rows) : cap_rows) : 0;
-        Q.max_pages = (o.mode == FM_MODE_TABLE) ? (uint32_t)((Q.max_rows + Q.page_rows - 1) / Q.page_rows) : 0;
-        Q.page_table_off = (uint32_t)pt_entries;
-        pt_entries += Q.max_pages;
+        // pages are filled by several waves at once, so a query may hold one
+        // partly filled page per wave beyond the full ones
+        uint64_t want_pages = (Q.max_rows + Q.page_rows - 1) / Q.page_rows + (o.max_rows ? e->grid_waves : 0);
+        Q.max_pages = (o.mode == FM_MODE_TABLE) ? (uint32_t)std::min<uint64_t>(want_pages, e->npages) : 0;
         Q.max_count = o.max_count;
         Q.max_partials = o.max_partials;
         Q.root_chunk_begin = (uint32_t)chunks;
@@ -1303,7 +1422,6 @@ int fm_match_batch(fm_engine *e, const fm_graph *g, const fm_query *const *qs, u
         chunks += Q.root_chunks;
     }
     if (chunks >= (1ull << 32)) return fail(FM_ERR_INVALID_ARG, "too many root chunks");
-    if (pt_entries >= (1ull << 32)) return fail(FM_ERR_INVALID_ARG, "page table too large");
     // ---- device buffers ----
     if (nq > e->q_cap) {
         (void)hipFree(e->d_plans);
@@ -1315,16 +1433,8 @@ int fm_match_batch(fm_engine *e, const fm_graph *g, const fm_query *const *qs, u
         HIP_TRY(hipMalloc((void **)&e->d_qs, cap * sizeof(DQuery)));
         e->q_cap = cap;
     }
-    if (pt_entries > e->page_table_cap) {
-        (void)hipFree(e->page_table);
-        e->page_table = nullptr;
-        size_t cap = std::max<size_t>(pt_entries, 1024);
-        HIP_TRY(hipMalloc((void **)&e->page_table, cap * 4));
-        e->page_table_cap = cap;
-    }
     HIP_TRY(hipMemcpyAsync(e->d_plans, plans.data(), nq * sizeof(DPlan), hipMemcpyHostToDevice, e->stream));
     HIP_TRY(hipMemcpyAsync(e->d_qs, qh.data(), nq * sizeof(DQuery), hipMemcpyHostToDevice, e->stream));
-    if (pt_entries) HIP_TRY(hipMemsetAsync(e->page_table, 0, pt_entries * 4, e->stream));
     Ctl reset{};
     reset.pending = (long long)chunks;
     reset.timeout_ticks = (unsigned long long)((double)o.timeout_ms * (e->wall_hz / 1000.0));
@@ -1345,7 +1455,8 @@ int fm_match_batch(fm_engine *e, const fm_graph *g, const fm_query *const *qs, u
     P.bank_ref = e->bank_ref;
     P.banks = e->banks;
     P.pages = e->pages;
-    P.page_table = e->page_table;
+    P.page_owner = e->page_owner;
+    P.page_fill = e->page_fill;
     P.bank_words = e->bank_words;
     P.page_words = e->page_words;
     P.nq = nq;
@@ -1392,7 +1503,10 @@ int fm_match_batch(fm_engine *e, const fm_graph *g, const fm_query *const *qs, u
     st.banks_allocated = ctl_back.stats[ST_BANKS_FRESH];
     st.banks_reused = ctl_back.stats[ST_BANKS_REUSED];
     st.alloc_failures = ctl_back.stats[ST_ALLOC_FAIL];
+    st.tasks_split = ctl_back.stats[ST_SPLIT];
+    st.requeues = ctl_back.stats[ST_REQUEUE];
     st.pages_allocated = ctl_back.stats[ST_PAGES];
+    r->pages_used = std::min(ctl_back.page_bump, e->npages);
     st.status = ctl_back.error;
     if (ctl_back.error) {
         // an aborted launch leaves tasks and bank references behind
@@ -1402,8 +1516,10 @@ int fm_match_batch(fm_engine *e, const fm_graph *g, const fm_query *const *qs, u
             Q.flags |= FM_RESULT_ABORTED;
         }
     }
-    for (auto &Q : r->qs)
+    for (auto &Q : r->qs) {
         if (Q.status == FM_ERR_COUNT_LIMIT) Q.flags |= FM_RESULT_ABORTED;
+        if (o.mode == FM_MODE_TABLE && Q.count > std::min<uint64_t>(Q.rows, Q.max_rows)) Q.flags |= FM_RESULT_TRUNCATED;
+    }
     st.total_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     r->stats = st;
@@ -1438,6 +1554,8 @@ int fm_result_query(const fm_result *r, uint32_t qi, fm_query_result *out) {
     out->alg_bytes = Q.row_words * 4;
     out->row_lookups = Q.lookups;
     for (uint32_t d = 0; d < MAXN; d++) out->partials[d] = Q.partials[d];
+    out->candidates = Q.candidates;
+    out->probes = Q.probes;
     return FM_OK;
 }
 
@@ -1459,15 +1577,18 @@ int fm_result_copy_table(const fm_result *r, uint32_t qi, uint32_t *dst, uint64_
     if (rows == 0) return FM_OK;
     if (!dst) return fail(FM_ERR_INVALID_ARG, "null destination");
     HIP_TRY(hipSetDevice(e->device));
-    uint64_t npg = (rows + Q.page_rows - 1) / Q.page_rows;
-    std::vector<uint32_t> pt(npg);
-    HIP_TRY(hipMemcpy(pt.data(), e->page_table + Q.page_table_off, npg * 4, hipMemcpyDeviceToHost));
+    fm_result *rw = const_cast<fm_result *>(r);
+    if (rw->page_owner.size() != r->pages_used) {
+        rw->page_owner.resize(r->pages_used);
+        rw->page_fill.resize(r->pages_used);
+        HIP_TRY(hipMemcpy(rw->page_owner.data(), e->page_owner, r->pages_used * 4ull, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(rw->page_fill.data(), e->page_fill, r->pages_used * 4ull, hipMemcpyDeviceToHost));
+    }
     uint64_t done = 0;
-    for (uint64_t p = 0; p < npg && done < rows; p++) {
-        if (pt[p] < PAGE_BASE) break;  // a page that could not be allocated ends the table
-        uint64_t take = std::min<uint64_t>(Q.page_rows, rows - done);
-        HIP_TRY(hipMemcpy(dst + done * n, e->pages + (size_t)(pt[p] - PAGE_BASE) * e->page_words,
-                          take * n * 4, hipMemcpyDeviceToHost));
+    for (uint32_t p = 0; p < r->pages_used && done < rows; p++) {
+        if (r->page_owner[p] != qi || r->page_fill[p] == 0) continue;
+        uint64_t take = std::min<uint64_t>(r->page_fill[p], rows - done);
+        HIP_TRY(hipMemcpy(dst + done * n, e->pages + (size_t)p * e->page_words, take * n * 4, hipMemcpyDeviceToHost));
         done += take;
     }
     if (sorted && done > 1) {

@@ -47,8 +47,9 @@
  * each block a fixed root range, src/smsm.cu:803-811). */
 #define TooLargeThreshold 1024
 
-/* Bytes of one output-table page. */
-#define OUTPUT_PAGE_BYTES (1u << 20)
+/* Bytes of one output-table page. A wave appends to its own page; a
+ * partly filled page is handed to the next wave of the same query. */
+#define OUTPUT_PAGE_BYTES (1u << 18)
 
 /* Limits of the device plan. The reference supports queries of at most 19
  * vertices and 3 backward edges per vertex (src/ccsr.cuh:381-385,

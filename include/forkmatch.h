@@ -64,7 +64,8 @@ enum fm_status {
     FM_ERR_COUNT_LIMIT = 306,     /* a query passed opts.max_count */
     FM_ERR_QUERY_TOO_LARGE = 307, /* query beyond the device plan limits */
     FM_ERR_STALE_RESULT = 308,    /* the engine ran again; tables are gone */
-    FM_ERR_NOT_SUPPORTED = 309
+    FM_ERR_NOT_SUPPORTED = 309,
+    FM_ERR_INTERNAL = 310         /* the device caught an inconsistent task */
 };
 
 /* Query size limit of the device plan (environment.h FM_MAX_QUERY_VERTICES). */
@@ -258,6 +259,10 @@ typedef struct fm_query_result {
     uint64_t alg_bytes;
     uint64_t row_lookups;
     uint64_t partials[FM_MAX_QUERY_VERTICES_ABI];
+    /* candidates drawn from driver rows, and membership probes into the
+     * other requirement rows (instrumented launches only) */
+    uint64_t candidates;
+    uint64_t probes;
 } fm_query_result;
 
 typedef struct fm_launch_stats {
@@ -269,6 +274,8 @@ typedef struct fm_launch_stats {
     uint64_t banks_reused;   /* banks taken from the free list */
     uint64_t alloc_failures; /* requeues after a failed bank allocation */
     uint64_t pages_allocated;
+    uint64_t tasks_split;    /* tasks donated to idle waves (work splitting) */
+    uint64_t requeues;       /* tasks re-queued by a yielding wave */
     uint32_t grid_waves;
     uint32_t status;
 } fm_launch_stats;

@@ -258,3 +258,27 @@ def test_intersection_microkernel(engine):
     for i, ex in enumerate(expected):
         assert counts[i] == len(ex)
         assert np.array_equal(out[out_off[i]: out_off[i] + counts[i]], ex)
+
+
+def test_hub_rows_are_split_across_waves(engine):
+    # one hub joined to 6000 leaves plus a sparse ring among the leaves: the
+    # hub's driver rows are far longer than a step, so idle waves get halves
+    V = 6001
+    leaves = np.arange(1, V, dtype=np.uint32)
+    ring_a = leaves
+    ring_b = np.roll(leaves, 1)
+    s = np.concatenate([np.zeros(V - 1, np.uint32), leaves, ring_a, ring_b])
+    d = np.concatenate([leaves, np.zeros(V - 1, np.uint32), ring_b, ring_a])
+    lab = np.zeros(len(s), np.uint32)
+    g = fm.Graph.from_edges(V, s, d, lab).upload()
+    og = orc.OracleGraph(V, s, d, lab)
+    split = 0
+    for n, e in [(3, [(0, 1, 0), (1, 0, 0), (0, 2, 0), (2, 0, 0)]),
+                 (4, [(0, 1, 0), (1, 0, 0), (0, 2, 0), (2, 0, 0), (0, 3, 0), (3, 0, 0), (1, 2, 0), (2, 1, 0)])]:
+        q = fm.Query.from_edges(n, e)
+        ref = orc.match(og, n, q.edges())
+        r = engine.match(g, q, timeout_ms=TIMEOUT_MS, task_rows=64)
+        split += r.stats().tasks_split
+        assert r.count(0) == ref.count
+        assert np.array_equal(r.table(0), ref.table)
+    assert split > 0
