@@ -116,6 +116,7 @@ struct Ctl {
     uint32_t error;
     unsigned long long t0;  // wall clock of the first wave (deadline origin)
     unsigned long long timeout_ticks;
+    uint32_t diag[12];  // the task that raised FM_ERR_INTERNAL
     alignas(128) uint32_t page_bump;
     alignas(128) unsigned long long stats[ST_N];
     // persistent
@@ -290,6 +291,16 @@ __device__ bool spin_ok(const Ctl *c, uint32_t &iter) {
     return true;
 }
 
+// Waits until *p == want, polling with relaxed loads (no cache invalidation
+// per poll) and acquiring once at the end. false: the launch is aborting.
+__device__ bool wait_eq(const uint32_t *p, uint32_t want, const Ctl *c) {
+    uint32_t it = 0;
+    while (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != want)
+        if (!spin_ok(c, it)) return false;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return true;
+}
+
 __device__ void raise_error(const KParams &P, uint32_t code) {
     uint32_t zero = 0;
     __hip_atomic_compare_exchange_strong(&P.ctl->error, &zero, code, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
@@ -312,9 +323,7 @@ __device__ uint32_t bank_alloc(const KParams &P, unsigned long long *stats) {
         if (a > 0) {
             uint32_t pos = atom_add(&c->f_head, 1u);
             FSlot *s = &P.free_ring[pos & (P.fcap - 1)];
-            uint32_t it = 0;
-            while (ld_acq(&s->seq) != pos + 1)
-                if (!spin_ok(c, it)) return NO_BANK;
+            if (!wait_eq(&s->seq, pos + 1, c)) return NO_BANK;
             uint32_t id = s->id;
             st_rel(&s->seq, pos + P.fcap);
             __hip_atomic_store(&P.bank_ref[id], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -339,9 +348,7 @@ __device__ void bank_release(const KParams &P, uint32_t id) {
         Ctl *c = P.ctl;
         uint32_t pos = atom_add(&c->f_tail, 1u);
         FSlot *s = &P.free_ring[pos & (P.fcap - 1)];
-        uint32_t it = 0;
-        while (ld_acq(&s->seq) != pos)
-            if (!spin_ok(c, it)) return;
+        if (!wait_eq(&s->seq, pos, c)) return;
         s->id = id;
         st_rel(&s->seq, pos + 1);
         atom_add(&c->f_avail, 1);
@@ -387,10 +394,7 @@ __device__ void push_tasks(const KParams &P, uint32_t lvl, uint32_t m, const Tas
     if (lane < m) {
         uint32_t pos = pos0 + lane;
         Slot *s = &P.ring[(size_t)lvl * P.qcap + (pos & (P.qcap - 1))];
-        uint32_t it = 0;
-        bool ok = true;
-        while (ok && ld_acq(&s->seq) != pos) ok = spin_ok(c, it);
-        if (ok) {
+        if (wait_eq(&s->seq, pos, c)) {
         s->q = t.q;
         s->depth = t.depth;
         s->bank = t.bank;
@@ -415,9 +419,7 @@ __device__ bool pop_task(const KParams &P, TaskDesc &t) {
         }
         uint32_t pos = atom_add(&c->lq[lvl].head, 1u);
         Slot *s = &P.ring[(size_t)lvl * P.qcap + (pos & (P.qcap - 1))];
-        uint32_t it = 0;
-        while (ld_acq(&s->seq) != pos + 1)
-            if (!spin_ok(c, it)) return false;
+        if (!wait_eq(&s->seq, pos + 1, c)) return false;
         t.q = s->q;
         t.depth = s->depth;
         t.bank = s->bank;
@@ -622,7 +624,24 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
         if (root) bad = bad || d != 0 || (unsigned long long)t.begin + t.count > pl.nroots;
         else bad = bad || t.bank >= P.nbanks || (unsigned long long)t.begin + (unsigned long long)t.count * W > P.bank_words;
         if (bad) {
-            if (lane == 0) raise_error(P, FM_ERR_INTERNAL);
+            if (lane == 0) {
+                uint32_t zero = 0;
+                if (__hip_atomic_compare_exchange_strong(&P.ctl->error, &zero, (uint32_t)FM_ERR_INTERNAL,
+                                                         __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_AGENT)) {
+                    uint32_t *dg = P.ctl->diag;
+                    dg[0] = t.q;
+                    dg[1] = t.depth;
+                    dg[2] = t.bank;
+                    dg[3] = t.begin;
+                    dg[4] = t.count;
+                    dg[5] = t.skip;
+                    dg[6] = t.last_end;
+                    dg[7] = n;
+                    dg[8] = pl.nroots;
+                }
+                st_rlx(&P.ctl->abort, 1u);
+            }
             return;
         }
     }
@@ -957,7 +976,7 @@ __global__ __launch_bounds__(WAVE) void fm_match_kernel(KParams P) {
         process_task(P, t, s, o, wt, stats, starve_since, deadline);
         __syncthreads();
         if (lane == 0) {
-            if (t.bank != ROOT_BANK) bank_release(P, t.bank);
+            if (t.bank != ROOT_BANK && t.bank < P.nbanks) bank_release(P, t.bank);
             atom_add(&P.ctl->pending, -1ll);
         }
     }
@@ -1524,6 +1543,13 @@ int fm_match_batch(fm_engine *e, const fm_graph *g, const fm_query *const *qs, u
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     r->stats = st;
     *out = r;
+    if (ctl_back.error == FM_ERR_INTERNAL) {
+        const uint32_t *dg = ctl_back.diag;
+        return fail(FM_ERR_INTERNAL,
+                    "inconsistent task q=%u depth=%u bank=%u begin=%u count=%u skip=%u last_end=%u (n=%u nroots=%u "
+                    "bank_words=%llu)",
+                    dg[0], dg[1], dg[2], dg[3], dg[4], dg[5], dg[6], dg[7], dg[8], (unsigned long long)e->bank_words);
+    }
     if (ctl_back.error)
         return fail((int)ctl_back.error, "match launch aborted with status %u", ctl_back.error);
     return FM_OK;

@@ -261,9 +261,9 @@ def test_intersection_microkernel(engine):
 
 
 def test_hub_rows_are_split_across_waves(engine):
-    # one hub joined to 6000 leaves plus a sparse ring among the leaves: the
-    # hub's driver rows are far longer than a step, so idle waves get halves
-    V = 6001
+    # one hub joined to 2000 leaves plus a ring among the leaves: the hub's
+    # driver rows are far longer than a step, so idle waves get halves
+    V = 2001
     leaves = np.arange(1, V, dtype=np.uint32)
     ring_a = leaves
     ring_b = np.roll(leaves, 1)
