@@ -56,9 +56,12 @@ def make_candidates(g, cfg: int, seed: int, count: int):
     while len(out) < count:
         i += 1
         if cfg == 2:
-            shape = 0 if len(out) % 2 == 0 else 1
+            # tree-like: a walk tree plus exactly one induced chord (every pure
+            # 6-8 vertex tree on this graph has > 4e8 embeddings); the other
+            # half has >= 2 independent cycles
+            shape = 3 if len(out) % 2 == 0 else 1
             n = int(rng.integers(6, 9))
-            cyc = 2 if shape == 1 else 0
+            cyc = 1 if shape == 3 else 2
         else:
             u = rng.random()
             shape = 0 if u < 0.4 else (1 if u < 0.8 else 2)
@@ -78,10 +81,10 @@ def screen(engine, g, cands, budget: int, want, per_shape=None):
     keep, counts = [], []
     taken = {}
     hist = {}
-    for b0 in range(0, len(cands), 32):
-        chunk = cands[b0:b0 + 32]
+    for b0 in range(0, len(cands), 64):
+        chunk = cands[b0:b0 + 64]
         r = engine.match_batch(g, [c[2] for c in chunk], table=False, max_count=budget,
-                               max_partials=8 * budget, timeout_ms=120000, allow_errors=True)
+                               max_partials=budget, timeout_ms=120000, allow_errors=True)
         for j, c in enumerate(chunk):
             o = r.query(j)
             hist[o.status] = hist.get(o.status, 0) + 1
@@ -172,8 +175,8 @@ def intersect_bench(engine, pairs: int = 2048, mean_len: int = 32768, reps: int 
     # spot-check 8 pairs against numpy
     ok = True
     for i in rng.integers(0, pairs, 8):
-        A = data[a_off[i]:a_off[i] + la[i]]
-        B = data[b_off[i]:b_off[i] + lb[i]]
+        A = data[int(a_off[i]):int(a_off[i]) + int(la[i])]
+        B = data[int(b_off[i]):int(b_off[i]) + int(lb[i])]
         ok = ok and int(res["counts"][i]) == len(np.intersect1d(A, B))
     ms = []
     for _ in range(reps):
@@ -235,11 +238,11 @@ def main():
         counts = [m["count"] for m in manifest["queries"]]
     elif args.config == 2:
         cands = make_candidates(g, 2, args.seed, 4 * nq)
-        chosen, counts = screen(engine, g, cands, args.budget, nq, per_shape={0: nq // 2, 1: nq - nq // 2})
+        chosen, counts = screen(engine, g, cands, args.budget, nq, per_shape={3: nq // 2, 1: nq - nq // 2})
         if len(chosen) < nq:
             more = make_candidates(g, 2, args.seed + 1, 16 * nq)
             c2, n2 = screen(engine, g, more, args.budget, nq - len(chosen),
-                            per_shape={0: nq // 2 - sum(1 for c in chosen if c[0] == 0),
+                            per_shape={3: nq // 2 - sum(1 for c in chosen if c[0] == 3),
                                        1: nq - nq // 2 - sum(1 for c in chosen if c[0] == 1)})
             chosen += c2
             counts += n2

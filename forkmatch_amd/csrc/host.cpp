@@ -728,6 +728,7 @@ int gen_rmat(uint32_t scale, uint64_t m, double a, double b, double c, uint32_t 
 //    (edges into the set)^3, which favours vertices that close cycles; the
 //    query is the growth tree plus `min_cycles` (or one more) induced chords.
 //  shape 2 (near-clique): the same growth; every induced edge is kept.
+//  shape 3 (tree-like): the growth tree plus exactly `min_cycles` chords.
 int gen_query_walk(const HostGraph &g, uint32_t k, int shape, uint32_t min_cycles, uint64_t seed,
                    HostQuery &q, uint32_t max_attempts) {
     if (k == 0 || k > FM_MAX_QUERY_VERTICES) return fail(FM_ERR_INVALID_ARG, "walk size %u", k);
@@ -825,10 +826,11 @@ int gen_query_walk(const HostGraph &g, uint32_t k, int shape, uint32_t min_cycle
                         }
                 }
             if (chords.size() < std::max<uint32_t>(min_cycles, shape == 2 ? 0u : 1u)) continue;
-            if (shape == 1) {
-                // keep min_cycles chords (at least one), or one more when available
+            if (shape == 1 || shape == 3) {
+                // keep min_cycles chords (at least one); shape 1 sometimes
+                // keeps one more, shape 3 exactly min_cycles
                 uint32_t want = std::max<uint32_t>(min_cycles, 1);
-                if (chords.size() > want && (rng.next() & 1)) want++;
+                if (shape == 1 && chords.size() > want && (rng.next() & 1)) want++;
                 for (uint32_t c = 0; c < want; c++) {
                     uint64_t pick = c + rng.below(chords.size() - c);
                     std::swap(chords[c], chords[pick]);

@@ -144,8 +144,9 @@ int fm_gen_rmat(uint32_t scale, uint64_t undirected_edges, double a, double b,
 
 /* Query extraction by seeded random walk on a symmetric data graph (the
  * walk crosses edges of every label). shape 0: tree (the edge that first
- * reached each vertex); 1: every traversed edge, with at least min_cycles
- * independent cycles; 2: the whole induced subgraph of the walked vertices.
+ * reached each vertex); 1: a weighted-growth tree plus min_cycles (or one
+ * more) induced chords; 2: the whole induced subgraph of the grown vertex
+ * set; 3: the growth tree plus exactly min_cycles chords.
  * Edge labels are copied from the data, so the query has at least one
  * embedding. */
 int fm_gen_query_walk(const fm_graph *g, uint32_t vertices, int shape,
