@@ -72,6 +72,8 @@ def make_candidates(g, cfg: int, seed: int, count: int):
         except fm.ForkMatchError:
             continue
         out.append((shape, n, q))
+        if len(out) % 256 == 0:
+            log(f"candidates: {len(out)}/{count}")
     return out
 
 
@@ -98,6 +100,8 @@ def screen(engine, g, cands, budget: int, want, per_shape=None):
             if len(keep) >= want:
                 log(f"screen: statuses {hist}")
                 return keep, counts
+        r.close()
+        log(f"screen: {b0 + len(chunk)}/{len(cands)} screened, {len(keep)} kept")
     log(f"screen: statuses {hist}")
     return keep, counts
 
@@ -203,13 +207,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--seed", type=int, default=0xF0F2)
-    ap.add_argument("--budget", type=int, default=1 << 26, help="max embeddings per query")
+    ap.add_argument("--budget", type=int, default=0, help="max embeddings per query (default 2^26, 2^24 for config 3)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--queries", type=int, default=0)
     ap.add_argument("--save-queries", default="", help="write the selected queries + manifest here")
     ap.add_argument("--load-queries", default="", help="reuse a saved query set (no screening launches)")
     args = ap.parse_args()
+    if not args.budget:
+        args.budget = (1 << 24) if args.config == 3 else (1 << 26)
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py runs on one MI355X only: this build has no multi-GPU path "
@@ -223,7 +229,8 @@ def main():
     g.upload()
     gi = g.info()
     log(f"graph: V={gi.vertices} E={gi.edges} labels={gi.labels} built in {time.time() - t0:.1f}s")
-    engine = fm.Engine()
+    # config 3 keeps 1,024 tables resident at once: a larger table arena
+    engine = fm.Engine(output_prealloc=(64 << 30) if args.config == 3 else 0)
     ei = engine.info()
     log(f"engine: {ei.device_name.decode()} CUs={ei.compute_units} waves/CU={ei.waves_per_cu} "
         f"grid={ei.grid_waves} banks={ei.banks}x{ei.bank_bytes}B")
@@ -247,8 +254,25 @@ def main():
             chosen += c2
             counts += n2
     else:
-        cands = make_candidates(g, 3, args.seed, int(1.5 * nq))
-        chosen, counts = screen(engine, g, cands, args.budget, nq)
+        # Progressive screening from a seeded pool (at most 32 x nq
+        # candidates) with the 40/40/20 shape caps; shapes that cannot fill
+        # their share within the pool are topped up by the others.
+        caps = {0: int(0.4 * nq), 1: int(0.4 * nq), 2: nq - 2 * int(0.4 * nq)}
+        chosen, counts, pool = [], [], []
+        rnd = 0
+        while len(chosen) < nq and len(pool) < 32 * nq:
+            cands = make_candidates(g, 3, args.seed + 7919 * rnd, 2 * nq)
+            rnd += 1
+            pool += cands
+            left = {k: caps[k] - sum(1 for c in chosen if c[0] == k) for k in caps}
+            c2, n2 = screen(engine, g, cands, args.budget, nq - len(chosen), per_shape=left)
+            chosen += c2
+            counts += n2
+        if len(chosen) < nq:
+            taken = {id(c[2]) for c in chosen}
+            c2, n2 = screen(engine, g, [c for c in pool if id(c[2]) not in taken], args.budget, nq - len(chosen))
+            chosen += c2
+            counts += n2
     log(f"selected {len(chosen)} queries in {time.time() - t1:.1f}s; total embeddings {sum(counts)}")
     queries = [c[2] for c in chosen]
     ns = [c[1] for c in chosen]
@@ -325,23 +349,31 @@ def main():
         }
         workload = "rmat20_plain_32q"
     else:
-        # plain (one launch per query) then batched (one persistent launch)
-        for _ in range(args.warmup):
-            engine.match_batch(g, queries, timeout_ms=300000).close()
+        from forkmatch_amd.autotune import autotune
+        tune = autotune(engine, g, queries, table=True)
+        best = tune.best
+        log(f"autotune: best {best} {tune.best_ms:.2f} ms over {len(tune.trials)} configurations")
+        # plain: one launch per query (same task/root knobs, full grid)
+        plain_opts = {"root_chunk": best["root_chunk"], "task_rows": best["task_rows"]}
+        for q in queries[:16]:
+            engine.match(g, q, timeout_ms=300000, **plain_opts).close()
         tp = time.perf_counter()
         plain_counts = []
         for q in queries:
-            r = engine.match(g, q, timeout_ms=300000)
+            r = engine.match(g, q, timeout_ms=300000, **plain_opts)
             plain_counts.append(r.count(0))
             r.close()
         plain_s = time.perf_counter() - tp
+        # batched: all queries in one persistent launch
+        for _ in range(args.warmup):
+            engine.match_batch(g, queries, timeout_ms=300000, **best).close()
         step_times, kms = [], []
         rb = None
         for _ in range(args.steps):
             if rb is not None:
                 rb.close()
             ts = time.perf_counter()
-            rb = engine.match_batch(g, queries, timeout_ms=300000)
+            rb = engine.match_batch(g, queries, timeout_ms=300000, **best)
             step_times.append(time.perf_counter() - ts)
             kms.append(rb.stats().kernel_ms)
         ok = all(rb.query(i).count == counts[i] == plain_counts[i] and rb.query(i).status == 0
@@ -358,6 +390,10 @@ def main():
             "ms_per_step": ms_step,
             "plain_queries_per_s": len(queries) / plain_s,
             "batched_over_plain": value / (len(queries) / plain_s),
+            "autotune": {"best": best, "best_kernel_ms": tune.best_ms, "trials": len(tune.trials)},
+            "shape_mix": {name: sum(1 for c in chosen if c[0] == k)
+                          for k, name in ((0, "tree"), (1, "cycle+chord"), (2, "near-clique"))},
+            "embeddings_per_step": embeddings,
             "embeddings_per_s": embeddings / (ms_step / 1e3),
             "parity_checks_passed": bool(ok),
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -1,0 +1,62 @@
+"""Autotuner for the batched persistent launch.
+
+The launch shape of ``fm_match_batch`` has three runtime knobs:
+- ``root_chunk``: roots a wave claims at a time. This sets how many waves
+  one query's roots spread over; it is the "wavefronts per query" knob.
+- ``task_rows``: parents per queued task. This is the fork threshold, the
+  replacement of the reference's soft bank limit (SubMaxMulSpill).
+- ``grid_waves``: persistent waves. Fewer waves leave more LDS and L2 per
+  wave; more waves hide more memory latency.
+
+Each candidate configuration is run on the real workload in count mode and
+timed with the kernel's HIP events. The counts of every run must agree, so
+that tuning cannot trade correctness for speed. The fastest configuration
+is returned.
+"""
+from __future__ import annotations
+
+import itertools
+from dataclasses import dataclass, field
+
+
+@dataclass
+class TuneResult:
+    best: dict
+    best_ms: float
+    trials: list = field(default_factory=list)
+
+
+DEFAULT_SPACE = {
+    "root_chunk": [64, 256, 1024, 4096],
+    "task_rows": [64, 256, 1024],
+    "grid_frac": [0.5, 1.0],
+}
+
+
+def autotune(engine, graph, queries, space: dict | None = None, table: bool = False, reps: int = 1,
+             timeout_ms: int = 120000) -> TuneResult:
+    space = space or DEFAULT_SPACE
+    grid = engine.info().grid_waves
+    ref_counts = None
+    trials = []
+    best, best_ms = None, float("inf")
+    keys = list(space)
+    for combo in itertools.product(*(space[k] for k in keys)):
+        cfg = dict(zip(keys, combo))
+        opts = {"root_chunk": cfg.get("root_chunk", 0), "task_rows": cfg.get("task_rows", 0),
+                "grid_waves": max(1, int(grid * cfg.get("grid_frac", 1.0)))}
+        ms = []
+        for _ in range(reps):
+            r = engine.match_batch(graph, queries, table=table, timeout_ms=timeout_ms, **opts)
+            counts = [r.count(i) for i in range(len(queries))]
+            if ref_counts is None:
+                ref_counts = counts
+            elif counts != ref_counts:
+                raise AssertionError(f"autotune: configuration {opts} changed the result")
+            ms.append(r.stats().kernel_ms)
+            r.close()
+        t = min(ms)
+        trials.append({**opts, "kernel_ms": t})
+        if t < best_ms:
+            best, best_ms = opts, t
+    return TuneResult(best, best_ms, trials)

@@ -9,6 +9,7 @@
 #include <cstring>
 #include <numeric>
 #include <thread>
+#include <unordered_map>
 #include <unordered_set>
 
 #include "fm_internal.hpp"
@@ -775,39 +776,55 @@ int gen_query_walk(const HostGraph &g, uint32_t k, int shape, uint32_t min_cycle
                 cur = idx;
             }
         } else {
-            // weighted growth
-            std::vector<std::pair<uint32_t, uint32_t>> cand;  // (vertex, weight)
-            std::vector<std::array<uint32_t, 3>> link;         // first link of cand: (member idx, label, _)
-            while (verts.size() < k) {
-                cand.clear();
-                link.clear();
-                for (uint32_t i = 0; i < verts.size(); i++) {
-                    neighbours(verts[i], nb);
-                    for (auto &p : nb) {
-                        if (std::find(verts.begin(), verts.end(), p.first) != verts.end()) continue;
-                        size_t j = 0;
-                        for (; j < cand.size(); j++)
-                            if (cand[j].first == p.first) break;
-                        if (j == cand.size()) {
-                            if (cand.size() >= 4096) continue;
-                            cand.push_back({p.first, 0});
-                            link.push_back({i, p.second, 0});
-                        }
-                        cand[j].second++;
+            // weighted growth: weight of a candidate = its edges into the set;
+            // updated incrementally as members join. Members with more than
+            // kScan neighbours contribute a random sample of kScan of them.
+            constexpr size_t kScan = 2048;
+            std::unordered_map<uint32_t, uint32_t> where;  // vertex -> index in cand
+            std::vector<std::array<uint32_t, 4>> cand;      // (vertex, weight, member idx, label)
+            auto add_member = [&](uint32_t idx) {
+                neighbours(verts[idx], nb);
+                size_t take = nb.size();
+                if (take > kScan) {
+                    for (size_t i = 0; i < kScan; i++) std::swap(nb[i], nb[i + rng.below(nb.size() - i)]);
+                    take = kScan;
+                }
+                for (size_t i = 0; i < take; i++) {
+                    auto &p = nb[i];
+                    auto it = where.find(p.first);
+                    if (it == where.end()) {
+                        where.emplace(p.first, (uint32_t)cand.size());
+                        cand.push_back({p.first, 1, idx, p.second});
+                    } else {
+                        cand[it->second][1]++;
                     }
                 }
-                if (cand.empty()) break;
+            };
+            add_member(0);
+            while (verts.size() < k) {
                 double total = 0;
-                for (auto &c : cand) total += (double)c.second * c.second * c.second;
+                for (auto &c : cand)
+                    if (c[1]) total += (double)c[1] * c[1] * c[1];
+                if (total == 0) break;
                 double u = rng.uniform() * total;
-                size_t j = 0;
-                for (; j + 1 < cand.size(); j++) {
-                    u -= (double)cand[j].second * cand[j].second * cand[j].second;
+                size_t j = 0, last = 0;
+                for (; j < cand.size(); j++) {
+                    if (!cand[j][1]) continue;
+                    last = j;
+                    u -= (double)cand[j][1] * cand[j][1] * cand[j][1];
                     if (u < 0) break;
                 }
+                if (j == cand.size()) j = last;
                 uint32_t idx = (uint32_t)verts.size();
-                verts.push_back(cand[j].first);
-                tree.push_back({link[j][0], idx, link[j][1]});
+                verts.push_back(cand[j][0]);
+                tree.push_back({cand[j][2], idx, cand[j][3]});
+                cand[j][1] = 0;  // a member is no longer a candidate
+                add_member(idx);
+                // neighbours already in the set must not be drawn again
+                for (uint32_t m = 0; m <= idx; m++) {
+                    auto it = where.find(verts[m]);
+                    if (it != where.end()) cand[it->second][1] = 0;
+                }
             }
         }
         if (verts.size() < k) continue;
