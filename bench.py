@@ -86,7 +86,7 @@ def screen(engine, g, cands, budget: int, want, per_shape=None):
     for b0 in range(0, len(cands), 64):
         chunk = cands[b0:b0 + 64]
         r = engine.match_batch(g, [c[2] for c in chunk], table=False, max_count=budget,
-                               max_partials=budget, timeout_ms=120000, allow_errors=True)
+                               max_partials=min(budget, 1 << 20), timeout_ms=120000, allow_errors=True)
         for j, c in enumerate(chunk):
             o = r.query(j)
             hist[o.status] = hist.get(o.status, 0) + 1
@@ -207,7 +207,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--seed", type=int, default=0xF0F2)
-    ap.add_argument("--budget", type=int, default=0, help="max embeddings per query (default 2^26, 2^24 for config 3)")
+    ap.add_argument("--budget", type=int, default=0, help="max embeddings per query (default 2^26, 2^22 for config 3)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--queries", type=int, default=0)
@@ -215,7 +215,7 @@ def main():
     ap.add_argument("--load-queries", default="", help="reuse a saved query set (no screening launches)")
     args = ap.parse_args()
     if not args.budget:
-        args.budget = (1 << 24) if args.config == 3 else (1 << 26)
+        args.budget = (1 << 22) if args.config == 3 else (1 << 26)
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py runs on one MI355X only: this build has no multi-GPU path "
@@ -230,7 +230,7 @@ def main():
     gi = g.info()
     log(f"graph: V={gi.vertices} E={gi.edges} labels={gi.labels} built in {time.time() - t0:.1f}s")
     # config 3 keeps 1,024 tables resident at once: a larger table arena
-    engine = fm.Engine(output_prealloc=(64 << 30) if args.config == 3 else 0)
+    engine = fm.Engine(output_prealloc=(96 << 30) if args.config == 3 else 0)
     ei = engine.info()
     log(f"engine: {ei.device_name.decode()} CUs={ei.compute_units} waves/CU={ei.waves_per_cu} "
         f"grid={ei.grid_waves} banks={ei.banks}x{ei.bank_bytes}B")
@@ -254,14 +254,14 @@ def main():
             chosen += c2
             counts += n2
     else:
-        # Progressive screening from a seeded pool (at most 32 x nq
+        # Progressive screening from a seeded pool (at most 16 x nq
         # candidates) with the 40/40/20 shape caps; shapes that cannot fill
         # their share within the pool are topped up by the others.
         caps = {0: int(0.4 * nq), 1: int(0.4 * nq), 2: nq - 2 * int(0.4 * nq)}
         chosen, counts, pool = [], [], []
         rnd = 0
-        while len(chosen) < nq and len(pool) < 32 * nq:
-            cands = make_candidates(g, 3, args.seed + 7919 * rnd, 2 * nq)
+        while len(chosen) < nq and len(pool) < 16 * nq:
+            cands = make_candidates(g, 3, args.seed + 7919 * rnd, nq // 2)
             rnd += 1
             pool += cands
             left = {k: caps[k] - sum(1 for c in chosen if c[0] == k) for k in caps}
@@ -278,6 +278,7 @@ def main():
     ns = [c[1] for c in chosen]
 
     # algorithmic bytes (instrumented, untimed)
+    log("instrumented run for the algorithmic bytes")
     if manifest is not None:
         balg = [m["balg"] for m in manifest["queries"]]
         rows_bytes = sum(m["row_bytes"] for m in manifest["queries"])
@@ -350,7 +351,8 @@ def main():
         workload = "rmat20_plain_32q"
     else:
         from forkmatch_amd.autotune import autotune
-        tune = autotune(engine, g, queries, table=True)
+        space = {"root_chunk": [256, 4096], "task_rows": [256, 1024], "grid_frac": [0.5, 1.0]}
+        tune = autotune(engine, g, queries, space=space, table=False, log=log)
         best = tune.best
         log(f"autotune: best {best} {tune.best_ms:.2f} ms over {len(tune.trials)} configurations")
         # plain: one launch per query (same task/root knobs, full grid)

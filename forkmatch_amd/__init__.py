@@ -36,7 +36,7 @@ STATUS_NAMES = {
     300: "FM_ERR_INVALID_ARG", 301: "FM_ERR_DEVICE", 302: "FM_ERR_OUT_OF_MEMORY",
     303: "FM_ERR_POOL_EXHAUSTED", 304: "FM_ERR_QUEUE_FULL", 305: "FM_ERR_TIMEOUT",
     306: "FM_ERR_COUNT_LIMIT", 307: "FM_ERR_QUERY_TOO_LARGE", 308: "FM_ERR_STALE_RESULT",
-    309: "FM_ERR_NOT_SUPPORTED", 310: "FM_ERR_INTERNAL",
+    309: "FM_ERR_NOT_SUPPORTED", 310: "FM_ERR_INTERNAL", 311: "FM_ERR_BUSY",
 }
 
 
@@ -394,6 +394,77 @@ class Engine:
             pass
 
 
+FM_ERR_BUSY = 311
+
+
+class Server:
+    """Continuous-batching server: the graph and the bank pool stay on the
+    device; submitted queries join the running persistent launch.
+
+        with Server(engine, graph) as srv:
+            t = srv.submit(q)          # None when refused (slots / bank budget)
+            for c in srv.poll(): ...   # completions: ticket, count, rows, status
+            rows = srv.table(t); srv.release(t)
+    """
+
+    def __init__(self, engine: Engine, graph: Graph, slots: int = 256, slot_pages: int = 256, bank_budget: int = 0,
+                 idle_ms: int = 20, root_chunk: int = 0, task_rows: int = 0, table: bool = True,
+                 max_launch_ms: int = 0):
+        cfg = L.ServerConfig()
+        cfg.slots, cfg.slot_pages, cfg.bank_budget, cfg.idle_ms = slots, slot_pages, bank_budget, idle_ms
+        cfg.root_chunk, cfg.task_rows, cfg.table = root_chunk, task_rows, 1 if table else 0
+        cfg.max_launch_ms = max_launch_ms
+        h = C.c_void_p()
+        _check(L.lib().fm_server_create(engine._h, graph._h, C.byref(cfg), C.byref(h)))
+        self._h = h
+        self._keep = (engine, graph)
+
+    def submit(self, query: Query, max_rows: int = 0):
+        t = C.c_uint32()
+        rc = L.lib().fm_server_submit(self._h, query._h, max_rows, C.byref(t))
+        if rc == FM_ERR_BUSY:
+            return None
+        _check(rc)
+        return int(t.value)
+
+    def poll(self, max_items: int = 1024):
+        arr = (L.ServerCompletion * max_items)()
+        n = C.c_uint32()
+        _check(L.lib().fm_server_poll(self._h, arr, max_items, C.byref(n)))
+        return [arr[i] for i in range(n.value)]
+
+    def table(self, ticket: int, rows: int, vertices: int, sorted: bool = True) -> np.ndarray:
+        out = np.zeros((max(rows, 1), vertices), dtype=np.uint32)
+        copied = C.c_uint64()
+        _check(L.lib().fm_server_copy_table(self._h, ticket, _p32(out), rows, 1 if sorted else 0, C.byref(copied)))
+        return out[: copied.value]
+
+    def release(self, ticket: int) -> None:
+        _check(L.lib().fm_server_release(self._h, ticket))
+
+    def stats(self) -> L.ServerStats:
+        st = L.ServerStats()
+        _check(L.lib().fm_server_stats_get(self._h, C.byref(st)))
+        return st
+
+    def close(self) -> None:
+        if self._h:
+            L.lib().fm_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class DeviceBuffer:
     """A hipMalloc'd buffer owned from Python (for the intersection kernel)."""
 
@@ -465,5 +536,5 @@ def match(query_path: str, data_path: str, **engine_kw) -> tuple[int, np.ndarray
         return r.count(0), r.table(0)
 
 
-__all__ = ["Graph", "Query", "Engine", "Result", "Plan", "QueryOutcome", "ForkMatchError", "match",
+__all__ = ["Graph", "Query", "Engine", "Result", "Plan", "QueryOutcome", "ForkMatchError", "match", "Server",
            "device_count", "FM_MODE_COUNT", "FM_MODE_TABLE"]

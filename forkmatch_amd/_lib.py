@@ -58,6 +58,21 @@ class LaunchStats(C.Structure):
                 ("status", u32)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("slots", u32), ("slot_pages", u32), ("bank_budget", u32), ("idle_ms", u32), ("root_chunk", u32),
+                ("task_rows", u32), ("table", u32), ("max_launch_ms", u32), ("reserved", u32 * 8)]
+
+
+class ServerCompletion(C.Structure):
+    _fields_ = [("ticket", u32), ("status", u32), ("flags", u32), ("vertices", u32), ("count", u64), ("rows", u64),
+                ("submit_ms", f64), ("done_ms", f64)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", u64), ("completed", u64), ("refused", u64), ("launches", u64), ("in_flight", u32),
+                ("live_banks", u32)]
+
+
 # name -> (restype, argtypes); every function declared in include/forkmatch.h
 SIGNATURES = {
     "fm_last_error": (cstr, []),
@@ -94,6 +109,13 @@ SIGNATURES = {
     "fm_result_stats": (C.c_int, [vp, P(LaunchStats)]),
     "fm_result_copy_table": (C.c_int, [vp, u32, P(u32), u64, C.c_int, P(u64)]),
     "fm_result_destroy": (None, [vp]),
+    "fm_server_create": (C.c_int, [vp, vp, P(ServerConfig), P(vp)]),
+    "fm_server_submit": (C.c_int, [vp, vp, u64, P(u32)]),
+    "fm_server_poll": (C.c_int, [vp, P(ServerCompletion), u32, P(u32)]),
+    "fm_server_copy_table": (C.c_int, [vp, u32, P(u32), u64, C.c_int, P(u64)]),
+    "fm_server_release": (C.c_int, [vp, u32]),
+    "fm_server_stats_get": (C.c_int, [vp, P(ServerStats)]),
+    "fm_server_destroy": (None, [vp]),
     "fm_intersect_pairs": (C.c_int, [vp, vp, vp, vp, vp, vp, u32, vp, vp, vp, P(f64)]),
     "fm_device_alloc": (C.c_int, [u64, P(vp)]),
     "fm_device_free": (C.c_int, [vp]),

@@ -34,7 +34,7 @@ DEFAULT_SPACE = {
 
 
 def autotune(engine, graph, queries, space: dict | None = None, table: bool = False, reps: int = 1,
-             timeout_ms: int = 120000) -> TuneResult:
+             timeout_ms: int = 120000, log=None) -> TuneResult:
     space = space or DEFAULT_SPACE
     grid = engine.info().grid_waves
     ref_counts = None
@@ -57,6 +57,8 @@ def autotune(engine, graph, queries, space: dict | None = None, table: bool = Fa
             r.close()
         t = min(ms)
         trials.append({**opts, "kernel_ms": t})
+        if log:
+            log(f"autotune: {opts} {t:.2f} ms")
         if t < best_ms:
             best, best_ms = opts, t
     return TuneResult(best, best_ms, trials)

@@ -30,6 +30,7 @@
 #include <cstdio>
 #include <cstring>
 #include <numeric>
+#include <thread>
 #include <vector>
 
 #include "fm_internal.hpp"
@@ -76,6 +77,7 @@ struct DQuery {
     unsigned long long max_partials;
     unsigned long long stored;  // intermediate partial embeddings written
     unsigned long long spare;  // partly filled page handed between waves: (id + 1) << 32 | fill
+    long long qpending;        // this query's unfinished tasks (serving completion)
     uint32_t status;
     uint32_t flags;
     uint32_t page_rows;
@@ -125,6 +127,14 @@ struct Ctl {
     alignas(128) uint32_t f_tail;
     alignas(128) int32_t f_avail;
     alignas(128) uint32_t bank_bump;
+    // serving (persistent across serving launches)
+    alignas(128) uint32_t sub_head;      // next submission to admit
+    alignas(128) int32_t n_active;       // admitted, not completed
+    alignas(128) int32_t live_banks;     // banks out of the pool
+    alignas(128) uint32_t srv_page_bump;
+    alignas(128) uint32_t pf_head;       // recycled output pages
+    alignas(128) uint32_t pf_tail;
+    alignas(128) int32_t pf_avail;
 };
 constexpr size_t CTL_RESET_BYTES = offsetof(Ctl, lq);
 
@@ -142,6 +152,34 @@ struct alignas(32) Slot {
 struct alignas(8) FSlot {
     uint32_t seq;
     uint32_t id;
+};
+
+// Serving: host-pinned, coherent memory shared with the host.
+constexpr uint32_t SUB_CAP = 4096;
+struct SrvSub {
+    uint32_t slot;
+    uint32_t ticket;
+    uint32_t pad[2];
+};
+struct SrvQInit {
+    unsigned long long max_rows;
+    uint32_t page_rows;
+    uint32_t max_pages;
+    uint32_t ticket;
+    uint32_t pad[3];
+};
+struct SrvDone {
+    uint32_t count_lo, count_hi, rows_lo, rows_hi;
+    uint32_t status, flags, npages, ticket;  // ticket is written last
+};
+struct SrvShared {
+    uint32_t sub_tail;    // host: submissions written
+    uint32_t shutdown;    // host: stop when idle
+    uint32_t admitted;    // device: submissions admitted
+    uint32_t live_banks;  // device: banks out of the pool (admission budget)
+    uint32_t abort;       // host: end the launch now (shutdown that did not drain)
+    uint32_t pad[27];
+    SrvSub sub[SUB_CAP];
 };
 
 struct DGraph {
@@ -179,7 +217,20 @@ struct KParams {
     uint32_t nbanks;
     uint32_t npages;
     uint32_t nlevels;
-    uint32_t pad;
+    uint32_t serving;
+    // serving
+    SrvShared *srv;
+    const DPlan *srv_plans;    // pinned: plan of each slot
+    const SrvQInit *srv_qinit; // pinned
+    SrvDone *srv_done;         // pinned
+    DPlan *plans_rw;           // device copy of the slot plans (same as plans)
+    uint32_t *slot_pages;      // [max_slots * max_slot_pages]
+    FSlot *pfree_ring;
+    uint32_t pfcap;
+    uint32_t max_slots;
+    uint32_t max_slot_pages;
+    uint32_t pad2;
+    unsigned long long idle_ticks;
 };
 
 // ---------------------------------------------------------------------------
@@ -327,6 +378,7 @@ __device__ uint32_t bank_alloc(const KParams &P, unsigned long long *stats) {
             uint32_t id = s->id;
             st_rel(&s->seq, pos + P.fcap);
             __hip_atomic_store(&P.bank_ref[id], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            atom_add(&c->live_banks, 1);
             stats[ST_BANKS_REUSED]++;
             return id;
         }
@@ -336,6 +388,7 @@ __device__ uint32_t bank_alloc(const KParams &P, unsigned long long *stats) {
         uint32_t id = atom_add(&c->bank_bump, 1u);
         if (id < P.nbanks) {
             __hip_atomic_store(&P.bank_ref[id], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            atom_add(&c->live_banks, 1);
             stats[ST_BANKS_FRESH]++;
             return id;
         }
@@ -352,6 +405,7 @@ __device__ void bank_release(const KParams &P, uint32_t id) {
         s->id = id;
         st_rel(&s->seq, pos + 1);
         atom_add(&c->f_avail, 1);
+        atom_add(&c->live_banks, -1);
     }
 }
 
@@ -375,6 +429,7 @@ __device__ void push_tasks(const KParams &P, uint32_t lvl, uint32_t m, const Tas
     uint32_t pos0 = 0, full = 0;
     if (lane == 0) {
         atom_add(&c->pending, (long long)m);
+        atom_add(&P.qs[t.q].qpending, (long long)m);
         if (bank != ROOT_BANK) atom_add(&P.bank_ref[bank], (int32_t)m);
         pos0 = atom_add(&c->lq[lvl].tail, m);
         uint32_t head = ld_rlx(&c->lq[lvl].head);
@@ -501,10 +556,46 @@ __device__ bool tab_acquire(const KParams &P, WaveTab &w, uint32_t q, uint32_t c
             return true;
         }
     }
-    if (ld_rlx(&Q.npages) >= Q.max_pages || ld_rlx(&P.ctl->page_bump) >= P.npages) return false;
-    if (atom_add(&Q.npages, 1u) >= Q.max_pages) return false;
-    uint32_t id = atom_add(&P.ctl->page_bump, 1u);
-    if (id >= P.npages) return false;
+    if (ld_rlx(&Q.npages) >= Q.max_pages) return false;
+    uint32_t id = NO_PAGE;
+    if (P.serving) {
+        // a recycled page first, then a fresh one
+        Ctl *c = P.ctl;
+        if (ld_rlx(&c->pf_avail) > 0) {
+            if (atom_add(&c->pf_avail, -1) > 0) {
+                uint32_t pos = atom_add(&c->pf_head, 1u);
+                FSlot *fs = &P.pfree_ring[pos & (P.pfcap - 1)];
+                if (!wait_eq(&fs->seq, pos + 1, c)) return false;
+                id = fs->id;
+                st_rel(&fs->seq, pos + P.pfcap);
+            } else {
+                atom_add(&c->pf_avail, 1);
+            }
+        }
+        if (id == NO_PAGE) {
+            if (ld_rlx(&c->srv_page_bump) >= P.npages) return false;
+            id = atom_add(&c->srv_page_bump, 1u);
+            if (id >= P.npages) return false;
+        }
+        uint32_t idx = atom_add(&Q.npages, 1u);
+        if (idx >= Q.max_pages) {
+            // over budget: give the page back
+            uint32_t pos = atom_add(&c->pf_tail, 1u);
+            FSlot *fs = &P.pfree_ring[pos & (P.pfcap - 1)];
+            if (wait_eq(&fs->seq, pos, c)) {
+                fs->id = id;
+                st_rel(&fs->seq, pos + 1);
+                atom_add(&c->pf_avail, 1);
+            }
+            return false;
+        }
+        st_rlx(&P.slot_pages[(size_t)q * P.max_slot_pages + idx], id);
+    } else {
+        if (ld_rlx(&P.ctl->page_bump) >= P.npages) return false;
+        if (atom_add(&Q.npages, 1u) >= Q.max_pages) return false;
+        id = atom_add(&P.ctl->page_bump, 1u);
+        if (id >= P.npages) return false;
+    }
     st_rlx(&P.page_owner[id], q);
     w.page = id;
     w.fill = w.fill0 = 0;
@@ -569,6 +660,8 @@ struct Lds {
     uint32_t cum[WAVE];                 // inclusive scan of driver lengths
     uint32_t didx[WAVE];                // driver requirement index
     uint32_t pad[WAVE];
+    // instrumented launches: per-task sums, flushed once per task (lane 0)
+    unsigned long long inst[4 + MAXN];  // row words, lookups, candidates, probes, partials
 };
 
 // Pushes the rows written since run_begin as tasks of at most task_rows
@@ -663,11 +756,14 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
             uint32_t cnt = __popcll(mask);
             uint32_t rank = __popcll(mask & ((1ull << lane) - 1ull));
             cnt_acc += cnt;
-            if (P.instrument && lane == 0) atom_add(&Q.partials[0], (unsigned long long)cnt);
+            if (P.instrument && lane == 0) s.inst[4 + 0] += cnt;
             if (P.table && cnt)
                 emit_rows(P, wt, t.q, 1, ok, rank, cnt, stats, [&](uint32_t *dst) { dst[0] = v; });
         }
-        if (lane == 0) atom_add(&Q.count, cnt_acc);
+        if (lane == 0) {
+            atom_add(&Q.count, cnt_acc);
+            if (P.table) tab_release(P, wt);
+        }
         return;
     }
 
@@ -745,10 +841,10 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
             unsigned long long wsum = wave_sum_u64(alive ? words : 0ull);
             uint32_t nalive = __popcll(__ballot(alive));
             if (lane == 0) {
-                atom_add(&Q.row_words, wsum);
-                atom_add(&Q.lookups, (unsigned long long)nalive * k);
-                if (root) atom_add(&Q.partials[0], (unsigned long long)nalive);
-                atom_add(&Q.candidates, (unsigned long long)T);
+                s.inst[0] += wsum;
+                s.inst[1] += (unsigned long long)nalive * k;
+                if (root) s.inst[4 + 0] += nalive;
+                s.inst[2] += T;
             }
         }
         __syncthreads();
@@ -878,13 +974,13 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
             }
             if (P.instrument) {
                 unsigned long long np = wave_sum_u64(nprobe);
-                if (lane == 0) atom_add(&Q.probes, np);
+                if (lane == 0) s.inst[3] += np;
             }
             const unsigned long long mask = __ballot(ok);
             const uint32_t cnt = __popcll(mask);
             if (cnt == 0) continue;
             const uint32_t rank = __popcll(mask & ((1ull << lane) - 1ull));
-            if (P.instrument && lane == 0) atom_add(&Q.partials[D], (unsigned long long)cnt);
+            if (P.instrument && lane == 0) s.inst[4 + D] += cnt;
             if (final_level) {
                 cnt_acc += cnt;
                 if (P.table) {
@@ -914,9 +1010,103 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
         __syncthreads();
     }
     if (!final_level && o.bank != NO_BANK) flush_run(P, o, t.q, D, lvl_children, stats);
+    // hand the output page on at the end of every final-level task, so the
+    // query's stored-row count is complete when its last task finishes
+    if (final_level && P.table && lane == 0) tab_release(P, wt);
     if (lane == 0 && cnt_acc) {
         unsigned long long now_cnt = atom_add(&Q.count, cnt_acc) + cnt_acc;
         if (Q.max_count && now_cnt > Q.max_count) query_fail(P, t.q, FM_ERR_COUNT_LIMIT);
+    }
+}
+
+// ---- serving: admission and completion -------------------------------------
+
+__device__ __forceinline__ uint32_t ld_sys(const uint32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ __forceinline__ void st_sys(uint32_t *p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// Lane 0: publishes a finished query to the host. Every task of the query
+// released its rows before its pending decrement, which this wave acquired.
+__device__ void complete_query(const KParams &P, uint32_t q) {
+    DQuery &Q = P.qs[q];
+    SrvDone *d = &P.srv_done[q];
+    unsigned long long count = ld_rlx(&Q.count), rows = ld_rlx(&Q.rows);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");  // table rows visible to the host
+    st_sys(&d->count_lo, (uint32_t)count);
+    st_sys(&d->count_hi, (uint32_t)(count >> 32));
+    st_sys(&d->rows_lo, (uint32_t)rows);
+    st_sys(&d->rows_hi, (uint32_t)(rows >> 32));
+    st_sys(&d->status, ld_rlx(&Q.status));
+    st_sys(&d->flags, __hip_atomic_load(&Q.flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    st_sys(&d->npages, min(ld_rlx(&Q.npages), Q.max_pages));
+    __hip_atomic_store(&d->ticket, Q.pad, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    atom_add(&P.ctl->n_active, -1);
+}
+
+// Lane 0: claims the next submission; returns its slot or NO_PAGE.
+__device__ uint32_t claim_submission(const KParams &P) {
+    Ctl *c = P.ctl;
+    uint32_t h = ld_rlx(&c->sub_head);
+    if (h == ld_sys(&P.srv->sub_tail)) return NO_PAGE;
+    if (!cas_u32(&c->sub_head, h, h + 1)) return NO_PAGE;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
+    return ld_sys(&P.srv->sub[h % SUB_CAP].slot);
+}
+
+// All lanes: admits the query in `slot`: recycles the pages of the slot's
+// previous query, copies the plan into device memory, initialises the
+// query state and queues its root chunks.
+__device__ void admit(const KParams &P, uint32_t slot, unsigned long long *stats) {
+    const uint32_t lane = lane_id();
+    DQuery &Q = P.qs[slot];
+    Ctl *c = P.ctl;
+    // pages of the previous query in this slot go back to the free ring
+    const uint32_t old = min(Q.npages, Q.max_pages);
+    for (uint32_t i = lane; i < old; i += WAVE) {
+        uint32_t id = P.slot_pages[(size_t)slot * P.max_slot_pages + i];
+        uint32_t pos = atom_add(&c->pf_tail, 1u);
+        FSlot *fs = &P.pfree_ring[pos & (P.pfcap - 1)];
+        if (wait_eq(&fs->seq, pos, c)) {
+            fs->id = id;
+            st_rel(&fs->seq, pos + 1);
+            atom_add(&c->pf_avail, 1);
+        }
+    }
+    // plan: pinned host -> device
+    const uint32_t *src = (const uint32_t *)&P.srv_plans[slot];
+    uint32_t *dst = (uint32_t *)&P.plans_rw[slot];
+    for (uint32_t i = lane; i < sizeof(DPlan) / 4; i += WAVE) dst[i] = ld_sys(src + i);
+    const SrvQInit *qi = &P.srv_qinit[slot];
+    if (lane == 0) {
+        DQuery z{};
+        z.max_rows = ((unsigned long long)ld_sys((const uint32_t *)&qi->max_rows + 1) << 32) |
+                     ld_sys((const uint32_t *)&qi->max_rows);
+        z.page_rows = ld_sys(&qi->page_rows);
+        z.max_pages = ld_sys(&qi->max_pages);
+        z.pad = ld_sys(&qi->ticket);  // ticket, echoed in the completion
+        z.qpending = 1;               // held until every root chunk is queued
+        Q = z;
+        atom_add(&c->n_active, 1);
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    const DPlan &pl = P.plans[slot];
+    const uint32_t n = pl.n;
+    const uint32_t chunks = (pl.nroots + P.root_chunk - 1) / P.root_chunk;
+    const uint32_t lvl = max(n, 2u) - 2;
+    for (uint32_t c0 = 0; c0 < chunks; c0 += WAVE) {
+        const uint32_t m = min(WAVE, chunks - c0);
+        const uint32_t first = (c0 + lane) * P.root_chunk;
+        TaskDesc t{slot, 0, ROOT_BANK, first, lane < m ? min(P.root_chunk, pl.nroots - first) : 0, 0, ALL_CANDIDATES};
+        push_tasks(P, lvl, m, t, ROOT_BANK, stats);
+    }
+    if (lane == 0) {
+        st_sys(&P.srv->admitted, ld_rlx(&c->sub_head));
+        st_sys(&P.srv->live_banks, (uint32_t)max(0, ld_rlx(&c->live_banks)));
+        if (atom_add(&Q.qpending, -1ll) == 1) complete_query(P, slot);
     }
 }
 
@@ -928,6 +1118,7 @@ __global__ __launch_bounds__(WAVE) void fm_match_kernel(KParams P) {
     const uint32_t lane = lane_id();
     WaveOut o{NO_BANK, 0, 0};
     uint32_t idle = 0;
+    unsigned long long idle_since = 0;  // lane 0: when this wave ran out of work
     if (lane == 0) wt = WaveTab{0, NO_PAGE, 0, 0, 0};
     unsigned long long stats[ST_N] = {0, 0, 0, 0, 0, 0, 0, 0};
     unsigned long long starve_since = 0;
@@ -950,9 +1141,21 @@ __global__ __launch_bounds__(WAVE) void fm_match_kernel(KParams P) {
                 flag = 2;
             } else {
                 TaskDesc t;
-                if (pop_task(P, t) || claim_root(P, t, stats)) {
+                if (pop_task(P, t) || (!P.serving && claim_root(P, t, stats))) {
                     s_task = t;
                     flag = 0;
+                } else if (P.serving) {
+                    const uint32_t slot = claim_submission(P);
+                    if (slot != NO_PAGE) {
+                        s_task.q = slot;
+                        flag = 3;
+                    } else if (ld_sys(&P.srv->abort)) {
+                        raise_error(P, FM_ERR_TIMEOUT);
+                        flag = 2;
+                    } else if (ld_rlx(&P.ctl->n_active) <= 0 &&
+                               (ld_sys(&P.srv->shutdown) || (idle_since && wall_clock64() - idle_since > P.idle_ticks))) {
+                        flag = 2;
+                    }
                 } else if (ld_rlx(&P.ctl->pending) <= 0) {
                     flag = 2;
                 }
@@ -962,8 +1165,18 @@ __global__ __launch_bounds__(WAVE) void fm_match_kernel(KParams P) {
         __syncthreads();
         const uint32_t flag = s_flag;
         if (flag == 2) break;
+        if (flag == 3) {
+            const uint32_t slot = s_task.q;
+            __syncthreads();
+            admit(P, slot, stats);
+            idle_since = 0;
+            continue;
+        }
         if (flag == 1) {
-            if (idle == 0 && lane == 0) atom_add(&P.ctl->hungry, 1);
+            if (idle == 0 && lane == 0) {
+                atom_add(&P.ctl->hungry, 1);
+                idle_since = wall_clock64();
+            }
             // exponential back-off keeps idle waves off the queue counters
             for (uint32_t i = 0, lim = 1u << min(idle, 5u); i < lim; i++) __builtin_amdgcn_s_sleep(8);
             idle++;
@@ -973,11 +1186,27 @@ __global__ __launch_bounds__(WAVE) void fm_match_kernel(KParams P) {
         idle = 0;
         const TaskDesc t = s_task;
         __syncthreads();
+        if (P.instrument && lane == 0)
+            for (uint32_t i = 0; i < 4 + MAXN; i++) s.inst[i] = 0;
         process_task(P, t, s, o, wt, stats, starve_since, deadline);
         __syncthreads();
+        if (P.instrument && lane == 0) {
+            DQuery &Q = P.qs[t.q];
+            unsigned long long *dst[4] = {&Q.row_words, &Q.lookups, &Q.candidates, &Q.probes};
+            for (uint32_t i = 0; i < 4; i++)
+                if (s.inst[i]) atom_add(dst[i], s.inst[i]);
+            for (uint32_t i = 0; i < MAXN; i++)
+                if (s.inst[4 + i]) atom_add(&Q.partials[i], s.inst[4 + i]);
+        }
         if (lane == 0) {
             if (t.bank != ROOT_BANK && t.bank < P.nbanks) bank_release(P, t.bank);
             atom_add(&P.ctl->pending, -1ll);
+            if (P.serving) {
+                // rows of this task reach memory before the pending decrement
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+                if (__hip_atomic_fetch_add(&P.qs[t.q].qpending, -1ll, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == 1)
+                    complete_query(P, t.q);
+            }
         }
     }
     if (lane == 0) {
@@ -1003,78 +1232,163 @@ __global__ void fm_init_slots_kernel(Slot *ring, uint32_t qcap, uint32_t levels,
 }
 
 // ---------------------------------------------------------------------------
-// Wave64 intersection microkernel: one wave per pair, grid-stride. The
-// shorter list is staged in LDS in tiles; the longer list's slice that
-// overlaps the tile's value range is streamed with coalesced loads and each
-// lane binary-searches its element in the LDS tile. Matches are compacted
-// in order with a ballot, so the output is ascending.
+// Intersection microkernel: |A n B| and the ordered intersection for a batch
+// of pairs of sorted u32 lists. One 256-thread block per pair (grid-stride).
+//  - Similar lengths: merge path. The block walks the merged order of A and
+//    B in tiles of 2048 elements; a tile's A and B slices are staged in LDS
+//    with coalesced loads, each thread finds its 8-element stretch of the
+//    merge path by binary search in LDS and merges it sequentially; a block
+//    scan of the per-thread match counts places the matches in order. Every
+//    element is read from HBM once.
+//  - Skewed lengths (longer >= 32x shorter): the shorter list is read in
+//    chunks of 256; the block narrows the longer list to the chunk's value
+//    range with two binary searches, then every thread binary-searches its
+//    element there. The longer list is only touched along search paths.
 
-constexpr uint32_t ITILE = 2048;
+constexpr uint32_t IB = 256;              // threads per block
+constexpr uint32_t IVT = 8;               // merge-path elements per thread
+constexpr uint32_t ITILE = IB * IVT;      // merge-path elements per tile
+constexpr uint32_t ISKEW = 32;
 
-__global__ __launch_bounds__(WAVE) void fm_intersect_kernel(const uint32_t *__restrict__ data,
-                                                            const uint64_t *a_off, const uint32_t *a_len,
-                                                            const uint64_t *b_off, const uint32_t *b_len,
-                                                            uint32_t npairs, uint32_t *counts,
-                                                            const uint64_t *out_off, uint32_t *out) {
-    __shared__ uint32_t tile[ITILE];
-    const uint32_t lane = lane_id();
+// Number of A elements among the first `diag` elements of merge(A, B), with
+// A taken first on ties.
+__device__ __forceinline__ uint32_t merge_split(const uint32_t *A, uint32_t na, const uint32_t *B, uint32_t nb,
+                                                uint32_t diag) {
+    uint32_t lo = diag > nb ? diag - nb : 0, hi = min(diag, na);
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (A[mid] <= B[diag - 1 - mid]) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Exclusive block scan over IB threads; total returned through *total.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *s_wave, uint32_t &total) {
+    const uint32_t lane = lane_id(), wid = threadIdx.x / WAVE;
+    const uint32_t incl = wave_incl_scan(v);
+    if (lane == WAVE - 1) s_wave[wid] = incl;
+    __syncthreads();
+    uint32_t pre = 0, tot = 0;
+    for (uint32_t w = 0; w < IB / WAVE; w++) {
+        if (w < wid) pre += s_wave[w];
+        tot += s_wave[w];
+    }
+    __syncthreads();
+    total = tot;
+    return pre + incl - v;
+}
+
+__global__ __launch_bounds__(IB) void fm_intersect_kernel(const uint32_t *__restrict__ data, const uint64_t *a_off,
+                                                          const uint32_t *a_len, const uint64_t *b_off,
+                                                          const uint32_t *b_len, uint32_t npairs, uint32_t *counts,
+                                                          const uint64_t *out_off, uint32_t *out) {
+    __shared__ uint32_t sA[ITILE];
+    __shared__ uint32_t sB[ITILE + 1];
+    __shared__ uint32_t s_wave[IB / WAVE];
+    __shared__ uint32_t s_split[4];
+    const uint32_t tid = threadIdx.x;
     for (uint32_t p = blockIdx.x; p < npairs; p += gridDim.x) {
         const uint32_t *A = data + a_off[p];
         const uint32_t *B = data + b_off[p];
-        uint32_t na = a_len[p], nbl = b_len[p];
-        if (na > nbl) {
+        uint32_t na = a_len[p], nb = b_len[p];
+        if (na > nb) {
             const uint32_t *tp = A;
             A = B;
             B = tp;
             uint32_t tn = na;
-            na = nbl;
-            nbl = tn;
+            na = nb;
+            nb = tn;
         }
-        uint32_t found = 0;
         uint32_t *o = out_off ? out + out_off[p] : nullptr;
-        uint32_t bpos = 0;  // B elements below the current tile are skipped
-        for (uint32_t t0 = 0; t0 < na && bpos < nbl; t0 += ITILE) {
-            const uint32_t tn = min(ITILE, na - t0);
-            __syncthreads();
-            for (uint32_t i = lane; i < tn; i += WAVE) tile[i] = A[t0 + i];
-            __syncthreads();
-            const uint32_t lo_v = tile[0], hi_v = tile[tn - 1];
-            // first B index >= lo_v (from bpos), and first > hi_v
-            uint32_t lo = bpos, hi = nbl;
-            while (lo < hi) {
-                uint32_t mid = (lo + hi) >> 1;
-                if (B[mid] < lo_v) lo = mid + 1;
-                else hi = mid;
-            }
-            uint32_t b_begin = lo;
-            hi = nbl;
-            while (lo < hi) {
-                uint32_t mid = (lo + hi) >> 1;
-                if (B[mid] <= hi_v) lo = mid + 1;
-                else hi = mid;
-            }
-            const uint32_t b_end = lo;
-            for (uint32_t j0 = b_begin; j0 < b_end; j0 += WAVE) {
-                const uint32_t j = j0 + lane;
+        uint32_t found = 0;
+        if (na == 0) {
+            // nothing to intersect
+        } else if ((unsigned long long)nb >= (unsigned long long)ISKEW * na) {
+            for (uint32_t c0 = 0; c0 < na; c0 += IB) {
+                const uint32_t cn = min(IB, na - c0);
+                if (tid < 2) {
+                    // B range covering [A[c0], A[c0 + cn - 1]]: thread 0 the
+                    // lower bound of the first, thread 1 the upper bound of the last
+                    const uint32_t v = A[c0 + (tid == 0 ? 0 : cn - 1)];
+                    uint32_t lo = 0, hi = nb;
+                    while (lo < hi) {
+                        uint32_t mid = (lo + hi) >> 1;
+                        if (tid == 0 ? B[mid] < v : B[mid] <= v) lo = mid + 1;
+                        else hi = mid;
+                    }
+                    s_split[tid] = lo;
+                }
+                __syncthreads();
+                const uint32_t blo = s_split[0], bhi = s_split[1];
                 bool hit = false;
                 uint32_t x = 0;
-                if (j < b_end) {
-                    x = B[j];
-                    uint32_t l2 = 0, h2 = tn;
-                    while (l2 < h2) {
-                        uint32_t mid = (l2 + h2) >> 1;
-                        if (tile[mid] < x) l2 = mid + 1;
-                        else h2 = mid;
+                if (tid < cn) {
+                    x = A[c0 + tid];
+                    uint32_t lo = blo, hi = bhi;
+                    while (lo < hi) {
+                        uint32_t mid = (lo + hi) >> 1;
+                        if (B[mid] < x) lo = mid + 1;
+                        else hi = mid;
                     }
-                    hit = l2 < tn && tile[l2] == x;
+                    hit = lo < bhi && B[lo] == x;
                 }
-                const unsigned long long m = __ballot(hit);
-                if (o && hit) o[found + __popcll(m & ((1ull << lane) - 1ull))] = x;
-                found += __popcll(m);
+                uint32_t tot;
+                const uint32_t at = block_excl_scan(hit ? 1u : 0u, s_wave, tot);
+                if (o && hit) o[found + at] = x;
+                found += tot;
             }
-            bpos = b_end;
+        } else {
+            const uint32_t total_path = na + nb;
+            uint32_t a0 = 0, b0 = 0;
+            for (uint32_t d0 = 0; d0 < total_path; d0 += ITILE) {
+                const uint32_t d1 = min(d0 + ITILE, total_path);
+                if (tid == 0) {
+                    const uint32_t a1 = merge_split(A, na, B, nb, d1);
+                    s_split[0] = a1;
+                    s_split[1] = d1 - a1;
+                }
+                __syncthreads();
+                const uint32_t a1 = s_split[0], b1 = s_split[1];
+                const uint32_t ta = a1 - a0, tb = b1 - b0;
+                const uint32_t tbx = tb + (b1 < nb ? 1u : 0u);  // one B element past the tile
+                for (uint32_t i = tid; i < ta; i += IB) sA[i] = A[a0 + i];
+                for (uint32_t i = tid; i < tbx; i += IB) sB[i] = B[b0 + i];
+                __syncthreads();
+                // this thread's stretch of the tile's merge path
+                const uint32_t dl = min(tid * IVT, ta + tb);
+                const uint32_t steps = min(IVT, ta + tb - dl);
+                uint32_t i = merge_split(sA, ta, sB, tb, dl);
+                uint32_t j = dl - i;
+                uint32_t m[IVT];
+                uint32_t c = 0;
+#pragma unroll
+                for (uint32_t st = 0; st < IVT; st++) {
+                    if (st < steps) {
+                        if (i < ta && (j >= tb || sA[i] <= sB[j])) {
+                            // consuming A[i]; an equal B element is the next one on B's side
+                            if (j < tbx && sB[j] == sA[i]) m[c++] = sA[i];
+                            i++;
+                        } else {
+                            j++;
+                        }
+                    }
+                }
+                uint32_t tot;
+                const uint32_t at = block_excl_scan(c, s_wave, tot);
+                if (o) {
+#pragma unroll
+                    for (uint32_t k = 0; k < IVT; k++)
+                        if (k < c) o[found + at + k] = m[k];
+                }
+                found += tot;
+                a0 = a1;
+                b0 = b1;
+                __syncthreads();
+            }
         }
-        if (lane == 0) counts[p] = found;
+        if (tid == 0) counts[p] = found;
+        __syncthreads();
     }
 }
 
@@ -1123,6 +1437,9 @@ struct fm_engine {
     uint32_t max_queries = 0;
     uint64_t epoch = 0;
     bool dirty = true;  // queues/pool need a full re-initialisation
+    fm_server *server = nullptr;  // a server owns the engine while it lives
+    FSlot *pfree_ring = nullptr;
+    uint32_t pfcap = 0;
 };
 
 struct fm_result {
@@ -1143,6 +1460,11 @@ static int full_reset(fm_engine *e) {
     hipLaunchKernelGGL(fm_init_slots_kernel, dim3(blocks), dim3(256), 0, e->stream, e->ring, e->qcap,
                        NLEVELS, e->free_ring, e->fcap);
     HIP_TRY(hipGetLastError());
+    if (e->pfree_ring) {
+        hipLaunchKernelGGL(fm_init_slots_kernel, dim3((e->pfcap + 255) / 256), dim3(256), 0, e->stream, e->ring, 0u,
+                           0u, e->pfree_ring, e->pfcap);
+        HIP_TRY(hipGetLastError());
+    }
     HIP_TRY(hipStreamSynchronize(e->stream));
     e->dirty = false;
     return FM_OK;
@@ -1322,6 +1644,7 @@ void fm_engine_destroy(fm_engine *e) {
     (void)hipFree(e->ctl);
     (void)hipFree(e->page_owner);
     (void)hipFree(e->page_fill);
+    (void)hipFree(e->pfree_ring);
     (void)hipFree(e->d_plans);
     (void)hipFree(e->d_qs);
     if (e->ev0) (void)hipEventDestroy(e->ev0);
@@ -1403,6 +1726,7 @@ int fm_match_batch(fm_engine *e, const fm_graph *g, const fm_query *const *qs, u
     if (!e || !g || !out || (nq && !qs)) return fail(FM_ERR_INVALID_ARG, "null argument");
     if (g->device != e->device) return fail(FM_ERR_INVALID_ARG, "graph is not uploaded to the engine's device");
     if (nq == 0 || nq > e->max_queries) return fail(FM_ERR_INVALID_ARG, "query count %u out of range", nq);
+    if (e->server) return fail(FM_ERR_BUSY, "the engine is owned by a server");
     fm_match_opts o{};
     if (opts) o = *opts;
     if (o.timeout_ms == 0) o.timeout_ms = 120000;
@@ -1636,6 +1960,357 @@ int fm_result_copy_table(const fm_result *r, uint32_t qi, uint32_t *dst, uint64_
 
 void fm_result_destroy(fm_result *r) { delete r; }
 
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Serving host side.
+
+struct fm_server {
+    fm_engine *e = nullptr;
+    const fm_graph *g = nullptr;
+    fm_server_config cfg{};
+    SrvShared *sh = nullptr;      // pinned, coherent
+    DPlan *hplans = nullptr;      // pinned
+    SrvQInit *hqinit = nullptr;   // pinned
+    SrvDone *hdone = nullptr;     // pinned
+    uint32_t *slot_pages = nullptr;
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t kernel_done = nullptr;
+    bool launched = false;
+    std::chrono::steady_clock::time_point t0;
+    std::vector<uint32_t> free_slots;
+    std::vector<uint32_t> slot_ticket;   // ticket in the slot (0 = none)
+    std::vector<uint8_t> slot_state;     // 0 free, 1 in flight, 2 done
+    std::vector<double> slot_submit_ms;
+    std::vector<uint32_t> slot_n;
+    std::vector<uint32_t> page_fill_cache;
+    uint32_t next_ticket = 1;
+    uint32_t sub_tail = 0;
+    fm_server_stats st{};
+};
+
+static double server_ms(const fm_server *s) {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - s->t0).count();
+}
+
+static int server_launch(fm_server *s) {
+    fm_engine *e = s->e;
+    HIP_TRY(hipSetDevice(e->device));
+    if (e->dirty) {
+        int rc = full_reset(e);
+        if (rc) return rc;
+    }
+    Ctl reset{};
+    reset.timeout_ticks = (unsigned long long)(s->cfg.max_launch_ms * (e->wall_hz / 1000.0));
+    HIP_TRY(hipMemcpyAsync(e->ctl, &reset, CTL_RESET_BYTES, hipMemcpyHostToDevice, e->stream));
+    KParams P{};
+    const fm_graph *g = s->g;
+    P.g.V = g->h.V;
+    P.g.L = g->h.L();
+    P.g.ptr[0] = g->d_ptr[0];
+    P.g.ptr[1] = g->d_ptr[1];
+    P.g.col = g->d_col[0];
+    P.g.roots = g->d_roots[0];
+    P.plans = e->d_plans;
+    P.plans_rw = e->d_plans;
+    P.qs = e->d_qs;
+    P.ctl = e->ctl;
+    P.ring = e->ring;
+    P.free_ring = e->free_ring;
+    P.bank_ref = e->bank_ref;
+    P.banks = e->banks;
+    P.pages = e->pages;
+    P.page_owner = e->page_owner;
+    P.page_fill = e->page_fill;
+    P.bank_words = e->bank_words;
+    P.page_words = e->page_words;
+    P.nq = s->cfg.slots;
+    P.total_root_chunks = 0;
+    P.root_chunk = s->cfg.root_chunk;
+    P.task_rows = s->cfg.task_rows;
+    P.table = s->cfg.table ? 1 : 0;
+    P.qcap = e->qcap;
+    P.fcap = e->fcap;
+    P.nbanks = e->nbanks;
+    P.npages = e->npages;
+    P.nlevels = NLEVELS - 1;
+    P.starve_ticks = (unsigned long long)(e->wall_hz * 2.0);
+    P.timeout_ticks = reset.timeout_ticks;
+    P.serving = 1;
+    P.srv = s->sh;
+    P.srv_plans = s->hplans;
+    P.srv_qinit = s->hqinit;
+    P.srv_done = s->hdone;
+    P.slot_pages = s->slot_pages;
+    P.pfree_ring = e->pfree_ring;
+    P.pfcap = e->pfcap;
+    P.max_slots = s->cfg.slots;
+    P.max_slot_pages = s->cfg.slot_pages;
+    P.idle_ticks = (unsigned long long)(s->cfg.idle_ms * (e->wall_hz / 1000.0));
+    hipLaunchKernelGGL(fm_match_kernel, dim3(e->grid_waves), dim3(WAVE), 0, e->stream, P);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(s->kernel_done, e->stream));
+    s->launched = true;
+    s->st.launches++;
+    return FM_OK;
+}
+
+// Relaunches the persistent kernel when it has gone idle and work waits.
+static int server_pump(fm_server *s) {
+    if (s->launched) {
+        hipError_t q = hipEventQuery(s->kernel_done);
+        if (q == hipErrorNotReady) return FM_OK;
+        if (q != hipSuccess) return fail(FM_ERR_DEVICE, "serving kernel failed: %s", hipGetErrorString(q));
+        s->launched = false;
+        Ctl back{};
+        HIP_TRY(hipMemcpy(&back, s->e->ctl, CTL_RESET_BYTES, hipMemcpyDeviceToHost));
+        if (back.error) {
+            s->e->dirty = true;
+            return fail((int)back.error, "serving kernel aborted with status %u", back.error);
+        }
+    }
+    uint32_t admitted = __atomic_load_n(&s->sh->admitted, __ATOMIC_ACQUIRE);
+    bool in_flight = false;
+    for (auto st : s->slot_state) in_flight |= st == 1;
+    if (admitted != s->sub_tail || in_flight) return server_launch(s);
+    return FM_OK;
+}
+
+extern "C" {
+
+int fm_server_create(fm_engine *e, const fm_graph *g, const fm_server_config *cfg, fm_server **out) {
+    if (!e || !g || !out) return fail(FM_ERR_INVALID_ARG, "null argument");
+    if (e->server) return fail(FM_ERR_BUSY, "the engine already has a server");
+    if (g->device != e->device) return fail(FM_ERR_INVALID_ARG, "graph is not uploaded to the engine's device");
+    auto *s = new fm_server();
+    s->e = e;
+    s->g = g;
+    if (cfg) s->cfg = *cfg;
+    if (!s->cfg.slots) s->cfg.slots = 256;
+    if (!s->cfg.slot_pages) s->cfg.slot_pages = 256;
+    if (!s->cfg.idle_ms) s->cfg.idle_ms = 20;
+    if (!s->cfg.root_chunk) s->cfg.root_chunk = TooLargeThreshold;
+    if (!s->cfg.task_rows) s->cfg.task_rows = SubMaxMulSpill;
+    if (!s->cfg.bank_budget) s->cfg.bank_budget = e->nbanks;
+    if (!s->cfg.max_launch_ms) s->cfg.max_launch_ms = 120000;
+    if (!cfg) s->cfg.table = 1;
+    const uint32_t S = s->cfg.slots;
+    auto fail_out = [&](int rc) {
+        s->e = nullptr;  // not owned yet
+        delete s;
+        return rc;
+    };
+    if (S > e->max_queries) return fail_out(fail(FM_ERR_INVALID_ARG, "slots %u exceed the engine's max_queries", S));
+    HIP_TRY(hipSetDevice(e->device));
+    if (hipHostMalloc((void **)&s->sh, sizeof(SrvShared), hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess ||
+        hipHostMalloc((void **)&s->hplans, S * sizeof(DPlan), hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess ||
+        hipHostMalloc((void **)&s->hqinit, S * sizeof(SrvQInit), hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess ||
+        hipHostMalloc((void **)&s->hdone, S * sizeof(SrvDone), hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess)
+        return fail_out(fail(FM_ERR_OUT_OF_MEMORY, "pinned serving buffers"));
+    std::memset(s->sh, 0, sizeof(SrvShared));
+    std::memset(s->hplans, 0, S * sizeof(DPlan));
+    std::memset(s->hqinit, 0, S * sizeof(SrvQInit));
+    std::memset(s->hdone, 0, S * sizeof(SrvDone));
+    if (hipMalloc((void **)&s->slot_pages, (size_t)S * s->cfg.slot_pages * 4) != hipSuccess ||
+        hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&s->kernel_done, hipEventDisableTiming) != hipSuccess)
+        return fail_out(fail(FM_ERR_OUT_OF_MEMORY, "serving device buffers"));
+    // slot state on the device starts zeroed (no pages to recycle)
+    if (S > e->q_cap) {
+        (void)hipFree(e->d_plans);
+        (void)hipFree(e->d_qs);
+        e->d_plans = nullptr;
+        e->d_qs = nullptr;
+        HIP_TRY(hipMalloc((void **)&e->d_plans, S * sizeof(DPlan)));
+        HIP_TRY(hipMalloc((void **)&e->d_qs, S * sizeof(DQuery)));
+        e->q_cap = S;
+    }
+    HIP_TRY(hipMemset(e->d_qs, 0, S * sizeof(DQuery)));
+    if (!e->pfree_ring) {
+        e->pfcap = pow2_at_least(e->npages);
+        HIP_TRY(hipMalloc((void **)&e->pfree_ring, (size_t)e->pfcap * sizeof(FSlot)));
+    }
+    int rc = full_reset(e);  // also clears the serving counters
+    if (rc) return fail_out(rc);
+    s->slot_ticket.assign(S, 0);
+    s->slot_state.assign(S, 0);
+    s->slot_submit_ms.assign(S, 0);
+    s->slot_n.assign(S, 0);
+    for (uint32_t i = 0; i < S; i++) s->free_slots.push_back(S - 1 - i);
+    s->t0 = std::chrono::steady_clock::now();
+    e->server = s;
+    *out = s;
+    return FM_OK;
+}
+
+static int slot_of(const fm_server *s, uint32_t ticket) {
+    for (size_t i = 0; i < s->slot_ticket.size(); i++)
+        if (s->slot_ticket[i] == ticket && s->slot_state[i]) return (int)i;
+    return -1;
+}
+
+int fm_server_submit(fm_server *s, const fm_query *q, uint64_t max_rows, uint32_t *ticket) {
+    if (!s || !q || !ticket) return fail(FM_ERR_INVALID_ARG, "null argument");
+    uint32_t live = __atomic_load_n(&s->sh->live_banks, __ATOMIC_RELAXED);
+    uint32_t admitted = __atomic_load_n(&s->sh->admitted, __ATOMIC_ACQUIRE);
+    if (s->free_slots.empty() || live >= s->cfg.bank_budget || s->sub_tail - admitted >= SUB_CAP) {
+        s->st.refused++;
+        return fail(FM_ERR_BUSY, "no slot or bank budget available");
+    }
+    uint32_t slot = s->free_slots.back();
+    DPlan dp;
+    int rc = make_dplan(s->g, q->h, false, dp);
+    if (rc) return rc;
+    const uint32_t n = dp.n;
+    SrvQInit qi{};
+    qi.page_rows = (uint32_t)(s->e->page_words / n);
+    qi.max_pages = s->cfg.table ? s->cfg.slot_pages : 0;
+    uint64_t cap = (uint64_t)qi.max_pages * qi.page_rows;
+    qi.max_rows = s->cfg.table ? (max_rows ? std::min<uint64_t>(max_rows, cap) : cap) : 0;
+    uint32_t t = s->next_ticket++;
+    if (s->next_ticket == 0) s->next_ticket = 1;
+    qi.ticket = t;
+    s->free_slots.pop_back();
+    std::memcpy(&s->hplans[slot], &dp, sizeof dp);
+    std::memcpy(&s->hqinit[slot], &qi, sizeof qi);
+    __atomic_store_n(&s->hdone[slot].ticket, 0u, __ATOMIC_RELAXED);
+    SrvSub sub{slot, t, {0, 0}};
+    std::memcpy(&s->sh->sub[s->sub_tail % SUB_CAP], &sub, sizeof sub);
+    s->sub_tail++;
+    __atomic_store_n(&s->sh->sub_tail, s->sub_tail, __ATOMIC_RELEASE);
+    s->slot_ticket[slot] = t;
+    s->slot_state[slot] = 1;
+    s->slot_submit_ms[slot] = server_ms(s);
+    s->slot_n[slot] = n;
+    s->st.submitted++;
+    *ticket = t;
+    return server_pump(s);
+}
+
+int fm_server_poll(fm_server *s, fm_server_completion *out, uint32_t max, uint32_t *n) {
+    if (!s || !n) return fail(FM_ERR_INVALID_ARG, "null argument");
+    *n = 0;
+    for (size_t i = 0; i < s->slot_state.size() && *n < max; i++) {
+        if (s->slot_state[i] != 1) continue;
+        SrvDone *d = &s->hdone[i];
+        if (__atomic_load_n(&d->ticket, __ATOMIC_ACQUIRE) != s->slot_ticket[i]) continue;
+        fm_server_completion c{};
+        c.ticket = s->slot_ticket[i];
+        c.status = __atomic_load_n(&d->status, __ATOMIC_RELAXED);
+        c.flags = __atomic_load_n(&d->flags, __ATOMIC_RELAXED);
+        c.vertices = s->slot_n[i];
+        c.count = ((uint64_t)__atomic_load_n(&d->count_hi, __ATOMIC_RELAXED) << 32) | d->count_lo;
+        c.rows = ((uint64_t)__atomic_load_n(&d->rows_hi, __ATOMIC_RELAXED) << 32) | d->rows_lo;
+        if (s->cfg.table && c.count > c.rows) c.flags |= FM_RESULT_TRUNCATED;
+        c.submit_ms = s->slot_submit_ms[i];
+        c.done_ms = server_ms(s);
+        s->slot_state[i] = 2;
+        s->st.completed++;
+        if (out) out[*n] = c;
+        (*n)++;
+    }
+    return server_pump(s);
+}
+
+int fm_server_copy_table(fm_server *s, uint32_t ticket, uint32_t *dst, uint64_t max_rows, int sorted,
+                         uint64_t *copied) {
+    if (!s) return fail(FM_ERR_INVALID_ARG, "null server");
+    int slot = slot_of(s, ticket);
+    if (slot < 0 || s->slot_state[slot] != 2) return fail(FM_ERR_INVALID_ARG, "ticket %u is not complete", ticket);
+    if (copied) *copied = 0;
+    fm_engine *e = s->e;
+    const SrvDone *d = &s->hdone[slot];
+    const uint32_t n = s->slot_n[slot];
+    uint64_t rows = std::min<uint64_t>(((uint64_t)d->rows_hi << 32) | d->rows_lo, max_rows);
+    uint32_t npg = d->npages;
+    if (rows == 0 || npg == 0) return FM_OK;
+    if (!dst) return fail(FM_ERR_INVALID_ARG, "null destination");
+    HIP_TRY(hipSetDevice(e->device));
+    std::vector<uint32_t> ids(npg);
+    HIP_TRY(hipMemcpyAsync(ids.data(), s->slot_pages + (size_t)slot * s->cfg.slot_pages, npg * 4ull,
+                           hipMemcpyDeviceToHost, s->copy_stream));
+    s->page_fill_cache.resize(e->npages);
+    HIP_TRY(hipMemcpyAsync(s->page_fill_cache.data(), e->page_fill, e->npages * 4ull, hipMemcpyDeviceToHost,
+                           s->copy_stream));
+    HIP_TRY(hipStreamSynchronize(s->copy_stream));
+    uint64_t done = 0;
+    for (uint32_t k = 0; k < npg && done < rows; k++) {
+        uint32_t p = ids[k];
+        if (p >= e->npages) continue;
+        uint64_t take = std::min<uint64_t>(s->page_fill_cache[p], rows - done);
+        if (!take) continue;
+        HIP_TRY(hipMemcpyAsync(dst + done * n, e->pages + (size_t)p * e->page_words, take * n * 4ull,
+                               hipMemcpyDeviceToHost, s->copy_stream));
+        done += take;
+    }
+    HIP_TRY(hipStreamSynchronize(s->copy_stream));
+    if (sorted && done > 1) {
+        std::vector<uint64_t> idx(done);
+        std::iota(idx.begin(), idx.end(), 0);
+        std::sort(idx.begin(), idx.end(), [&](uint64_t a, uint64_t b) {
+            const uint32_t *x = dst + a * n, *y = dst + b * n;
+            for (uint32_t c = 0; c < n; c++)
+                if (x[c] != y[c]) return x[c] < y[c];
+            return false;
+        });
+        std::vector<uint32_t> tmp(done * n);
+        for (uint64_t i = 0; i < done; i++) std::memcpy(&tmp[i * n], dst + idx[i] * n, n * 4);
+        std::memcpy(dst, tmp.data(), done * n * 4);
+    }
+    if (copied) *copied = done;
+    return FM_OK;
+}
+
+int fm_server_release(fm_server *s, uint32_t ticket) {
+    if (!s) return fail(FM_ERR_INVALID_ARG, "null server");
+    int slot = slot_of(s, ticket);
+    if (slot < 0 || s->slot_state[slot] != 2) return fail(FM_ERR_INVALID_ARG, "ticket %u is not complete", ticket);
+    s->slot_state[slot] = 0;
+    s->slot_ticket[slot] = 0;
+    s->free_slots.push_back((uint32_t)slot);
+    return FM_OK;
+}
+
+int fm_server_stats_get(fm_server *s, fm_server_stats *out) {
+    if (!s || !out) return fail(FM_ERR_INVALID_ARG, "null argument");
+    *out = s->st;
+    uint32_t inflight = 0;
+    for (auto st : s->slot_state) inflight += st == 1;
+    out->in_flight = inflight;
+    out->live_banks = __atomic_load_n(&s->sh->live_banks, __ATOMIC_RELAXED);
+    return FM_OK;
+}
+
+void fm_server_destroy(fm_server *s) {
+    if (!s) return;
+    if (s->sh) __atomic_store_n(&s->sh->shutdown, 1u, __ATOMIC_RELEASE);
+    if (s->launched) {
+        // give in-flight queries 10 s to drain, then stop the launch
+        auto t_stop = std::chrono::steady_clock::now() + std::chrono::seconds(10);
+        while (hipEventQuery(s->kernel_done) == hipErrorNotReady && std::chrono::steady_clock::now() < t_stop)
+            std::this_thread::sleep_for(std::chrono::milliseconds(1));
+        if (hipEventQuery(s->kernel_done) == hipErrorNotReady) __atomic_store_n(&s->sh->abort, 1u, __ATOMIC_RELEASE);
+        (void)hipEventSynchronize(s->kernel_done);
+    }
+    if (s->e) {
+        (void)hipSetDevice(s->e->device);
+        s->e->server = nullptr;
+        s->e->dirty = true;  // batch launches re-initialise the shared pools
+    }
+    if (s->copy_stream) (void)hipStreamDestroy(s->copy_stream);
+    if (s->kernel_done) (void)hipEventDestroy(s->kernel_done);
+    (void)hipFree(s->slot_pages);
+    (void)hipHostFree(s->sh);
+    (void)hipHostFree(s->hplans);
+    (void)hipHostFree(s->hqinit);
+    (void)hipHostFree(s->hdone);
+    delete s;
+}
+
+}  // extern "C"
+
+extern "C" {
+
 int fm_intersect_pairs(fm_engine *e, const uint32_t *data, const uint64_t *a_off, const uint32_t *a_len,
                        const uint64_t *b_off, const uint32_t *b_len, uint32_t npairs, uint32_t *counts,
                        const uint64_t *out_off, uint32_t *out, double *kernel_ms) {
@@ -1645,12 +2320,12 @@ int fm_intersect_pairs(fm_engine *e, const uint32_t *data, const uint64_t *a_off
     if (npairs == 0) return FM_OK;
     HIP_TRY(hipSetDevice(e->device));
     int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fm_intersect_kernel, WAVE, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fm_intersect_kernel, IB, 0) != hipSuccess ||
         blocks <= 0)
-        blocks = 8;
+        blocks = 4;
     uint32_t grid = std::min<uint32_t>(npairs, (uint32_t)(blocks * e->cu_count));
     HIP_TRY(hipEventRecord(e->ev0, e->stream));
-    hipLaunchKernelGGL(fm_intersect_kernel, dim3(grid), dim3(WAVE), 0, e->stream, data, a_off, a_len, b_off, b_len,
+    hipLaunchKernelGGL(fm_intersect_kernel, dim3(grid), dim3(IB), 0, e->stream, data, a_off, a_len, b_off, b_len,
                        npairs, counts, out_off, out);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(e->ev1, e->stream));

@@ -292,6 +292,62 @@ int fm_result_copy_table(const fm_result *r, uint32_t qi, uint32_t *dst,
                          uint64_t max_rows, int sorted, uint64_t *copied);
 void fm_result_destroy(fm_result *r);
 
+/* ---- serving ---------------------------------------------------------------
+ * No reference counterpart (the reference matches one query per process).
+ * A server keeps the graph and the bank pool resident and runs one
+ * persistent matcher launch that admits queries while others are running
+ * (continuous batching): the host writes a query into a slot and appends it
+ * to a submission ring in pinned memory; an idle wave admits it by queueing
+ * its root chunks, so its work interleaves with the queries already on the
+ * device. Completion (count, rows, status) is published to pinned memory.
+ * The launch ends after `idle_ms` without work and is relaunched by the next
+ * submit or poll. Admission is refused (FM_ERR_BUSY) when all slots are in
+ * use or the live banks exceed `bank_budget`. */
+#define FM_ERR_BUSY 311
+typedef struct fm_server fm_server;
+typedef struct fm_server_config {
+    uint32_t slots;          /* concurrent queries (default 256) */
+    uint32_t slot_pages;     /* output pages per query (default 256) */
+    uint32_t bank_budget;    /* admit while live banks < budget (0 = pool) */
+    uint32_t idle_ms;        /* launch lifetime without work (default 20) */
+    uint32_t root_chunk;     /* roots per root task (default TooLargeThreshold) */
+    uint32_t task_rows;      /* parents per task (default SubMaxMulSpill) */
+    uint32_t table;          /* 1: store tables (default), 0: count only */
+    uint32_t max_launch_ms;  /* lifetime cap of one launch (default 120000) */
+    uint32_t reserved[8];
+} fm_server_config;
+typedef struct fm_server_completion {
+    uint32_t ticket;         /* returned by fm_server_submit */
+    uint32_t status;
+    uint32_t flags;
+    uint32_t vertices;
+    uint64_t count;
+    uint64_t rows;
+    double submit_ms;        /* host clock, ms since the server started */
+    double done_ms;          /* host clock when the completion was seen */
+} fm_server_completion;
+typedef struct fm_server_stats {
+    uint64_t submitted, completed, refused, launches;
+    uint32_t in_flight, live_banks;
+} fm_server_stats;
+int fm_server_create(fm_engine *e, const fm_graph *g,
+                     const fm_server_config *cfg, fm_server **out);
+/* Queues a query (max_rows = table capacity, 0 = slot pages). The ticket
+ * names the request until fm_server_release. */
+int fm_server_submit(fm_server *s, const fm_query *q, uint64_t max_rows,
+                     uint32_t *ticket);
+/* Returns up to `max` completions not returned before; non-blocking. */
+int fm_server_poll(fm_server *s, fm_server_completion *out, uint32_t max,
+                   uint32_t *n);
+/* Copies a completed request's table (query-vertex column order). */
+int fm_server_copy_table(fm_server *s, uint32_t ticket, uint32_t *dst,
+                         uint64_t max_rows, int sorted, uint64_t *copied);
+/* Frees the request's slot (its pages are recycled when the slot is reused). */
+int fm_server_release(fm_server *s, uint32_t ticket);
+int fm_server_stats_get(fm_server *s, fm_server_stats *out);
+/* Waits for the in-flight requests and stops the launch. */
+void fm_server_destroy(fm_server *s);
+
 /* ---- intersection microkernel -------------------------------------------
  * Replaces the warp-cooperative intersect<CoopSize,Decomp> family
  * (src/intersect.cuh:31-191). pairs of sorted u32 lists inside one device

@@ -282,3 +282,41 @@ def test_hub_rows_are_split_across_waves(engine):
         assert r.count(0) == ref.count
         assert np.array_equal(r.table(0), ref.table)
     assert split > 0
+
+
+def test_serving_results_equal_alone_and_in_the_mix(er):
+    """Continuous batching: queries submitted while others run are admitted
+    into the running launch; each result equals the query matched alone."""
+    import time
+    g, og = er
+    qs = [fm.Query.random_walk(g, 3 + i % 5, shape=i % 3, min_cycles=1 if i % 3 == 1 else 0, seed=900 + i)
+          for i in range(40)]
+    refs = [orc.match(og, q.vertices, q.edges()) for q in qs]
+    with fm.Engine(soln_prealloc=1 << 29, output_prealloc=1 << 29, bank_size=1 << 13, output_page_bytes=1 << 16) as e:
+        with fm.Server(e, g, slots=16, slot_pages=64, idle_ms=5, max_launch_ms=60000) as srv:
+            pending = {}
+            done = {}
+            i = 0
+            t_end = time.time() + 60
+            while (i < len(qs) or pending) and time.time() < t_end:
+                # submit in bursts while earlier queries are still running
+                for _ in range(3):
+                    if i < len(qs):
+                        t = srv.submit(qs[i])
+                        if t is None:
+                            break
+                        pending[t] = i
+                        i += 1
+                for c in srv.poll():
+                    k = pending.pop(c.ticket)
+                    assert c.status == 0, c.status
+                    tab = srv.table(c.ticket, c.rows, c.vertices)
+                    done[k] = (c.count, tab)
+                    srv.release(c.ticket)
+                time.sleep(0.001)
+            st = srv.stats()
+    assert len(done) == len(qs)
+    assert st.launches >= 1
+    for k, ref in enumerate(refs):
+        assert done[k][0] == ref.count, k
+        assert np.array_equal(done[k][1], ref.table), k
