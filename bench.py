@@ -86,7 +86,7 @@ def screen(engine, g, cands, budget: int, want, per_shape=None):
     for b0 in range(0, len(cands), 64):
         chunk = cands[b0:b0 + 64]
         r = engine.match_batch(g, [c[2] for c in chunk], table=False, max_count=budget,
-                               max_partials=min(budget, 1 << 20), timeout_ms=120000, allow_errors=True)
+                               max_partials=budget, timeout_ms=120000, allow_errors=True)
         for j, c in enumerate(chunk):
             o = r.query(j)
             hist[o.status] = hist.get(o.status, 0) + 1
@@ -254,13 +254,13 @@ def main():
             chosen += c2
             counts += n2
     else:
-        # Progressive screening from a seeded pool (at most 16 x nq
+        # Progressive screening from a seeded pool (at most 48 x nq
         # candidates) with the 40/40/20 shape caps; shapes that cannot fill
         # their share within the pool are topped up by the others.
         caps = {0: int(0.4 * nq), 1: int(0.4 * nq), 2: nq - 2 * int(0.4 * nq)}
         chosen, counts, pool = [], [], []
         rnd = 0
-        while len(chosen) < nq and len(pool) < 16 * nq:
+        while len(chosen) < nq and len(pool) < 48 * nq:
             cands = make_candidates(g, 3, args.seed + 7919 * rnd, nq // 2)
             rnd += 1
             pool += cands

@@ -1283,8 +1283,8 @@ __global__ __launch_bounds__(IB) void fm_intersect_kernel(const uint32_t *__rest
                                                           const uint32_t *a_len, const uint64_t *b_off,
                                                           const uint32_t *b_len, uint32_t npairs, uint32_t *counts,
                                                           const uint64_t *out_off, uint32_t *out) {
-    __shared__ uint32_t sA[ITILE];
-    __shared__ uint32_t sB[ITILE + 1];
+    __shared__ uint32_t sAB[ITILE + 1];   // a tile's A slice followed by its B slice (+1)
+    __shared__ uint32_t s_cut[IB + 1];    // merge-path cuts of up to IB tiles
     __shared__ uint32_t s_wave[IB / WAVE];
     __shared__ uint32_t s_split[4];
     const uint32_t tid = threadIdx.x;
@@ -1339,51 +1339,75 @@ __global__ __launch_bounds__(IB) void fm_intersect_kernel(const uint32_t *__rest
                 found += tot;
             }
         } else {
+            // Tile boundaries are found for up to IB tiles at once (one
+            // global merge-path search per thread), then the tiles stream
+            // through LDS with the next tile's elements loaded into registers
+            // while the current one is merged.
             const uint32_t total_path = na + nb;
-            uint32_t a0 = 0, b0 = 0;
-            for (uint32_t d0 = 0; d0 < total_path; d0 += ITILE) {
-                const uint32_t d1 = min(d0 + ITILE, total_path);
-                if (tid == 0) {
-                    const uint32_t a1 = merge_split(A, na, B, nb, d1);
-                    s_split[0] = a1;
-                    s_split[1] = d1 - a1;
+            const uint32_t ntiles = (total_path + ITILE - 1) / ITILE;
+            for (uint32_t r0 = 0; r0 < ntiles; r0 += IB) {
+                const uint32_t rn = min(IB, ntiles - r0);
+                // s_cut[i] = A elements before tile r0 + i (i = 0..rn)
+                for (uint32_t i = tid; i <= rn; i += IB) {
+                    const uint32_t d = min((r0 + i) * ITILE, total_path);
+                    s_cut[i] = merge_split(A, na, B, nb, d);
                 }
                 __syncthreads();
-                const uint32_t a1 = s_split[0], b1 = s_split[1];
-                const uint32_t ta = a1 - a0, tb = b1 - b0;
-                const uint32_t tbx = tb + (b1 < nb ? 1u : 0u);  // one B element past the tile
-                for (uint32_t i = tid; i < ta; i += IB) sA[i] = A[a0 + i];
-                for (uint32_t i = tid; i < tbx; i += IB) sB[i] = B[b0 + i];
-                __syncthreads();
-                // this thread's stretch of the tile's merge path
-                const uint32_t dl = min(tid * IVT, ta + tb);
-                const uint32_t steps = min(IVT, ta + tb - dl);
-                uint32_t i = merge_split(sA, ta, sB, tb, dl);
-                uint32_t j = dl - i;
-                uint32_t m[IVT];
-                uint32_t c = 0;
+                // registers holding the next tile (A slice, then B slice + 1)
+                uint32_t reg[IVT + 1];
+                auto load_tile = [&](uint32_t ti) {
+                    const uint32_t d0 = (r0 + ti) * ITILE, d1 = min(d0 + ITILE, total_path);
+                    const uint32_t a0 = s_cut[ti], a1 = s_cut[ti + 1];
+                    const uint32_t b0 = d0 - a0, b1 = d1 - a1;
+                    const uint32_t ta = a1 - a0, tbx = (b1 - b0) + (b1 < nb ? 1u : 0u);
 #pragma unroll
-                for (uint32_t st = 0; st < IVT; st++) {
-                    if (st < steps) {
-                        if (i < ta && (j >= tb || sA[i] <= sB[j])) {
-                            // consuming A[i]; an equal B element is the next one on B's side
-                            if (j < tbx && sB[j] == sA[i]) m[c++] = sA[i];
-                            i++;
-                        } else {
-                            j++;
+                    for (uint32_t k = 0; k <= IVT; k++) {
+                        const uint32_t e = tid + k * IB;
+                        reg[k] = e < ta ? A[a0 + e] : (e < ta + tbx ? B[b0 + e - ta] : 0u);
+                    }
+                };
+                load_tile(0);
+                for (uint32_t ti = 0; ti < rn; ti++) {
+                    const uint32_t d0 = (r0 + ti) * ITILE, d1 = min(d0 + ITILE, total_path);
+                    const uint32_t a0 = s_cut[ti], a1 = s_cut[ti + 1];
+                    const uint32_t b0 = d0 - a0, b1 = d1 - a1;
+                    const uint32_t ta = a1 - a0, tb = b1 - b0;
+                    const uint32_t tbx = tb + (b1 < nb ? 1u : 0u);
+#pragma unroll
+                    for (uint32_t k = 0; k <= IVT; k++) {
+                        const uint32_t e = tid + k * IB;
+                        if (e < ta + tbx) sAB[e] = reg[k];
+                    }
+                    __syncthreads();
+                    if (ti + 1 < rn) load_tile(ti + 1);  // in flight during the merge
+                    const uint32_t *sA = sAB, *sB = sAB + ta;
+                    const uint32_t dl = min(tid * IVT, ta + tb);
+                    const uint32_t steps = min(IVT, ta + tb - dl);
+                    uint32_t i = merge_split(sA, ta, sB, tb, dl);
+                    uint32_t j = dl - i;
+                    uint32_t m[IVT];
+                    uint32_t c = 0;
+#pragma unroll
+                    for (uint32_t st = 0; st < IVT; st++) {
+                        if (st < steps) {
+                            if (i < ta && (j >= tb || sA[i] <= sB[j])) {
+                                // consuming A[i]; an equal B element is the next one on B's side
+                                if (j < tbx && sB[j] == sA[i]) m[c++] = sA[i];
+                                i++;
+                            } else {
+                                j++;
+                            }
                         }
                     }
-                }
-                uint32_t tot;
-                const uint32_t at = block_excl_scan(c, s_wave, tot);
-                if (o) {
+                    uint32_t tot;
+                    const uint32_t at = block_excl_scan(c, s_wave, tot);
+                    if (o) {
 #pragma unroll
-                    for (uint32_t k = 0; k < IVT; k++)
-                        if (k < c) o[found + at + k] = m[k];
+                        for (uint32_t k = 0; k < IVT; k++)
+                            if (k < c) o[found + at + k] = m[k];
+                    }
+                    found += tot;
                 }
-                found += tot;
-                a0 = a1;
-                b0 = b1;
                 __syncthreads();
             }
         }
