@@ -14,6 +14,14 @@ resident before the timed region).
 (batched), with the plain one-launch-per-query rate measured beside it;
 value = queries/s of the batched form.
 
+--config 4: serving. R-MAT 2^22 vertices, 2^26 undirected edges, edge
+labels Zipf(1.2) over 0..127, 4 vertex labels; 8 MiB banks (CurBankSize =
+1 << 20). 512 queries of 4..16 vertices (at most 2^18 embeddings each)
+form the request pool; Poisson arrival streams at 200, 1,000 and 5,000
+requests/s go through one continuous-batching server, admission by bank
+budget. value = completed queries/s at the highest offered rate; p50/p99
+latency (arrival to completion seen by the client) per rate.
+
 Prints one JSON line. Only rank 0 prints; --gpus above 1 is refused (the
 build targets one MI355X).
 """
@@ -65,7 +73,7 @@ def make_candidates(g, cfg: int, seed: int, count: int):
         else:
             u = rng.random()
             shape = 0 if u < 0.4 else (1 if u < 0.8 else 2)
-            n = int(rng.integers(4, 13))
+            n = int(rng.integers(4, 13 if cfg == 3 else 17))
             cyc = 1 if shape == 1 else 0
         try:
             q = fm.Query.random_walk(g, n, shape=shape, min_cycles=cyc, seed=seed * 1_000_003 + i)
@@ -104,6 +112,29 @@ def screen(engine, g, cands, budget: int, want, per_shape=None):
         log(f"screen: {b0 + len(chunk)}/{len(cands)} screened, {len(keep)} kept")
     log(f"screen: statuses {hist}")
     return keep, counts
+
+
+def select_mixed(engine, g, cfg: int, seed: int, nq: int, budget: int):
+    """Progressive screening from a seeded pool (at most 48 x nq candidates)
+    with the 40/40/20 tree / cycle+chord / near-clique caps; shapes that
+    cannot fill their share within the pool are topped up by the others."""
+    caps = {0: int(0.4 * nq), 1: int(0.4 * nq), 2: nq - 2 * int(0.4 * nq)}
+    chosen, counts, pool = [], [], []
+    rnd = 0
+    while len(chosen) < nq and len(pool) < 48 * nq:
+        cands = make_candidates(g, cfg, seed + 7919 * rnd, max(64, nq // 2))
+        rnd += 1
+        pool += cands
+        left = {k: caps[k] - sum(1 for c in chosen if c[0] == k) for k in caps}
+        c2, n2 = screen(engine, g, cands, budget, nq - len(chosen), per_shape=left)
+        chosen += c2
+        counts += n2
+    if len(chosen) < nq:
+        taken = {id(c[2]) for c in chosen}
+        c2, n2 = screen(engine, g, [c for c in pool if id(c[2]) not in taken], budget, nq - len(chosen))
+        chosen += c2
+        counts += n2
+    return chosen, counts
 
 
 _CSR = {}
@@ -147,11 +178,17 @@ def cpu_baseline(g, queries, target_s: float, threads: int):
                 for _, _, q in queries)
     if probe > 0:
         stride = int(max(1, min(V, stride * probe / target_s)))
-    total_cnt, total_s = 0, 0.0
-    for _, _, q in queries:
-        r = orc.match(og, q.vertices, q.edges(), threads=threads, table=False, stride=stride)
-        total_cnt += r.count
-        total_s += r.seconds
+    for _ in range(4):
+        total_cnt, total_s = 0, 0.0
+        for _, _, q in queries:
+            r = orc.match(og, q.vertices, q.edges(), threads=threads, table=False, stride=stride)
+            total_cnt += r.count
+            total_s += r.seconds
+        # thread start-up dominates a tiny probe: refine until the sample is
+        # at least a third of the target
+        if total_s >= target_s / 3 or stride == 1:
+            break
+        stride = max(1, int(stride * max(total_s, 1e-3) / target_s))
     return total_cnt, total_s, stride
 
 
@@ -194,6 +231,95 @@ def intersect_bench(engine, pairs: int = 2048, mean_len: int = 32768, reps: int 
             "frac": gbs / HBM_PEAK_GBS, "checked": bool(ok)}
 
 
+def serve_rate(srv, pool, counts, rate: float, seconds: float, rng, balg=None, deadline_s: float = 240.0):
+    """One Poisson arrival stream through the server. Requests are submitted
+    at their arrival time (or, when admission refuses them, as soon as a
+    slot and bank budget free up: the wait counts toward their latency).
+    Every completion's count is checked against the screening count."""
+    from collections import deque
+    n_req = max(1, int(rate * seconds))
+    sched = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    qidx = rng.integers(0, len(pool), n_req)
+    done = np.full(n_req, np.nan)
+    t2req, backlog = {}, deque()
+    nxt = ndone = bad = retries = 0
+    inflight_max = 0
+    emb = 0
+    t0 = time.perf_counter()
+    last_log = t0
+    while ndone < n_req:
+        now = time.perf_counter() - t0
+        while nxt < n_req and sched[nxt] <= now:
+            backlog.append(nxt)
+            nxt += 1
+        while backlog:
+            k = backlog[0]
+            t = srv.submit(pool[qidx[k]])
+            if t is None:
+                retries += 1
+                break
+            t2req[t] = k
+            backlog.popleft()
+        inflight_max = max(inflight_max, len(t2req))
+        comps = srv.poll()
+        if comps:
+            tn = time.perf_counter() - t0
+            for c in comps:
+                k = t2req.pop(c.ticket)
+                done[k] = tn
+                want = counts[qidx[k]]
+                if c.status != 0 or c.count != want:
+                    bad += 1
+                emb += c.count
+                srv.release(c.ticket)
+                ndone += 1
+        tn = time.perf_counter()
+        if tn - last_log > 20:
+            last_log = tn
+            log(f"serve {rate:.0f} req/s: {ndone}/{n_req} done, {len(t2req)} in flight, {len(backlog)} waiting")
+        if tn - t0 > deadline_s:
+            raise RuntimeError(f"serving run at {rate} req/s did not drain in {deadline_s} s "
+                               f"({ndone}/{n_req} done)")
+    lat = (done - sched) * 1e3
+    span = float(np.nanmax(done) - sched[0])
+    out = {"offered_req_per_s": rate, "requests": n_req, "achieved_queries_per_s": n_req / span,
+           "latency_ms_p50": float(np.percentile(lat, 50)), "latency_ms_p99": float(np.percentile(lat, 99)),
+           "latency_ms_max": float(lat.max()), "embeddings_per_s": emb / span, "max_in_flight": inflight_max,
+           "refused_submits": retries, "mismatches": bad, "seconds": span}
+    if balg is not None:
+        out["alg_bytes"] = int(sum(balg[i] for i in qidx))
+        out["alg_gbs_wall"] = out["alg_bytes"] / span / 1e9
+    return out
+
+
+def serve_mix_check(srv, pool, alone, rng, extra: int = 56) -> bool:
+    """The first len(alone) pool queries submitted together with `extra`
+    others: each table must equal the table of the same query matched alone."""
+    reqs = list(range(len(alone))) + [int(i) for i in rng.integers(0, len(pool), extra)]
+    rng.shuffle(reqs)
+    t2q, got = {}, {}
+    pending = list(reqs)
+    t0 = time.perf_counter()
+    while pending or t2q:
+        while pending:
+            t = srv.submit(pool[pending[0]])
+            if t is None:
+                break
+            t2q[t] = pending.pop(0)
+        for c in srv.poll():
+            qi = t2q.pop(c.ticket)
+            if qi < len(alone) and qi not in got:
+                got[qi] = (c.status, c.count, srv.table(c.ticket, c.rows, c.vertices, sorted=True))
+            srv.release(c.ticket)
+        if time.perf_counter() - t0 > 120:
+            raise RuntimeError("mix check did not drain in 120 s")
+    ok = True
+    for qi, (count, table) in enumerate(alone):
+        st, cnt, tab = got[qi]
+        ok = ok and st == 0 and cnt == count and tab.shape == table.shape and bool(np.array_equal(tab, table))
+    return ok
+
+
 def b_alg(o, n: int, table: bool) -> int:
     """Algorithmic bytes: neighbour words of every requirement row of every
     partial embedding, an 8-byte record per partial, the final table."""
@@ -213,9 +339,11 @@ def main():
     ap.add_argument("--queries", type=int, default=0)
     ap.add_argument("--save-queries", default="", help="write the selected queries + manifest here")
     ap.add_argument("--load-queries", default="", help="reuse a saved query set (no screening launches)")
+    ap.add_argument("--rates", default="200,1000,5000", help="config 4: offered loads in requests/s")
+    ap.add_argument("--serve-seconds", type=float, default=6.0, help="config 4: arrival window per rate")
     args = ap.parse_args()
     if not args.budget:
-        args.budget = (1 << 22) if args.config == 3 else (1 << 26)
+        args.budget = {3: 1 << 22, 4: 1 << 18}.get(args.config, 1 << 26)
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py runs on one MI355X only: this build has no multi-GPU path "
@@ -229,13 +357,19 @@ def main():
     g.upload()
     gi = g.info()
     log(f"graph: V={gi.vertices} E={gi.edges} labels={gi.labels} built in {time.time() - t0:.1f}s")
-    # config 3 keeps 1,024 tables resident at once: a larger table arena
-    engine = fm.Engine(output_prealloc=(96 << 30) if args.config == 3 else 0)
+    if args.config == 3:
+        # 1,024 tables resident at once: a larger table arena
+        engine = fm.Engine(output_prealloc=96 << 30)
+    elif args.config == 4:
+        # CurBankSize = 1 << 20 (8 MiB banks); 64 GiB of banks, 64 GiB of pages
+        engine = fm.Engine(soln_prealloc=64 << 30, output_prealloc=64 << 30, bank_size=1 << 20)
+    else:
+        engine = fm.Engine()
     ei = engine.info()
     log(f"engine: {ei.device_name.decode()} CUs={ei.compute_units} waves/CU={ei.waves_per_cu} "
         f"grid={ei.grid_waves} banks={ei.banks}x{ei.bank_bytes}B")
 
-    nq = args.queries or (32 if args.config == 2 else 1024)
+    nq = args.queries or {2: 32, 3: 1024, 4: 512}[args.config]
     t1 = time.time()
     manifest = None
     if args.load_queries:
@@ -254,25 +388,7 @@ def main():
             chosen += c2
             counts += n2
     else:
-        # Progressive screening from a seeded pool (at most 48 x nq
-        # candidates) with the 40/40/20 shape caps; shapes that cannot fill
-        # their share within the pool are topped up by the others.
-        caps = {0: int(0.4 * nq), 1: int(0.4 * nq), 2: nq - 2 * int(0.4 * nq)}
-        chosen, counts, pool = [], [], []
-        rnd = 0
-        while len(chosen) < nq and len(pool) < 48 * nq:
-            cands = make_candidates(g, 3, args.seed + 7919 * rnd, nq // 2)
-            rnd += 1
-            pool += cands
-            left = {k: caps[k] - sum(1 for c in chosen if c[0] == k) for k in caps}
-            c2, n2 = screen(engine, g, cands, args.budget, nq - len(chosen), per_shape=left)
-            chosen += c2
-            counts += n2
-        if len(chosen) < nq:
-            taken = {id(c[2]) for c in chosen}
-            c2, n2 = screen(engine, g, [c for c in pool if id(c[2]) not in taken], args.budget, nq - len(chosen))
-            chosen += c2
-            counts += n2
+        chosen, counts = select_mixed(engine, g, args.config, args.seed, nq, args.budget)
     log(f"selected {len(chosen)} queries in {time.time() - t1:.1f}s; total embeddings {sum(counts)}")
     queries = [c[2] for c in chosen]
     ns = [c[1] for c in chosen]
@@ -349,6 +465,52 @@ def main():
                          "kernel": "fm_match_kernel", "kernel_ms_per_step": dom_ms},
         }
         workload = "rmat20_plain_32q"
+    elif args.config == 4:
+        rng = np.random.default_rng(args.seed)
+        # tables of 8 queries matched alone, for the mix check (untimed)
+        alone = []
+        for q in queries[:8]:
+            r = engine.match(g, q, timeout_ms=300000)
+            alone.append((r.count(0), r.table(0, sorted=True)))
+            r.close()
+        rates = [float(x) for x in args.rates.split(",") if x]
+        per_rate = []
+        banks = engine.info().banks
+        with fm.Server(engine, g, slots=256, slot_pages=4096, bank_budget=max(1, banks * 3 // 4)) as srv:
+            serve_rate(srv, queries, counts, rates[0], 1.0, rng)  # warmup
+            for rate in rates:
+                pr = serve_rate(srv, queries, counts, rate, args.serve_seconds, rng, balg=balg)
+                log(f"serve {rate:.0f} req/s: {pr['achieved_queries_per_s']:.1f} q/s, "
+                    f"p50 {pr['latency_ms_p50']:.2f} ms, p99 {pr['latency_ms_p99']:.2f} ms, "
+                    f"mismatches {pr['mismatches']}")
+                per_rate.append(pr)
+            mix_ok = serve_mix_check(srv, queries, alone, rng)
+            sst = srv.stats()
+        top = per_rate[-1]
+        ok = mix_ok and all(pr["mismatches"] == 0 for pr in per_rate)
+        achieved = top["alg_gbs_wall"]
+        result = {
+            "metric": f"queries/s (serving, Poisson arrivals at {top['offered_req_per_s']:.0f} req/s)",
+            "value": top["achieved_queries_per_s"],
+            "unit": "queries/s",
+            "ms_per_step": 1e3 / top["achieved_queries_per_s"],
+            "latency_ms_p50": top["latency_ms_p50"],
+            "latency_ms_p99": top["latency_ms_p99"],
+            "rates": per_rate,
+            "server": {"slots": 256, "bank_bytes": engine.info().bank_bytes, "banks": banks,
+                       "bank_budget": max(1, banks * 3 // 4), "launches": int(sst.launches),
+                       "refused": int(sst.refused)},
+            "mix_equals_alone": bool(mix_ok),
+            "parity_checks_passed": bool(ok),
+            "shape_mix": {name: sum(1 for c in chosen if c[0] == k)
+                          for k, name in ((0, "tree"), (1, "cycle+chord"), (2, "near-clique"))},
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                         "alg_bytes_per_step": top["alg_bytes"] / top["requests"],
+                         "kernel": "fm_match_kernel (persistent serving launch, wall clock)",
+                         "kernel_ms_per_step": 1e3 / top["achieved_queries_per_s"]},
+        }
+        workload = f"rmat22_serving_{len(queries)}q_poisson"
     else:
         from forkmatch_amd.autotune import autotune
         space = {"root_chunk": [256, 4096], "task_rows": [256, 1024], "grid_frac": [0.5, 1.0]}
@@ -430,7 +592,8 @@ def main():
         "vs_baseline": None,
         "dtype": "u32",
         "data": "synthetic (seeded R-MAT graph and random-walk queries)",
-        "config": {"workload": workload, "graph": f"rmat scale 20, {gi.edges} directed edges, {gi.labels} labels",
+        "config": {"workload": workload,
+                   "graph": f"rmat scale {gi.vertices.bit_length() - 1}, {gi.edges} directed edges, {gi.labels} labels",
                    "queries": len(queries), "budget": args.budget, "seed": args.seed},
         "roofline": result.pop("roofline"),
         "cpu_baseline": cpu,

@@ -28,6 +28,8 @@ FM_MODE_COUNT = 0
 FM_MODE_TABLE = 1
 FM_RESULT_TRUNCATED = 1
 FM_RESULT_ABORTED = 2
+PLAN_COST = 0       # order from the data graph's label statistics
+PLAN_REFERENCE = 1  # the reference's query-degree order
 
 STATUS_NAMES = {
     0: "FM_OK", 1: "FM_ERR_GENERIC", 12: "FM_ERR_DATA_MISSING_FILE", 76: "FM_ERR_DATA_BAD_START",
@@ -231,15 +233,24 @@ class Query:
         _check(L.lib().fm_query_edges(self._h, _p32(s), _p32(d), _p32(l)))
         return np.stack([s, d, l], axis=1)[: m.value]
 
-    def plan(self) -> Plan:
+    def plan(self, graph: "Graph | None" = None, plan: int = None) -> Plan:
+        """The reference's plan, or with `graph` the plan matching uses on
+        that graph (`plan` = PLAN_COST, the default there, or PLAN_REFERENCE)."""
         info = L.PlanInfo()
-        _check(L.lib().fm_query_plan(self._h, C.byref(info), None, None, None, None, None))
+        if graph is None:
+            def call(*a):
+                return L.lib().fm_query_plan(self._h, *a)
+        else:
+            mode = PLAN_COST if plan is None else plan
+
+            def call(*a):
+                return L.lib().fm_query_plan_on(graph._h, self._h, mode, *a)
+        _check(call(C.byref(info), None, None, None, None, None))
         n, r = info.vertices, max(info.nreqs, 1)
         order = np.zeros(n, np.uint32)
         rb = np.zeros(n + 1, np.uint32)
         an, lb, dr = (np.zeros(r, np.uint32) for _ in range(3))
-        _check(L.lib().fm_query_plan(self._h, C.byref(info), _p32(order), _p32(rb), _p32(an), _p32(lb),
-                                     _p32(dr)))
+        _check(call(C.byref(info), _p32(order), _p32(rb), _p32(an), _p32(lb), _p32(dr)))
         k = info.nreqs
         return Plan(order.tolist(), rb.tolist(), an[:k].tolist(), lb[:k].tolist(), dr[:k].tolist())
 
@@ -344,9 +355,10 @@ class Engine:
 
     @staticmethod
     def _opts(mode, instrument, max_rows, max_count, timeout_ms, grid_waves, task_rows, root_chunk,
-              max_partials=0):
+              max_partials=0, plan=0):
         o = L.MatchOpts()
         o.max_partials = max_partials
+        o.plan = plan
         o.mode = mode
         o.instrument = 1 if instrument else 0
         o.max_rows = max_rows
@@ -363,9 +375,9 @@ class Engine:
     def match_batch(self, graph: Graph, queries, table: bool = True, instrument: bool = False,
                     max_rows: int = 0, max_count: int = 0, timeout_ms: int = 0, grid_waves: int = 0,
                     task_rows: int = 0, root_chunk: int = 0, max_partials: int = 0,
-                    allow_errors: bool = False) -> Result:
+                    allow_errors: bool = False, plan: int = 0) -> Result:
         o = self._opts(FM_MODE_TABLE if table else FM_MODE_COUNT, instrument, max_rows, max_count, timeout_ms,
-                       grid_waves, task_rows, root_chunk, max_partials)
+                       grid_waves, task_rows, root_chunk, max_partials, plan)
         arr = (C.c_void_p * len(queries))(*[q._h for q in queries])
         h = C.c_void_p()
         rc = L.lib().fm_match_batch(self._h, graph._h, arr, len(queries), C.byref(o), C.byref(h))
@@ -409,11 +421,11 @@ class Server:
 
     def __init__(self, engine: Engine, graph: Graph, slots: int = 256, slot_pages: int = 256, bank_budget: int = 0,
                  idle_ms: int = 20, root_chunk: int = 0, task_rows: int = 0, table: bool = True,
-                 max_launch_ms: int = 0):
+                 max_launch_ms: int = 0, plan: int = 0, slice_ms: int = 0):
         cfg = L.ServerConfig()
         cfg.slots, cfg.slot_pages, cfg.bank_budget, cfg.idle_ms = slots, slot_pages, bank_budget, idle_ms
         cfg.root_chunk, cfg.task_rows, cfg.table = root_chunk, task_rows, 1 if table else 0
-        cfg.max_launch_ms = max_launch_ms
+        cfg.max_launch_ms, cfg.plan, cfg.slice_ms = max_launch_ms, plan, slice_ms
         h = C.c_void_p()
         _check(L.lib().fm_server_create(engine._h, graph._h, C.byref(cfg), C.byref(h)))
         self._h = h

@@ -231,6 +231,7 @@ struct KParams {
     uint32_t max_slot_pages;
     uint32_t pad2;
     unsigned long long idle_ticks;
+    unsigned long long slice_ticks;  // serving: the launch retires after this long (0 = never)
 };
 
 // ---------------------------------------------------------------------------
@@ -1122,12 +1123,13 @@ __global__ __launch_bounds__(WAVE) void fm_match_kernel(KParams P) {
     if (lane == 0) wt = WaveTab{0, NO_PAGE, 0, 0, 0};
     unsigned long long stats[ST_N] = {0, 0, 0, 0, 0, 0, 0, 0};
     unsigned long long starve_since = 0;
-    unsigned long long deadline = 0;
+    unsigned long long deadline = 0, retire_at = ~0ull;  // retire_at: lane 0 only
     if (lane == 0) {
         unsigned long long now = wall_clock64(), zero = 0;
         __hip_atomic_compare_exchange_strong(&P.ctl->t0, &zero, now, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                              __HIP_MEMORY_SCOPE_AGENT);
         deadline = ld_rlx(&P.ctl->t0) + P.timeout_ticks;
+        if (P.serving && P.slice_ticks) retire_at = ld_rlx(&P.ctl->t0) + P.slice_ticks;
     }
     deadline = ((unsigned long long)bcast((uint32_t)(deadline >> 32), 0) << 32) | bcast((uint32_t)deadline, 0);
     while (true) {
@@ -1138,6 +1140,10 @@ __global__ __launch_bounds__(WAVE) void fm_match_kernel(KParams P) {
                 flag = 2;
             } else if (wall_clock64() > deadline) {
                 raise_error(P, FM_ERR_TIMEOUT);
+                flag = 2;
+            } else if (wall_clock64() > retire_at) {
+                // serving: leave between tasks; queued tasks, admitted
+                // queries and banks stay on the device for the next launch
                 flag = 2;
             } else {
                 TaskDesc t;
@@ -1692,9 +1698,15 @@ int fm_engine_info_get(fm_engine *e, fm_engine_info *out) {
 
 // Device plan of a query against a graph: dense label ids, symmetric
 // de-duplication, root list selection.
-static int make_dplan(const fm_graph *g, const HostQuery &q, bool all_roots, DPlan &dp) {
+static int make_dplan(const fm_graph *g, const HostQuery &q, bool all_roots, uint32_t plan, DPlan &dp) {
     const HostGraph &h = g->h;
-    const Plan &p = q.plan;
+    if (plan != FM_PLAN_COST && plan != FM_PLAN_REFERENCE) return fail(FM_ERR_INVALID_ARG, "unknown plan %u", plan);
+    Plan cost_plan;
+    if (plan == FM_PLAN_COST) {
+        int rc = make_cost_plan(q, h, cost_plan);
+        if (rc) return rc;
+    }
+    const Plan &p = plan == FM_PLAN_COST ? cost_plan : q.plan;
     std::memset(&dp, 0, sizeof dp);
     dp.n = p.n;
     uint32_t nr = 0;
@@ -1769,7 +1781,7 @@ int fm_match_batch(fm_engine *e, const fm_graph *g, const fm_query *const *qs, u
     uint32_t maxn = 1;
     for (uint32_t i = 0; i < nq; i++) {
         if (!qs[i]) return fail(FM_ERR_INVALID_ARG, "null query %u", i);
-        int rc = make_dplan(g, qs[i]->h, o.instrument != 0, plans[i]);
+        int rc = make_dplan(g, qs[i]->h, o.instrument != 0, o.plan, plans[i]);
         if (rc) return rc;
         DQuery &Q = qh[i];
         std::memset(&Q, 0, sizeof Q);
@@ -2071,6 +2083,7 @@ static int server_launch(fm_server *s) {
     P.max_slots = s->cfg.slots;
     P.max_slot_pages = s->cfg.slot_pages;
     P.idle_ticks = (unsigned long long)(s->cfg.idle_ms * (e->wall_hz / 1000.0));
+    P.slice_ticks = (unsigned long long)(s->cfg.slice_ms * (e->wall_hz / 1000.0));
     hipLaunchKernelGGL(fm_match_kernel, dim3(e->grid_waves), dim3(WAVE), 0, e->stream, P);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(s->kernel_done, e->stream));
@@ -2079,7 +2092,19 @@ static int server_launch(fm_server *s) {
     return FM_OK;
 }
 
-// Relaunches the persistent kernel when it has gone idle and work waits.
+// A submission not yet admitted, or an admitted request whose completion
+// the device has not published.
+static bool server_has_work(const fm_server *s) {
+    uint32_t admitted = __atomic_load_n(&s->sh->admitted, __ATOMIC_ACQUIRE);
+    if (admitted != s->sub_tail) return true;
+    for (size_t i = 0; i < s->slot_state.size(); i++)
+        if (s->slot_state[i] == 1 && __atomic_load_n(&s->hdone[i].ticket, __ATOMIC_ACQUIRE) != s->slot_ticket[i])
+            return true;
+    return false;
+}
+
+// Relaunches the persistent kernel when it has gone idle or retired and
+// work waits.
 static int server_pump(fm_server *s) {
     if (s->launched) {
         hipError_t q = hipEventQuery(s->kernel_done);
@@ -2093,10 +2118,7 @@ static int server_pump(fm_server *s) {
             return fail((int)back.error, "serving kernel aborted with status %u", back.error);
         }
     }
-    uint32_t admitted = __atomic_load_n(&s->sh->admitted, __ATOMIC_ACQUIRE);
-    bool in_flight = false;
-    for (auto st : s->slot_state) in_flight |= st == 1;
-    if (admitted != s->sub_tail || in_flight) return server_launch(s);
+    if (server_has_work(s)) return server_launch(s);
     return FM_OK;
 }
 
@@ -2117,6 +2139,10 @@ int fm_server_create(fm_engine *e, const fm_graph *g, const fm_server_config *cf
     if (!s->cfg.task_rows) s->cfg.task_rows = SubMaxMulSpill;
     if (!s->cfg.bank_budget) s->cfg.bank_budget = e->nbanks;
     if (!s->cfg.max_launch_ms) s->cfg.max_launch_ms = 120000;
+    if (!s->cfg.slice_ms) s->cfg.slice_ms = 2000;
+    if (s->cfg.slice_ms >= s->cfg.max_launch_ms)
+        return fail(FM_ERR_INVALID_ARG, "slice_ms %u must be below max_launch_ms %u", s->cfg.slice_ms,
+                    s->cfg.max_launch_ms);
     if (!cfg) s->cfg.table = 1;
     const uint32_t S = s->cfg.slots;
     auto fail_out = [&](int rc) {
@@ -2183,7 +2209,7 @@ int fm_server_submit(fm_server *s, const fm_query *q, uint64_t max_rows, uint32_
     }
     uint32_t slot = s->free_slots.back();
     DPlan dp;
-    int rc = make_dplan(s->g, q->h, false, dp);
+    int rc = make_dplan(s->g, q->h, false, s->cfg.plan, dp);
     if (rc) return rc;
     const uint32_t n = dp.n;
     SrvQInit qi{};
@@ -2308,13 +2334,22 @@ int fm_server_stats_get(fm_server *s, fm_server_stats *out) {
 void fm_server_destroy(fm_server *s) {
     if (!s) return;
     if (s->sh) __atomic_store_n(&s->sh->shutdown, 1u, __ATOMIC_RELEASE);
-    if (s->launched) {
-        // give in-flight queries 10 s to drain, then stop the launch
+    if (s->sh && s->e) {
+        // give in-flight queries 10 s to drain (relaunching retired
+        // launches), then stop the launch
         auto t_stop = std::chrono::steady_clock::now() + std::chrono::seconds(10);
-        while (hipEventQuery(s->kernel_done) == hipErrorNotReady && std::chrono::steady_clock::now() < t_stop)
-            std::this_thread::sleep_for(std::chrono::milliseconds(1));
-        if (hipEventQuery(s->kernel_done) == hipErrorNotReady) __atomic_store_n(&s->sh->abort, 1u, __ATOMIC_RELEASE);
-        (void)hipEventSynchronize(s->kernel_done);
+        while (std::chrono::steady_clock::now() < t_stop) {
+            if (s->launched && hipEventQuery(s->kernel_done) == hipErrorNotReady) {
+                std::this_thread::sleep_for(std::chrono::milliseconds(1));
+                continue;
+            }
+            if (server_pump(s) != FM_OK || !s->launched) break;
+        }
+        if (s->launched) {
+            if (hipEventQuery(s->kernel_done) == hipErrorNotReady)
+                __atomic_store_n(&s->sh->abort, 1u, __ATOMIC_RELEASE);
+            (void)hipEventSynchronize(s->kernel_done);
+        }
     }
     if (s->e) {
         (void)hipSetDevice(s->e->device);

@@ -26,7 +26,13 @@ struct Csr {
     std::vector<uint32_t> col;
     std::vector<uint32_t> root_ptr;
     std::vector<uint32_t> roots;
+    // per label: sum over vertices of the squared row length (the planner's
+    // size-biased row length is row_sq[l] / edges of label l)
+    std::vector<double> row_sq;
 };
+
+// Fills c.row_sq from c.ptr.
+void csr_stats(Csr &c, uint32_t V, uint32_t L);
 
 struct HostGraph {
     uint32_t V = 0;
@@ -72,6 +78,12 @@ struct HostQuery {
 int build_query(uint32_t n, uint32_t m, const uint32_t *src, const uint32_t *dst,
                 const uint32_t *label, HostQuery &q);
 int make_plan(const HostQuery &q, Plan &p);
+// Requirements of each depth for a given matching order (order[0] is the
+// root; every later vertex must be adjacent to an earlier one).
+int plan_from_order(const HostQuery &q, const std::vector<uint32_t> &order, Plan &p);
+// Matching order chosen from the data graph's label statistics: the order
+// whose estimated number of examined candidates is smallest.
+int make_cost_plan(const HostQuery &q, const HostGraph &g, Plan &p);
 
 // Reads a whole file. Returns false if it cannot be opened.
 bool read_file(const std::string &path, std::string &buf);

@@ -168,7 +168,21 @@ static int build_dir(uint32_t V, uint32_t L, uint64_t n, const uint32_t *lid,
             if (p[v + 1] > p[v]) c.roots.push_back(v);
     }
     c.root_ptr[L] = (uint32_t)c.roots.size();
+    csr_stats(c, V, L);
     return FM_OK;
+}
+
+void csr_stats(Csr &c, uint32_t V, uint32_t L) {
+    c.row_sq.assign(L, 0.0);
+    for (uint32_t l = 0; l < L; l++) {
+        const uint32_t *p = c.ptr.data() + (uint64_t)l * V;
+        double sq = 0;
+        for (uint32_t v = 0; v < V; v++) {
+            double k = (double)(p[v + 1] - p[v]);
+            sq += k * k;
+        }
+        c.row_sq[l] = sq;
+    }
 }
 
 static bool row_has(const Csr &c, uint32_t V, uint32_t l, uint32_t v, uint32_t x) {
@@ -412,14 +426,10 @@ int build_query(uint32_t n, uint32_t m, const uint32_t *src, const uint32_t *dst
     return make_plan(q, q.plan);
 }
 
-// Matching order and per-depth requirements, following
-// genereateCCSRRequirements/preProcessQuery (src/ccsr.cuh:229-326,
-// 485-549): rank vertices by (out-degree desc, id asc)
-// (src/hostparser.cuh:20-30,420); depth 0 is rank 0; each later depth
+// Matching order following genereateCCSRRequirements/preProcessQuery
+// (src/ccsr.cuh:229-326, 485-549): rank vertices by (out-degree desc, id
+// asc) (src/hostparser.cuh:20-30,420); depth 0 is rank 0; each later depth
 // takes the lowest-ranked unmapped vertex adjacent to the mapped set.
-// Requirements of a depth are its edges to earlier vertices, ordered by
-// climb (most recent anchor first) as the reference sorts them
-// (src/ccsr.cuh:293-300).
 int make_plan(const HostQuery &q, Plan &p) {
     uint32_t n = q.n;
     std::vector<uint32_t> outdeg(n, 0);
@@ -432,29 +442,42 @@ int make_plan(const HostQuery &q, Plan &p) {
     });
     std::vector<uint32_t> rank(n);
     for (uint32_t i = 0; i < n; i++) rank[rank_order[i]] = i;
-    p.n = n;
-    p.order.assign(n, 0);
-    p.depth_of.assign(n, UINT32_MAX);
-    std::vector<char> seen(n, 0);
-    p.order[0] = rank_order[0];
-    p.depth_of[rank_order[0]] = 0;
-    seen[rank_order[0]] = 1;
+    std::vector<uint32_t> order(n, 0);
+    std::vector<char> seen(n, 0), mapped(n, 0);
+    order[0] = rank_order[0];
+    mapped[rank_order[0]] = 1;
     auto discover = [&](uint32_t u) {
         for (auto &e : q.edges) {
-            if (e[0] == u && !seen[e[1]]) seen[e[1]] = 1;
-            if (e[1] == u && !seen[e[0]]) seen[e[0]] = 1;
+            if (e[0] == u) seen[e[1]] = 1;
+            if (e[1] == u) seen[e[0]] = 1;
         }
     };
     discover(rank_order[0]);
     for (uint32_t d = 1; d < n; d++) {
         uint32_t best = UINT32_MAX;
         for (uint32_t v = 0; v < n; v++)
-            if (seen[v] && p.depth_of[v] == UINT32_MAX && (best == UINT32_MAX || rank[v] < rank[best]))
-                best = v;
+            if (seen[v] && !mapped[v] && (best == UINT32_MAX || rank[v] < rank[best])) best = v;
         if (best == UINT32_MAX) return fail(FM_ERR_GENERIC, "query graph is disconnected");
-        p.order[d] = best;
-        p.depth_of[best] = d;
+        order[d] = best;
+        mapped[best] = 1;
         discover(best);
+    }
+    return plan_from_order(q, order, p);
+}
+
+// Requirements of a depth are its edges to earlier vertices, ordered by
+// climb (most recent anchor first) as the reference sorts them
+// (src/ccsr.cuh:293-300).
+int plan_from_order(const HostQuery &q, const std::vector<uint32_t> &order, Plan &p) {
+    const uint32_t n = q.n;
+    if (order.size() != n) return fail(FM_ERR_INVALID_ARG, "order has %zu entries for %u vertices", order.size(), n);
+    p.n = n;
+    p.order = order;
+    p.depth_of.assign(n, UINT32_MAX);
+    for (uint32_t d = 0; d < n; d++) {
+        if (order[d] >= n || p.depth_of[order[d]] != UINT32_MAX)
+            return fail(FM_ERR_INVALID_ARG, "order is not a permutation");
+        p.depth_of[order[d]] = d;
     }
     p.req_begin.assign(n + 1, 0);
     p.reqs.clear();
@@ -483,7 +506,7 @@ int make_plan(const HostQuery &q, Plan &p) {
         if (d > 0) {
             bool has_anchor = false;
             for (auto &r : rs) has_anchor |= r.anchor < d;
-            if (!has_anchor) return fail(FM_ERR_GENERIC, "plan check failed at depth %u", d);
+            if (!has_anchor) return fail(FM_ERR_GENERIC, "query graph is disconnected (depth %u has no anchor)", d);
         }
     }
     p.req_begin[n] = (uint32_t)p.reqs.size();
@@ -494,6 +517,125 @@ int make_plan(const HostQuery &q, Plan &p) {
         return fail(FM_ERR_QUERY_TOO_LARGE, "query has %zu requirements; the limit is %d",
                     p.reqs.size(), FM_MAX_REQS_TOTAL);
     return FM_OK;
+}
+
+// Cost-based matching order (no reference counterpart: the reference orders
+// by query degree only, which on skewed graphs expands hub rows of common
+// labels first). The embedding set does not depend on the order, only the
+// work does. Model, per direction d and label l of the data graph: E = edges,
+// R = vertices with a non-empty row, Q = sum of squared row lengths.
+//  - the root and the depth-1 vertex are joined by query edges; the root
+//    list is the smallest R among them and the depth-1 partials number E of
+//    that edge (every edge of the label out of a root);
+//  - a later vertex reached from a matched anchor through label l draws its
+//    candidates from a row of a vertex that was itself reached along edges,
+//    so its expected length is the size-biased Q / E;
+//  - each further edge to the mapped set keeps a candidate with probability
+//    (its expected row length) / V.
+// Every root is tried; from it the order grows greedily by the smallest
+// estimated number of partials. The order with the fewest examined
+// candidates (sum over depths of partials times the driver row length) wins;
+// ties keep the reference's order.
+int make_cost_plan(const HostQuery &q, const HostGraph &g, Plan &p) {
+    const uint32_t n = q.n;
+    Plan ref;
+    int rc = make_plan(q, ref);
+    if (rc || n <= 2) {
+        if (!rc) p = ref;
+        return rc;
+    }
+    const double V = std::max<double>(1.0, g.V);
+    struct QE {
+        uint32_t a, b;
+        double E[2], R[2], fan[2];  // [0]: a's out-row holds b; [1]: b's in-row holds a
+    };
+    std::vector<QE> qe;
+    for (auto &e : q.edges) {
+        QE x{e[0], e[1], {0, 0}, {0, 0}, {0, 0}};
+        uint32_t lid = g.label_id(e[2]);
+        if (lid != UINT32_MAX) {
+            for (int d = 0; d < 2; d++) {
+                const Csr &c = g.dir(d);
+                x.E[d] = (double)c.ptr[(uint64_t)(lid + 1) * g.V] - (double)c.ptr[(uint64_t)lid * g.V];
+                x.R[d] = (double)(c.root_ptr[lid + 1] - c.root_ptr[lid]);
+                x.fan[d] = x.E[d] > 0 ? c.row_sq[lid] / x.E[d] : 0.0;
+            }
+        }
+        qe.push_back(x);
+    }
+    // row of anchor w that holds candidate u along edge x: out-row of a if
+    // w == a (u == b), in-row of b if w == b (u == a)
+    auto dir_from = [](const QE &x, uint32_t w) { return x.a == w ? 0 : 1; };
+    double best_cost = 0;
+    std::vector<uint32_t> best_order;
+    std::vector<char> mapped(n);
+    for (uint32_t ri = 0; ri < n; ri++) {
+        const uint32_t r = ref.order[ri];  // try roots in the reference's order
+        std::fill(mapped.begin(), mapped.end(), 0);
+        mapped[r] = 1;
+        std::vector<uint32_t> order{r};
+        double card = 0, cost = 0;
+        for (uint32_t d = 1; d < n; d++) {
+            double best_next = 0, best_drv = 0, best_card0 = 0;
+            uint32_t best_u = UINT32_MAX, best_links = 0;
+            for (uint32_t u = 0; u < n; u++) {
+                if (mapped[u]) continue;
+                double drv = -1, keep = 1, card0 = -1;
+                uint32_t links = 0;
+                for (auto &x : qe) {
+                    uint32_t w;
+                    if (x.a == u && x.b == u) {
+                        keep *= std::min(1.0, (x.E[0] / std::max(1.0, x.R[0])) / V);
+                        continue;
+                    } else if (x.a == u && mapped[x.b]) {
+                        w = x.b;
+                    } else if (x.b == u && mapped[x.a]) {
+                        w = x.a;
+                    } else {
+                        continue;
+                    }
+                    links++;
+                    const int dd = dir_from(x, w);
+                    // depth 1: every edge of the label out of a root
+                    const double f = d == 1 ? x.E[dd] : x.fan[dd];
+                    if (d == 1 && (card0 < 0 || x.R[dd] < card0)) card0 = x.R[dd];
+                    if (drv < 0 || f < drv) {
+                        if (drv >= 0) keep *= std::min(1.0, drv / (d == 1 ? V * V : V));
+                        drv = f;
+                    } else {
+                        keep *= std::min(1.0, f / (d == 1 ? V * V : V));
+                    }
+                }
+                if (!links) continue;
+                const double next = d == 1 ? drv * keep : card * drv * keep;
+                if (best_u == UINT32_MAX || next < best_next || (next == best_next && links > best_links)) {
+                    best_u = u;
+                    best_next = next;
+                    best_drv = drv;
+                    best_card0 = card0;
+                    best_links = links;
+                }
+            }
+            if (best_u == UINT32_MAX) return fail(FM_ERR_GENERIC, "query graph is disconnected");
+            if (d == 1) {
+                card = best_card0;
+                cost = best_card0;
+                // the root's rows of the label: E candidates in all
+                cost += best_drv;
+            } else {
+                cost += card * best_drv;
+            }
+            card = best_next;
+            cost += card;
+            mapped[best_u] = 1;
+            order.push_back(best_u);
+        }
+        if (best_order.empty() || cost < best_cost) {
+            best_cost = cost;
+            best_order = order;
+        }
+    }
+    return plan_from_order(q, best_order, p);
 }
 
 // ---------------------------------------------------------------------------
@@ -593,6 +735,8 @@ static int load_bin(const std::string &path, HostGraph &g) {
     if (ok && !g.symmetric) ok = rcsr(g.in);
     fclose(f);
     if (!ok) return fail(FM_ERR_DATA_BAD_BODY, "truncated binary graph %s", path.c_str());
+    csr_stats(g.out, g.V, (uint32_t)L);
+    if (!g.symmetric) csr_stats(g.in, g.V, (uint32_t)L);
     return FM_OK;
 }
 
@@ -1057,10 +1201,8 @@ int fm_query_edges(const fm_query *q, uint32_t *src, uint32_t *dst, uint32_t *la
     return FM_OK;
 }
 
-int fm_query_plan(const fm_query *q, fm_plan_info *info, uint32_t *order, uint32_t *req_begin,
-                  uint32_t *req_anchor, uint32_t *req_label, uint32_t *req_dir) {
-    if (!q) return fail(FM_ERR_INVALID_ARG, "null argument");
-    const Plan &p = q->h.plan;
+static void export_plan(const Plan &p, fm_plan_info *info, uint32_t *order, uint32_t *req_begin,
+                        uint32_t *req_anchor, uint32_t *req_label, uint32_t *req_dir) {
     if (info) {
         info->vertices = p.n;
         info->nreqs = (uint32_t)p.reqs.size();
@@ -1078,6 +1220,27 @@ int fm_query_plan(const fm_query *q, fm_plan_info *info, uint32_t *order, uint32
         if (req_label) req_label[i] = p.reqs[i].label;
         if (req_dir) req_dir[i] = p.reqs[i].dir;
     }
+}
+
+int fm_query_plan(const fm_query *q, fm_plan_info *info, uint32_t *order, uint32_t *req_begin,
+                  uint32_t *req_anchor, uint32_t *req_label, uint32_t *req_dir) {
+    if (!q) return fail(FM_ERR_INVALID_ARG, "null argument");
+    export_plan(q->h.plan, info, order, req_begin, req_anchor, req_label, req_dir);
+    return FM_OK;
+}
+
+int fm_query_plan_on(const fm_graph *g, const fm_query *q, uint32_t plan, fm_plan_info *info, uint32_t *order,
+                     uint32_t *req_begin, uint32_t *req_anchor, uint32_t *req_label, uint32_t *req_dir) {
+    if (!g || !q) return fail(FM_ERR_INVALID_ARG, "null argument");
+    if (plan == FM_PLAN_REFERENCE) {
+        export_plan(q->h.plan, info, order, req_begin, req_anchor, req_label, req_dir);
+        return FM_OK;
+    }
+    if (plan != FM_PLAN_COST) return fail(FM_ERR_INVALID_ARG, "unknown plan %u", plan);
+    Plan p;
+    int rc = make_cost_plan(q->h, g->h, p);
+    if (rc) return rc;
+    export_plan(p, info, order, req_begin, req_anchor, req_label, req_dir);
     return FM_OK;
 }
 

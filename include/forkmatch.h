@@ -184,6 +184,12 @@ typedef struct fm_plan_info {
 int fm_query_plan(const fm_query *q, fm_plan_info *info, uint32_t *order,
                   uint32_t *req_begin, uint32_t *req_anchor,
                   uint32_t *req_label, uint32_t *req_dir);
+/* The plan fm_match uses for q on graph g under `plan` (FM_PLAN_*), in the
+ * layout of fm_query_plan. */
+int fm_query_plan_on(const fm_graph *g, const fm_query *q, uint32_t plan,
+                     fm_plan_info *info, uint32_t *order, uint32_t *req_begin,
+                     uint32_t *req_anchor, uint32_t *req_label,
+                     uint32_t *req_dir);
 void fm_query_destroy(fm_query *q);
 
 /* ---- engine -------------------------------------------------------------
@@ -223,6 +229,12 @@ int fm_engine_info_get(fm_engine *e, fm_engine_info *out);
 #define FM_MODE_COUNT 0u  /* count only; the last level is never stored */
 #define FM_MODE_TABLE 1u  /* count and store every embedding */
 
+/* Matching order. The embedding set does not depend on it; the work does.
+ * FM_PLAN_COST orders by estimated candidates from the data graph's label
+ * statistics (no reference counterpart); FM_PLAN_REFERENCE is the
+ * reference's query-degree order (src/ccsr.cuh:229-249). */
+enum { FM_PLAN_COST = 0, FM_PLAN_REFERENCE = 1 };
+
 typedef struct fm_match_opts {
     uint32_t mode;            /* FM_MODE_COUNT or FM_MODE_TABLE */
     uint32_t instrument;      /* 1: gather algorithmic-byte counters */
@@ -232,7 +244,7 @@ typedef struct fm_match_opts {
     uint32_t grid_waves;      /* persistent waves (0 = full residency) */
     uint32_t task_rows;       /* parents per queued task (SubMaxMulSpill) */
     uint32_t root_chunk;      /* roots per claim (TooLargeThreshold) */
-    uint32_t reserved0;
+    uint32_t plan;            /* FM_PLAN_COST (0) or FM_PLAN_REFERENCE */
     /* abort a query once it has stored more than this many intermediate
      * partial embeddings (0 = none); its status becomes COUNT_LIMIT */
     uint64_t max_partials;
@@ -313,8 +325,12 @@ typedef struct fm_server_config {
     uint32_t root_chunk;     /* roots per root task (default TooLargeThreshold) */
     uint32_t task_rows;      /* parents per task (default SubMaxMulSpill) */
     uint32_t table;          /* 1: store tables (default), 0: count only */
-    uint32_t max_launch_ms;  /* lifetime cap of one launch (default 120000) */
-    uint32_t reserved[8];
+    uint32_t max_launch_ms;  /* deadline of one launch (default 120000) */
+    uint32_t plan;           /* FM_PLAN_COST (0) or FM_PLAN_REFERENCE */
+    uint32_t slice_ms;       /* a launch retires after this long and the next
+                                submit/poll relaunches it; queued work stays
+                                on the device (default 2000) */
+    uint32_t reserved[6];
 } fm_server_config;
 typedef struct fm_server_completion {
     uint32_t ticket;         /* returned by fm_server_submit */

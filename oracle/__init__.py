@@ -32,9 +32,10 @@ def lib() -> C.CDLL:
         h.oracle_graph_symmetric.restype = C.c_int
         h.oracle_graph_symmetric.argtypes = [C.c_void_p]
         h.oracle_plan.restype = C.c_int
-        h.oracle_plan.argtypes = [C.c_uint32, C.c_uint32, u32p, u32p, u32p, u32p, u32p, u32p, u32p, u32p, u32p]
+        h.oracle_plan.argtypes = [C.c_uint32, C.c_uint32, u32p, u32p, u32p, u32p, u32p, u32p, u32p, u32p, u32p,
+                                  u32p]
         h.oracle_match.restype = C.c_int
-        h.oracle_match.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, u32p, u32p, u32p, C.c_int, C.c_uint64,
+        h.oracle_match.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, u32p, u32p, u32p, u32p, C.c_int, C.c_uint64,
                                    C.c_uint64, C.c_uint64, C.c_int, C.c_uint64, u32p, C.c_int, u64p, u64p, u64p, u64p,
                                    C.POINTER(C.c_double)]
         _lib = h
@@ -74,25 +75,40 @@ class OracleResult:
     seconds: float
 
 
-def plan(n: int, edges):
+def _order(order, n):
+    if order is None:
+        return None, None
+    o = _u32(order)
+    if len(o) != n:
+        raise ValueError("order must name every query vertex once")
+    return o, _p(o)
+
+
+def plan(n: int, edges, order=None):
+    """The reference's plan, or the requirements of a given matching order."""
+    keep, op = _order(order, n)
     e = np.asarray(edges, dtype=np.uint32).reshape(-1, 3)
     s, d, l = _u32(e[:, 0]), _u32(e[:, 1]), _u32(e[:, 2])
     cap = max(1, 4 * len(e) + 4)
-    order = np.zeros(max(n, 1), np.uint32)
     rb = np.zeros(n + 1, np.uint32)
     an, lb, dr = (np.zeros(cap, np.uint32) for _ in range(3))
     nr = C.c_uint32()
-    rc = lib().oracle_plan(n, len(e), _p(s), _p(d), _p(l), _p(order), _p(rb), _p(an), _p(lb), _p(dr),
+    out = np.zeros(max(n, 1), np.uint32)
+    rc = lib().oracle_plan(n, len(e), _p(s), _p(d), _p(l), op, _p(out), _p(rb), _p(an), _p(lb), _p(dr),
                            C.byref(nr))
     if rc:
         raise ValueError(f"oracle planner error {rc}")
     k = nr.value
-    return order[:n].tolist(), rb.tolist(), an[:k].tolist(), lb[:k].tolist(), dr[:k].tolist()
+    return out[:n].tolist(), rb.tolist(), an[:k].tolist(), lb[:k].tolist(), dr[:k].tolist()
 
 
 def match(graph: OracleGraph, n: int, edges, threads: int = 1, table: bool = True, max_rows: int | None = None,
-          instrument: bool = False, roots: tuple[int, int] | None = None, stride: int = 1) -> OracleResult:
-    """Full enumeration; rows sorted lexicographically, columns in query vertex order."""
+          instrument: bool = False, roots: tuple[int, int] | None = None, stride: int = 1,
+          order=None) -> OracleResult:
+    """Full enumeration; rows sorted lexicographically, columns in query vertex order.
+    `order` replaces the reference's matching order (same embeddings, its
+    own partials per depth)."""
+    keep, op = _order(order, n)
     e = np.asarray(edges, dtype=np.uint32).reshape(-1, 3)
     s, d, l = _u32(e[:, 0]), _u32(e[:, 1]), _u32(e[:, 2])
     rb, re_ = (0, 2 ** 64 - 1) if roots is None else roots
@@ -102,13 +118,13 @@ def match(graph: OracleGraph, n: int, edges, threads: int = 1, table: bool = Tru
     if table:
         if max_rows is None:
             # a count-only pass sizes the table exactly
-            first = match(graph, n, edges, threads, table=False, roots=roots, stride=stride)
+            first = match(graph, n, edges, threads, table=False, roots=roots, stride=stride, order=order)
             max_rows = first.count
         buf = np.zeros((max(max_rows, 1), n), np.uint32)
     else:
         max_rows = 0
         buf = np.zeros((1, max(n, 1)), np.uint32)
-    rc = lib().oracle_match(graph._h, n, len(e), _p(s), _p(d), _p(l), threads, rb, re_, stride, 1 if table else 0,
+    rc = lib().oracle_match(graph._h, n, len(e), _p(s), _p(d), _p(l), op, threads, rb, re_, stride, 1 if table else 0,
                             max_rows, _p(buf), 1 if instrument else 0, C.byref(cnt), C.byref(rows),
                             C.byref(words), partials.ctypes.data_as(u64p), C.byref(secs))
     if rc:

@@ -203,7 +203,7 @@ static int oreq_cmp(const void *x, const void *y) {
  * exit(1), src/ccsr.cuh:251-254), 2 for bad input. Duplicate lines are
  * collapsed first. */
 static int make_plan(uint32_t n, uint32_t m, const uint32_t *qs, const uint32_t *qd,
-                     const uint32_t *ql, oplan *p) {
+                     const uint32_t *ql, const uint32_t *order_in, oplan *p) {
     if (n == 0 || n > OMAXN) return 2;
     uint32_t es[OMAXN * OMAXN * 4][3];
     uint32_t ne = 0;
@@ -240,16 +240,23 @@ static int make_plan(uint32_t n, uint32_t m, const uint32_t *qs, const uint32_t 
     int state[OMAXN];
     for (uint32_t i = 0; i < n; i++) state[i] = -1;
     p->n = n;
-    state[rankv[0]] = 0;
+    if (order_in && order_in[0] >= n) return 2;
+    state[order_in ? order_in[0] : rankv[0]] = 0;
     p->nreqs = 0;
     for (uint32_t d = 0; d < n; d++) {
-        /* the frontier vertex of smallest rank */
         int pick = -1;
-        for (uint32_t r = 0; r < n; r++)
-            if (state[rankv[r]] == 0) {
-                pick = (int)rankv[r];
-                break;
-            }
+        if (order_in) {
+            /* a given order: each vertex must be on the frontier when taken */
+            if (order_in[d] >= n || state[order_in[d]] != 0) return 2;
+            pick = (int)order_in[d];
+        } else {
+            /* the frontier vertex of smallest rank */
+            for (uint32_t r = 0; r < n; r++)
+                if (state[rankv[r]] == 0) {
+                    pick = (int)rankv[r];
+                    break;
+                }
+        }
         if (pick < 0) return 1;
         uint32_t u = (uint32_t)pick;
         state[u] = 1;
@@ -286,10 +293,10 @@ static int make_plan(uint32_t n, uint32_t m, const uint32_t *qs, const uint32_t 
 }
 
 int oracle_plan(uint32_t n, uint32_t m, const uint32_t *qs, const uint32_t *qd, const uint32_t *ql,
-                uint32_t *order, uint32_t *req_begin, uint32_t *anchor, uint32_t *label,
-                uint32_t *dir, uint32_t *nreqs) {
+                const uint32_t *order_in, uint32_t *order, uint32_t *req_begin, uint32_t *anchor,
+                uint32_t *label, uint32_t *dir, uint32_t *nreqs) {
     oplan *p = (oplan *)malloc(sizeof(oplan));
-    int rc = make_plan(n, m, qs, qd, ql, p);
+    int rc = make_plan(n, m, qs, qd, ql, order_in, p);
     if (rc == 0) {
         memcpy(order, p->order, sizeof(uint32_t) * n);
         memcpy(req_begin, p->req_begin, sizeof(uint32_t) * (n + 1));
@@ -435,7 +442,9 @@ static double now_s(void) {
 
 /*
  * Enumerates every embedding of the query (n vertices, m directed lines) in
- * graph gp. Roots are restricted to root_begin, root_begin + root_stride, ...
+ * graph gp, in the reference's matching order, or in order_in when it is not
+ * NULL (each vertex adjacent to an earlier one; the embedding set is the
+ * same, the partials per depth are those of that order). Roots are restricted to root_begin, root_begin + root_stride, ...
  * below root_end (pass 0, UINT64_MAX and 1 for all). threads <= 0 selects 1. Writes up to max_rows rows
  * (query-vertex column order) into table when want_table. Outputs the count,
  * the rows written, the neighbour words of all requirement rows of all
@@ -444,14 +453,14 @@ static double now_s(void) {
  * Returns 0, or the planner's error (1 disconnected, 2 bad query).
  */
 int oracle_match(void *gp, uint32_t n, uint32_t m, const uint32_t *qs, const uint32_t *qd,
-                 const uint32_t *ql, int threads, uint64_t root_begin, uint64_t root_end, uint64_t root_stride,
-                 int want_table, uint64_t max_rows, uint32_t *table, int instrument,
-                 uint64_t *count, uint64_t *rows, uint64_t *row_words, uint64_t *partials,
+                 const uint32_t *ql, const uint32_t *order_in, int threads, uint64_t root_begin,
+                 uint64_t root_end, uint64_t root_stride, int want_table, uint64_t max_rows, uint32_t *table,
+                 int instrument, uint64_t *count, uint64_t *rows, uint64_t *row_words, uint64_t *partials,
                  double *seconds) {
     const ograph *g = (const ograph *)gp;
     octx *c = (octx *)calloc(1, sizeof(octx));
     oplan *p = (oplan *)malloc(sizeof(oplan));
-    int rc = make_plan(n, m, qs, qd, ql, p);
+    int rc = make_plan(n, m, qs, qd, ql, order_in, p);
     if (rc) {
         free(p);
         free(c);

@@ -171,15 +171,36 @@ def test_self_loops(engine):
         _check(engine, g, og, fm.Query.from_edges(n, e), str(e))
 
 
-def test_instrumented_counters_match_oracle(engine, er):
+@pytest.mark.parametrize("plan", [fm.PLAN_REFERENCE, fm.PLAN_COST])
+def test_instrumented_counters_match_oracle(engine, er, plan):
     g, og = er
     for name in ["_3_triangle", "_4_square", "_5_road5"]:
         q = fm.Query.load_g(str(fixture_path(name)))
-        ref = orc.match(og, q.vertices, q.edges(), table=False, instrument=True)
-        out = engine.match(g, q, table=False, instrument=True, timeout_ms=TIMEOUT_MS).query(0)
+        order = q.plan(g, plan=plan).order
+        ref = orc.match(og, q.vertices, q.edges(), table=False, instrument=True, order=order)
+        out = engine.match(g, q, table=False, instrument=True, timeout_ms=TIMEOUT_MS, plan=plan).query(0)
         assert out.count == ref.count
         assert out.alg_row_bytes == 4 * ref.row_words, name
         assert out.partials == ref.partials, name
+
+
+def test_cost_and_reference_plans_give_the_same_table(engine):
+    """Random-walk queries on a skewed R-MAT graph with Zipf labels: both
+    matching orders give the oracle's sorted table."""
+    g = fm.Graph.rmat(11, 12000, 12, zipf_s=1.2, seed=21).upload()
+    og = _oracle_graph(g)
+    # seeds whose cost-based order differs from the reference order, with
+    # 875 .. 480,028 embeddings
+    for i, (n, shape, seed) in enumerate([(5, 0, 401), (6, 0, 404), (6, 1, 401), (7, 1, 400), (7, 2, 405),
+                                          (9, 2, 405)]):
+        q = fm.Query.random_walk(g, n, shape=shape, min_cycles=1 if shape == 1 else 0, seed=seed)
+        assert q.plan(g).order != q.plan().order
+        ref = orc.match(og, n, q.edges(), threads=4)
+        for plan in (fm.PLAN_COST, fm.PLAN_REFERENCE):
+            r = engine.match(g, q, timeout_ms=TIMEOUT_MS, plan=plan)
+            assert r.count(0) == ref.count, (i, plan)
+            assert np.array_equal(r.table(0), ref.table), (i, plan)
+            r.close()
 
 
 def test_table_truncation_keeps_exact_count(engine, er):
@@ -284,16 +305,20 @@ def test_hub_rows_are_split_across_waves(engine):
     assert split > 0
 
 
-def test_serving_results_equal_alone_and_in_the_mix(er):
+@pytest.mark.parametrize("slice_ms,plan", [(0, fm.PLAN_COST), (1, fm.PLAN_REFERENCE)])
+def test_serving_results_equal_alone_and_in_the_mix(er, slice_ms, plan):
     """Continuous batching: queries submitted while others run are admitted
-    into the running launch; each result equals the query matched alone."""
+    into the running launch; each result equals the query matched alone.
+    With 1 ms slices the launch retires many times with work queued on the
+    device, and the next poll relaunches it."""
     import time
     g, og = er
     qs = [fm.Query.random_walk(g, 3 + i % 5, shape=i % 3, min_cycles=1 if i % 3 == 1 else 0, seed=900 + i)
           for i in range(40)]
     refs = [orc.match(og, q.vertices, q.edges()) for q in qs]
     with fm.Engine(soln_prealloc=1 << 29, output_prealloc=1 << 29, bank_size=1 << 13, output_page_bytes=1 << 16) as e:
-        with fm.Server(e, g, slots=16, slot_pages=64, idle_ms=5, max_launch_ms=60000) as srv:
+        with fm.Server(e, g, slots=16, slot_pages=64, idle_ms=5, max_launch_ms=60000, slice_ms=slice_ms,
+                       plan=plan) as srv:
             pending = {}
             done = {}
             i = 0

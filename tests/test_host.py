@@ -214,3 +214,54 @@ def test_cli_argv_and_parse_exit_codes(tmp_path):
     assert subprocess.run([str(CLI), "/nonexistent/q.g", d], capture_output=True).returncode == 232
     bad = write(tmp_path, "b.g", "t # 0\n2 2 1 1\nv 0 1\nv 1 1\ne 0 1 1\nq 1 0 1\nt # -1\n")
     assert subprocess.run([str(CLI), bad, d], capture_output=True).returncode == 233
+
+
+def _random_query(rng, n, labels):
+    edges = [(int(rng.integers(0, i)), i, int(rng.integers(0, labels))) for i in range(1, n)]
+    edges += [(int(rng.integers(0, n)), int(rng.integers(0, n)), int(rng.integers(0, labels)))
+              for _ in range(int(rng.integers(0, n)))]
+    return edges
+
+
+@pytest.mark.parametrize("directed", [False, True])
+def test_cost_plan_requirements_equal_oracle_for_its_order(directed):
+    """The product's requirements for its cost-based order equal the oracle's
+    requirements for the same order; the reference mode is the query's plan."""
+    rng = np.random.default_rng(11)
+    g = fm.Graph.rmat(10, 4000, 6, zipf_s=1.2, seed=3)
+    if directed:
+        s, d, l = g.edge_list()
+        keep = s < d
+        g = fm.Graph.from_edges(g.vertices, s[keep], d[keep], l[keep])
+    for t in range(150):
+        n = int(rng.integers(2, 10))
+        edges = _random_query(rng, n, 7)
+        q = fm.Query.from_edges(n, edges)
+        p = q.plan(g)
+        o = orc.plan(n, np.array(edges, np.uint32).reshape(-1, 3), order=p.order)
+        assert sorted(p.order) == list(range(n))
+        assert p.order == o[0]
+        assert p.req_begin == o[1]
+        assert (p.anchor, p.label, p.direction) == (o[2], o[3], o[4])
+        r = q.plan(g, plan=fm.PLAN_REFERENCE)
+        assert r == q.plan()
+
+
+def test_cost_plan_starts_at_the_rare_label():
+    """A path whose last edge has a label the graph holds twice starts there."""
+    rng = np.random.default_rng(2)
+    V = 500
+    s = rng.integers(0, V, 6000).astype(np.uint32)
+    d = rng.integers(0, V, 6000).astype(np.uint32)
+    lab = np.zeros(6000, np.uint32)
+    lab[:2] = 9
+    S = np.concatenate([s, d])
+    D = np.concatenate([d, s])
+    L = np.concatenate([lab, lab])
+    g = fm.Graph.from_edges(V, S, D, L)
+    path = [(0, 1, 0), (1, 2, 0), (2, 3, 0), (3, 4, 9)]
+    edges = path + [(b, a, c) for a, b, c in path]
+    p = fm.Query.from_edges(5, edges).plan(g)
+    assert set(p.order[:2]) == {3, 4}
+    # the reference order starts at the lowest id of highest out-degree
+    assert fm.Query.from_edges(5, edges).plan().order[0] == 1

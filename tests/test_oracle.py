@@ -228,3 +228,42 @@ def test_threads_and_root_sampling_agree():
     # strided root samples partition the roots
     parts = sum(orc.match(og, n, e, table=False, roots=(k, 400), stride=3).count for k in range(3))
     assert parts == one.count
+
+
+def _valid_orders(n, edges):
+    """Every order in which each vertex after the first touches an earlier one."""
+    nb = {u: set() for u in range(n)}
+    for a, b, _ in edges:
+        if a != b:
+            nb[a].add(b)
+            nb[b].add(a)
+    for perm in itertools.permutations(range(n)):
+        if all(any(w in nb[perm[k]] for w in perm[:k]) for k in range(1, n)):
+            yield list(perm)
+
+
+@pytest.mark.parametrize("gi", [0, 3])
+def test_any_matching_order_gives_the_same_embeddings(gi):
+    """The embedding set does not depend on the matching order; the partials
+    of each depth are those of the order (brute force over every prefix)."""
+    V, s, d, l = tiny_graphs()[gi]
+    og = orc.OracleGraph(V, s, d, l)
+    for n, e in QUERIES:
+        if n < 2 or n > 4:
+            continue
+        ref = brute_force(V, s, d, l, n, e)
+        ea = np.array(e, np.uint32).reshape(-1, 3)
+        for order in _valid_orders(n, e):
+            got = orc.match(og, n, ea, order=order)
+            assert np.array_equal(got.table, ref), (gi, e, order)
+            inst = orc.match(og, n, ea, table=False, instrument=True, order=order)
+            assert inst.partials == partials_brute(V, s, d, l, n, e, order), (gi, e, order)
+            assert orc.plan(n, ea, order=order)[0] == order
+
+
+def test_invalid_matching_order_is_rejected():
+    e = np.array([(0, 1, 0), (1, 2, 0), (2, 3, 0)], np.uint32)
+    with pytest.raises(ValueError):
+        orc.plan(4, e, order=[0, 2, 1, 3])  # 2 does not touch 0
+    with pytest.raises(ValueError):
+        orc.plan(4, e, order=[0, 1, 1, 3])
