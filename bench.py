@@ -276,7 +276,9 @@ def serve_rate(srv, pool, counts, rate: float, seconds: float, rng, balg=None, d
         tn = time.perf_counter()
         if tn - last_log > 20:
             last_log = tn
-            log(f"serve {rate:.0f} req/s: {ndone}/{n_req} done, {len(t2req)} in flight, {len(backlog)} waiting")
+            st = srv.stats()
+            log(f"serve {rate:.0f} req/s: {ndone}/{n_req} done, {len(t2req)} in flight, {len(backlog)} waiting; "
+                f"live banks {st.live_banks}, launches {st.launches}, refused {st.refused}")
         if tn - t0 > deadline_s:
             raise RuntimeError(f"serving run at {rate} req/s did not drain in {deadline_s} s "
                                f"({ndone}/{n_req} done)")
@@ -476,7 +478,8 @@ def main():
         rates = [float(x) for x in args.rates.split(",") if x]
         per_rate = []
         banks = engine.info().banks
-        with fm.Server(engine, g, slots=256, slot_pages=4096, bank_budget=max(1, banks * 3 // 4)) as srv:
+        with fm.Server(engine, g, slots=256, slot_pages=4096, bank_budget=max(1, banks * 3 // 4),
+                       max_launch_ms=30000) as srv:
             serve_rate(srv, queries, counts, rates[0], 1.0, rng)  # warmup
             for rate in rates:
                 pr = serve_rate(srv, queries, counts, rate, args.serve_seconds, rng, balg=balg)

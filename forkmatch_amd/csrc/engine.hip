@@ -118,7 +118,7 @@ struct Ctl {
     uint32_t error;
     unsigned long long t0;  // wall clock of the first wave (deadline origin)
     unsigned long long timeout_ticks;
-    uint32_t diag[12];  // the task that raised FM_ERR_INTERNAL
+    uint32_t diag[12];  // FM_ERR_INTERNAL: the task; FM_ERR_TIMEOUT: where the launch ran out of time
     alignas(128) uint32_t page_bump;
     alignas(128) unsigned long long stats[ST_N];
     // persistent
@@ -269,7 +269,7 @@ __device__ __forceinline__ bool cas_u32(uint32_t *p, uint32_t expect, uint32_t d
 __device__ __forceinline__ void backoff() { __builtin_amdgcn_s_sleep(2); }
 
 struct Ctl;
-__device__ bool spin_ok(const Ctl *c, uint32_t &iter);
+__device__ bool spin_ok(const Ctl *c, uint32_t &iter, uint32_t site, uint32_t a, uint32_t b);
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
@@ -327,17 +327,29 @@ __device__ __forceinline__ void row_of(const DGraph &g, uint32_t req, uint32_t v
 // Every spin-wait calls this between polls: it sleeps briefly and, every
 // 256 polls, gives up (returns false) once the launch is aborting or past
 // its deadline, so no wait can outlive the launch.
-__device__ bool spin_ok(const Ctl *c, uint32_t &iter) {
+// Records why the launch timed out (first caller wins): diag[0] = site,
+// diag[1..3] = site-specific values.
+__device__ void timeout_at(const Ctl *c, uint32_t site, uint32_t a, uint32_t b, uint32_t d) {
+    uint32_t zero = 0;
+    Ctl *w = const_cast<Ctl *>(c);
+    if (__hip_atomic_compare_exchange_strong(&w->error, &zero, (uint32_t)FM_ERR_TIMEOUT, __ATOMIC_RELAXED,
+                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+        w->diag[0] = site;
+        w->diag[1] = a;
+        w->diag[2] = b;
+        w->diag[3] = d;
+    }
+    __hip_atomic_store(&w->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ bool spin_ok(const Ctl *c, uint32_t &iter, uint32_t site, uint32_t a, uint32_t b) {
     backoff();
     if ((++iter & 255u) != 0) return true;
     if (__hip_atomic_load(&c->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
     unsigned long long t0 = __hip_atomic_load(&c->t0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     unsigned long long lim = __hip_atomic_load(&c->timeout_ticks, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (t0 && wall_clock64() - t0 > lim) {
-        uint32_t zero = 0;
-        __hip_atomic_compare_exchange_strong(const_cast<uint32_t *>(&c->error), &zero, (uint32_t)FM_ERR_TIMEOUT,
-                                             __ATOMIC_RELAXED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(const_cast<uint32_t *>(&c->abort), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        timeout_at(c, site, a, b, 0);
         return false;
     }
     return true;
@@ -345,10 +357,12 @@ __device__ bool spin_ok(const Ctl *c, uint32_t &iter) {
 
 // Waits until *p == want, polling with relaxed loads (no cache invalidation
 // per poll) and acquiring once at the end. false: the launch is aborting.
-__device__ bool wait_eq(const uint32_t *p, uint32_t want, const Ctl *c) {
+// `site` names the wait in a timeout report.
+__device__ bool wait_eq(const uint32_t *p, uint32_t want, const Ctl *c, uint32_t site) {
     uint32_t it = 0;
-    while (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != want)
-        if (!spin_ok(c, it)) return false;
+    uint32_t v;
+    while ((v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) != want)
+        if (!spin_ok(c, it, site, v, want)) return false;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     return true;
 }
@@ -375,7 +389,7 @@ __device__ uint32_t bank_alloc(const KParams &P, unsigned long long *stats) {
         if (a > 0) {
             uint32_t pos = atom_add(&c->f_head, 1u);
             FSlot *s = &P.free_ring[pos & (P.fcap - 1)];
-            if (!wait_eq(&s->seq, pos + 1, c)) return NO_BANK;
+            if (!wait_eq(&s->seq, pos + 1, c, 11)) return NO_BANK;
             uint32_t id = s->id;
             st_rel(&s->seq, pos + P.fcap);
             __hip_atomic_store(&P.bank_ref[id], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -402,7 +416,7 @@ __device__ void bank_release(const KParams &P, uint32_t id) {
         Ctl *c = P.ctl;
         uint32_t pos = atom_add(&c->f_tail, 1u);
         FSlot *s = &P.free_ring[pos & (P.fcap - 1)];
-        if (!wait_eq(&s->seq, pos, c)) return;
+        if (!wait_eq(&s->seq, pos, c, 12)) return;
         s->id = id;
         st_rel(&s->seq, pos + 1);
         atom_add(&c->f_avail, 1);
@@ -450,7 +464,7 @@ __device__ void push_tasks(const KParams &P, uint32_t lvl, uint32_t m, const Tas
     if (lane < m) {
         uint32_t pos = pos0 + lane;
         Slot *s = &P.ring[(size_t)lvl * P.qcap + (pos & (P.qcap - 1))];
-        if (wait_eq(&s->seq, pos, c)) {
+        if (wait_eq(&s->seq, pos, c, 13)) {
         s->q = t.q;
         s->depth = t.depth;
         s->bank = t.bank;
@@ -475,7 +489,7 @@ __device__ bool pop_task(const KParams &P, TaskDesc &t) {
         }
         uint32_t pos = atom_add(&c->lq[lvl].head, 1u);
         Slot *s = &P.ring[(size_t)lvl * P.qcap + (pos & (P.qcap - 1))];
-        if (!wait_eq(&s->seq, pos + 1, c)) return false;
+        if (!wait_eq(&s->seq, pos + 1, c, 14)) return false;
         t.q = s->q;
         t.depth = s->depth;
         t.bank = s->bank;
@@ -566,7 +580,7 @@ __device__ bool tab_acquire(const KParams &P, WaveTab &w, uint32_t q, uint32_t c
             if (atom_add(&c->pf_avail, -1) > 0) {
                 uint32_t pos = atom_add(&c->pf_head, 1u);
                 FSlot *fs = &P.pfree_ring[pos & (P.pfcap - 1)];
-                if (!wait_eq(&fs->seq, pos + 1, c)) return false;
+                if (!wait_eq(&fs->seq, pos + 1, c, 15)) return false;
                 id = fs->id;
                 st_rel(&fs->seq, pos + P.pfcap);
             } else {
@@ -583,7 +597,7 @@ __device__ bool tab_acquire(const KParams &P, WaveTab &w, uint32_t q, uint32_t c
             // over budget: give the page back
             uint32_t pos = atom_add(&c->pf_tail, 1u);
             FSlot *fs = &P.pfree_ring[pos & (P.pfcap - 1)];
-            if (wait_eq(&fs->seq, pos, c)) {
+            if (wait_eq(&fs->seq, pos, c, 16)) {
                 fs->id = id;
                 st_rel(&fs->seq, pos + 1);
                 atom_add(&c->pf_avail, 1);
@@ -785,7 +799,7 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
         if (lane == 0) stop = (ld_rlx(&P.ctl->abort) || ld_rlx(&Q.status)) ? 1u : 0u;
         if (bcast(stop, 0)) break;
         if (wall_clock64() > deadline) {
-            if (lane == 0) raise_error(P, FM_ERR_TIMEOUT);
+            if (lane == 0) timeout_at(P.ctl, 2, t.q, t.depth, t.count);
             break;
         }
         const uint32_t nb = min(WAVE, cnt_local - b0);
@@ -1045,6 +1059,7 @@ __device__ void complete_query(const KParams &P, uint32_t q) {
     st_sys(&d->npages, min(ld_rlx(&Q.npages), Q.max_pages));
     __hip_atomic_store(&d->ticket, Q.pad, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     atom_add(&P.ctl->n_active, -1);
+    st_sys(&P.srv->live_banks, (uint32_t)max(0, ld_rlx(&P.ctl->live_banks)));
 }
 
 // Lane 0: claims the next submission; returns its slot or NO_PAGE.
@@ -1070,7 +1085,7 @@ __device__ void admit(const KParams &P, uint32_t slot, unsigned long long *stats
         uint32_t id = P.slot_pages[(size_t)slot * P.max_slot_pages + i];
         uint32_t pos = atom_add(&c->pf_tail, 1u);
         FSlot *fs = &P.pfree_ring[pos & (P.pfcap - 1)];
-        if (wait_eq(&fs->seq, pos, c)) {
+        if (wait_eq(&fs->seq, pos, c, 17)) {
             fs->id = id;
             st_rel(&fs->seq, pos + 1);
             atom_add(&c->pf_avail, 1);
@@ -1139,7 +1154,7 @@ __global__ __launch_bounds__(WAVE) void fm_match_kernel(KParams P) {
             if (ld_rlx(&P.ctl->abort)) {
                 flag = 2;
             } else if (wall_clock64() > deadline) {
-                raise_error(P, FM_ERR_TIMEOUT);
+                timeout_at(P.ctl, 1, (uint32_t)ld_rlx(&P.ctl->n_active), (uint32_t)ld_rlx(&P.ctl->pending), idle);
                 flag = 2;
             } else if (wall_clock64() > retire_at) {
                 // serving: leave between tasks; queued tasks, admitted
@@ -1156,7 +1171,7 @@ __global__ __launch_bounds__(WAVE) void fm_match_kernel(KParams P) {
                         s_task.q = slot;
                         flag = 3;
                     } else if (ld_sys(&P.srv->abort)) {
-                        raise_error(P, FM_ERR_TIMEOUT);
+                        timeout_at(P.ctl, 3, (uint32_t)ld_rlx(&P.ctl->n_active), 0, 0);
                         flag = 2;
                     } else if (ld_rlx(&P.ctl->n_active) <= 0 &&
                                (ld_sys(&P.srv->shutdown) || (idle_since && wall_clock64() - idle_since > P.idle_ticks))) {
@@ -1498,6 +1513,18 @@ static int full_reset(fm_engine *e) {
     HIP_TRY(hipStreamSynchronize(e->stream));
     e->dirty = false;
     return FM_OK;
+}
+
+// Where a launch ran out of time, from Ctl::diag.
+static std::string timeout_site(const Ctl &c) {
+    static const char *names[] = {"?", "main loop", "task", "host abort", "?", "?", "?", "?", "?", "?", "?",
+                                  "bank from free ring", "bank release", "task push", "task pop",
+                                  "page from free ring", "page give-back", "page recycle"};
+    const uint32_t site = c.diag[0];
+    char buf[160];
+    snprintf(buf, sizeof buf, "at %s, values %u %u %u", site < sizeof(names) / sizeof(names[0]) ? names[site] : "?",
+             c.diag[1], c.diag[2], c.diag[3]);
+    return buf;
 }
 
 static uint32_t pow2_at_least(uint64_t x) {
@@ -1910,6 +1937,8 @@ int fm_match_batch(fm_engine *e, const fm_graph *g, const fm_query *const *qs, u
                     "bank_words=%llu)",
                     dg[0], dg[1], dg[2], dg[3], dg[4], dg[5], dg[6], dg[7], dg[8], (unsigned long long)e->bank_words);
     }
+    if (ctl_back.error == FM_ERR_TIMEOUT)
+        return fail(FM_ERR_TIMEOUT, "match launch passed its deadline (%s)", timeout_site(ctl_back).c_str());
     if (ctl_back.error)
         return fail((int)ctl_back.error, "match launch aborted with status %u", ctl_back.error);
     return FM_OK;
@@ -2092,11 +2121,9 @@ static int server_launch(fm_server *s) {
     return FM_OK;
 }
 
-// A submission not yet admitted, or an admitted request whose completion
-// the device has not published.
+// A request (admitted or not) whose completion the device has not
+// published.
 static bool server_has_work(const fm_server *s) {
-    uint32_t admitted = __atomic_load_n(&s->sh->admitted, __ATOMIC_ACQUIRE);
-    if (admitted != s->sub_tail) return true;
     for (size_t i = 0; i < s->slot_state.size(); i++)
         if (s->slot_state[i] == 1 && __atomic_load_n(&s->hdone[i].ticket, __ATOMIC_ACQUIRE) != s->slot_ticket[i])
             return true;
@@ -2115,6 +2142,8 @@ static int server_pump(fm_server *s) {
         HIP_TRY(hipMemcpy(&back, s->e->ctl, CTL_RESET_BYTES, hipMemcpyDeviceToHost));
         if (back.error) {
             s->e->dirty = true;
+            if (back.error == FM_ERR_TIMEOUT)
+                return fail(FM_ERR_TIMEOUT, "serving kernel passed its deadline (%s)", timeout_site(back).c_str());
             return fail((int)back.error, "serving kernel aborted with status %u", back.error);
         }
     }
@@ -2151,6 +2180,7 @@ int fm_server_create(fm_engine *e, const fm_graph *g, const fm_server_config *cf
         return rc;
     };
     if (S > e->max_queries) return fail_out(fail(FM_ERR_INVALID_ARG, "slots %u exceed the engine's max_queries", S));
+    if (S > SUB_CAP) return fail_out(fail(FM_ERR_INVALID_ARG, "slots %u exceed the submission ring (%u)", S, SUB_CAP));
     HIP_TRY(hipSetDevice(e->device));
     if (hipHostMalloc((void **)&s->sh, sizeof(SrvShared), hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess ||
         hipHostMalloc((void **)&s->hplans, S * sizeof(DPlan), hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess ||
@@ -2201,9 +2231,13 @@ static int slot_of(const fm_server *s, uint32_t ticket) {
 
 int fm_server_submit(fm_server *s, const fm_query *q, uint64_t max_rows, uint32_t *ticket) {
     if (!s || !q || !ticket) return fail(FM_ERR_INVALID_ARG, "null argument");
+    // The device publishes live_banks at admissions and completions; with
+    // nothing in flight the budget cannot be held by this server's requests,
+    // so an idle server always admits. Every submission holds a slot until
+    // it completes and slots <= SUB_CAP, so the ring cannot overrun.
     uint32_t live = __atomic_load_n(&s->sh->live_banks, __ATOMIC_RELAXED);
-    uint32_t admitted = __atomic_load_n(&s->sh->admitted, __ATOMIC_ACQUIRE);
-    if (s->free_slots.empty() || live >= s->cfg.bank_budget || s->sub_tail - admitted >= SUB_CAP) {
+    const bool idle = s->free_slots.size() == s->slot_state.size();
+    if (s->free_slots.empty() || (!idle && live >= s->cfg.bank_budget)) {
         s->st.refused++;
         return fail(FM_ERR_BUSY, "no slot or bank budget available");
     }
