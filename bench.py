@@ -422,7 +422,7 @@ def main():
         for _ in range(args.warmup):
             for q in queries:
                 engine.match(g, q, timeout_ms=300000)
-        step_times, kernel_ms, lat = [], np.zeros(len(queries)), []
+        step_times, kernel_ms, lat, busy = [], np.zeros(len(queries)), [], []
         last = []
         for _ in range(args.steps):
             ts = time.perf_counter()
@@ -433,6 +433,7 @@ def main():
                 lat.append((time.perf_counter() - tq) * 1e3)
                 st = r.stats()
                 kernel_ms[i] += st.kernel_ms
+                busy.append(st.busy_frac * st.kernel_ms)
                 last.append(r.query(0))
                 r.close()
             step_times.append(time.perf_counter() - ts)
@@ -459,6 +460,7 @@ def main():
             "kernel_embeddings_per_s": embeddings / (dom_ms / 1e3),
             "latency_ms_p50": float(np.percentile(lat, 50)),
             "latency_ms_p99": float(np.percentile(lat, 99)),
+            "wave_busy_frac": float(sum(busy) / (dom_ms * args.steps)),
             "parity_checks_passed": bool(ok),
             "embeddings_per_step": embeddings,
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -516,7 +518,7 @@ def main():
         workload = f"rmat22_serving_{len(queries)}q_poisson"
     else:
         from forkmatch_amd.autotune import autotune
-        space = {"root_chunk": [256, 4096], "task_rows": [256, 1024], "grid_frac": [0.5, 1.0]}
+        space = {"root_chunk": [256, 4096], "task_rows": [64, 1024], "grid_frac": [0.5, 1.0]}
         tune = autotune(engine, g, queries, space=space, table=False, log=log)
         best = tune.best
         log(f"autotune: best {best} {tune.best_ms:.2f} ms over {len(tune.trials)} configurations")
@@ -543,6 +545,7 @@ def main():
             rb = engine.match_batch(g, queries, timeout_ms=300000, **best)
             step_times.append(time.perf_counter() - ts)
             kms.append(rb.stats().kernel_ms)
+            busy_b = rb.stats().busy_frac
         ok = all(rb.query(i).count == counts[i] == plain_counts[i] and rb.query(i).status == 0
                  for i in range(len(queries)))
         embeddings = sum(rb.query(i).count for i in range(len(queries)))
@@ -562,6 +565,7 @@ def main():
                           for k, name in ((0, "tree"), (1, "cycle+chord"), (2, "near-clique"))},
             "embeddings_per_step": embeddings,
             "embeddings_per_s": embeddings / (ms_step / 1e3),
+            "wave_busy_frac": busy_b,
             "parity_checks_passed": bool(ok),
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": achieved / HBM_PEAK_GBS, "traffic": None, "alg_bytes_per_step": sum(balg),

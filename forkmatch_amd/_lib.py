@@ -55,7 +55,7 @@ class LaunchStats(C.Structure):
     _fields_ = [("kernel_ms", f64), ("total_ms", f64), ("tasks_pushed", u64), ("root_claims", u64),
                 ("banks_allocated", u64), ("banks_reused", u64), ("alloc_failures", u64),
                 ("pages_allocated", u64), ("tasks_split", u64), ("requeues", u64), ("grid_waves", u32),
-                ("status", u32)]
+                ("status", u32), ("busy_frac", f64)]
 
 
 class ServerConfig(C.Structure):

@@ -97,6 +97,8 @@ enum StatIdx {
     ST_PAGES,
     ST_REQUEUE,
     ST_SPLIT,
+    ST_BUSY,   // wall-clock ticks waves spent inside tasks and admissions
+    ST_ALIVE,  // wall-clock ticks waves were resident
     ST_N
 };
 
@@ -555,6 +557,25 @@ __device__ void tab_release(const KParams &P, WaveTab &w) {
     w.page = NO_PAGE;
 }
 
+// Lane 0 only: publishes the rows appended since the last sync (page fill
+// and the query's row count) and keeps the page.
+__device__ void tab_sync(const KParams &P, WaveTab &w) {
+    if (w.page == NO_PAGE || w.fill == w.fill0) return;
+    st_rlx(&P.page_fill[w.page], w.fill);
+    atom_add(&P.qs[w.q].rows, (unsigned long long)(w.fill - w.fill0));
+    w.fill0 = w.fill;
+}
+
+// Lane 0, at the end of a final-level task. A batch launch keeps the page
+// for the wave's next task of the same query, so a query uses about
+// rows / page_rows + waves pages however small its tasks are. A serving
+// launch hands the page on: a completed query's slot is recycled with its
+// pages, so no wave may hold one past the query's last task.
+__device__ void tab_task_end(const KParams &P, WaveTab &w) {
+    if (P.serving) tab_release(P, w);
+    else tab_sync(P, w);
+}
+
 // Lane 0 only. Takes the query's spare page (if allowed) or claims a fresh
 // one; false when the query's page budget or the arena is exhausted.
 __device__ bool tab_acquire(const KParams &P, WaveTab &w, uint32_t q, uint32_t cap, bool allow_spare,
@@ -667,21 +688,24 @@ struct WaveOut {
     uint32_t run_begin;  // word offset of the rows not yet pushed
 };
 
+// LDS words holding a batch's parent rows and requirement rows (start,
+// length); a batch takes min(64, POOL_WORDS / (W + 2k)) parents of width W
+// with k requirement rows each, so the pool fits any plan (W <= 32, k <= 16).
+constexpr uint32_t POOL_WORDS = 2048;
+static_assert(POOL_WORDS / (MAXN + 2 * MAXR) >= 1, "pool too small for the widest batch");
+
 struct Lds {
-    uint32_t par[WAVE * MAXN];          // parent rows of the batch
-    uint32_t rows_st[WAVE * MAXR];      // requirement rows: start
-    uint32_t rows_len[WAVE * MAXR];     // requirement rows: length
+    uint32_t pool[POOL_WORDS];          // parents | rows start | rows length
     uint32_t dstart[WAVE];              // driver row start
     uint32_t cum[WAVE];                 // inclusive scan of driver lengths
     uint32_t didx[WAVE];                // driver requirement index
-    uint32_t pad[WAVE];
     // instrumented launches: per-task sums, flushed once per task (lane 0)
     unsigned long long inst[4 + MAXN];  // row words, lookups, candidates, probes, partials
 };
 
-// Pushes the rows written since run_begin as tasks of at most task_rows
+// Pushes the rows written since run_begin as tasks of at most `per`
 // parents each (depth D, row width D + 1).
-__device__ void flush_run(const KParams &P, WaveOut &o, uint32_t q, uint32_t D, uint32_t lvl,
+__device__ void flush_run(const KParams &P, WaveOut &o, uint32_t q, uint32_t D, uint32_t lvl, uint32_t per,
                           unsigned long long *stats) {
     const uint32_t W = D + 1;
     uint32_t rows = (o.used - o.run_begin) / W;
@@ -691,7 +715,6 @@ __device__ void flush_run(const KParams &P, WaveOut &o, uint32_t q, uint32_t D, 
         unsigned long long stored = atom_add(&Q.stored, (unsigned long long)rows) + rows;
         if (Q.max_partials && stored > Q.max_partials) query_fail(P, q, FM_ERR_COUNT_LIMIT);
     }
-    uint32_t per = P.task_rows;
     uint32_t ntasks = (rows + per - 1) / per;
     const uint32_t lane = lane_id();
     for (uint32_t t0 = 0; t0 < ntasks; t0 += WAVE) {
@@ -777,7 +800,7 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
         }
         if (lane == 0) {
             atom_add(&Q.count, cnt_acc);
-            if (P.table) tab_release(P, wt);
+            if (P.table) tab_task_end(P, wt);
         }
         return;
     }
@@ -790,10 +813,20 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
     const uint32_t nself = pl.nself[D];
     const uint32_t k = pl.req_begin[D + 1] - rb - nself;  // anchored requirements
 
+    // batch layout in the LDS pool
+    const uint32_t NB = min(WAVE, POOL_WORDS / (W + 2 * k));
+    uint32_t *const par = s.pool;
+    uint32_t *const rows_st = s.pool + NB * W;
+    uint32_t *const rows_len = rows_st + NB * k;
+
+    // Children go out as tasks of up to task_rows parents; while waves are
+    // idle, of one batch (64) so that they spread sooner.
+    uint32_t run_rows = P.task_rows;
+
     unsigned long long cnt_acc = 0;
     uint32_t cnt_local = t.count;       // shrinks when the upper parents are donated
     uint32_t last_end = t.last_end;     // end of the last parent's driver range
-    for (uint32_t b0 = 0; b0 < cnt_local; b0 += WAVE) {
+    for (uint32_t b0 = 0; b0 < cnt_local; b0 += NB) {
         // periodic checks
         uint32_t stop = 0;
         if (lane == 0) stop = (ld_rlx(&P.ctl->abort) || ld_rlx(&Q.status)) ? 1u : 0u;
@@ -802,16 +835,16 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
             if (lane == 0) timeout_at(P.ctl, 2, t.q, t.depth, t.count);
             break;
         }
-        const uint32_t nb = min(WAVE, cnt_local - b0);
+        const uint32_t nb = min(NB, cnt_local - b0);
         // ---- parents into LDS ----
         if (root) {
             if (lane < nb) {
                 uint32_t idx = t.begin + b0 + lane;
-                s.par[lane] = pl.root_base == ALL_VERTICES ? idx : g.roots[pl.root_base + idx];
+                par[lane] = pl.root_base == ALL_VERTICES ? idx : g.roots[pl.root_base + idx];
             }
         } else {
             const uint32_t *src = P.banks + (size_t)t.bank * P.bank_words + t.begin + (size_t)b0 * W;
-            for (uint32_t w = lane; w < nb * W; w += WAVE) s.par[w] = src[w];
+            for (uint32_t w = lane; w < nb * W; w += WAVE) par[w] = src[w];
         }
         __syncthreads();
         // ---- requirement rows per parent; driver = shortest ----
@@ -819,7 +852,7 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
         unsigned long long words = 0;
         bool alive = lane < nb;
         if (alive && root && pl.nself[0]) {
-            uint32_t v = s.par[lane];
+            uint32_t v = par[lane];
             for (uint32_t i = 0; i < pl.nself[0] && alive; i++) {
                 uint32_t st, len;
                 row_of(g, pl.req[pl.req_begin[0] + i], v, st, len);
@@ -831,9 +864,9 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
             for (uint32_t i = 0; i < k; i++) {
                 uint32_t r = pl.req[rb + i];
                 uint32_t st, len;
-                row_of(g, r, s.par[lane * W + req_anchor(r)], st, len);
-                s.rows_st[lane * MAXR + i] = st;
-                s.rows_len[lane * MAXR + i] = len;
+                row_of(g, r, par[lane * W + req_anchor(r)], st, len);
+                rows_st[lane * k + i] = st;
+                rows_len[lane * k + i] = len;
                 words += len;
                 if (len < best) {
                     best = len;
@@ -872,7 +905,9 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
             if ((step & 3) == 0) {
                 uint32_t want = 0;
                 if (lane == 0) want = ld_rlx(&P.ctl->hungry) > 0 ? 1u : 0u;
-                if (bcast(want, 0)) {
+                want = bcast(want, 0);
+                run_rows = want ? min(P.task_rows, WAVE) : P.task_rows;
+                if (want) {
                     // a candidate split may already have ended the task inside this batch
                     const uint32_t rem_par = cnt_local > b0 + nb ? cnt_local - (b0 + nb) : 0;
                     if (rem_par >= 2 * WAVE) {
@@ -912,7 +947,7 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
                     // The output bank is full: hand its rows over as tasks and
                     // yield, so the deeper tasks just pushed run before this
                     // one produces more (bounded, depth-first memory use).
-                    flush_run(P, o, t.q, D, lvl_children, stats);
+                    flush_run(P, o, t.q, D, lvl_children, run_rows, stats);
                     if (lane == 0) bank_release(P, o.bank);
                     o.bank = NO_BANK;
                     yield = true;
@@ -973,12 +1008,12 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
                 uint32_t excl = pi ? s.cum[pi - 1] : 0;
                 x = g.col[s.dstart[pi] + (tt - excl)];
                 ok = true;
-                const uint32_t *prow = &s.par[pi * W];
+                const uint32_t *prow = &par[pi * W];
                 for (uint32_t a = 0; a < W; a++) ok &= prow[a] != x;
                 const uint32_t dx = s.didx[pi];
                 for (uint32_t i = 0; i < k && ok; i++)
                     if (i != dx) {
-                        ok = row_contains(g.col, s.rows_st[pi * MAXR + i], s.rows_len[pi * MAXR + i], x);
+                        ok = row_contains(g.col, rows_st[pi * k + i], rows_len[pi * k + i], x);
                         nprobe++;
                     }
                 for (uint32_t i = 0; i < nself && ok; i++) {
@@ -999,7 +1034,7 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
             if (final_level) {
                 cnt_acc += cnt;
                 if (P.table) {
-                    const uint32_t *prow = &s.par[pi * W];
+                    const uint32_t *prow = &par[pi * W];
                     emit_rows(P, wt, t.q, n, ok, rank, cnt, stats, [&](uint32_t *dst) {
                         for (uint32_t j = 0; j < W; j++) dst[pl.order[j]] = prow[j];
                         dst[pl.order[D]] = x;
@@ -1009,12 +1044,12 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
                 const uint32_t CW = D + 1;
                 if (ok) {
                     uint32_t *dst = P.banks + (size_t)o.bank * P.bank_words + o.used + rank * CW;
-                    const uint32_t *prow = &s.par[pi * W];
+                    const uint32_t *prow = &par[pi * W];
                     for (uint32_t j = 0; j < W; j++) dst[j] = prow[j];
                     dst[W] = x;
                 }
                 o.used += cnt * CW;
-                if ((o.used - o.run_begin) / CW >= P.task_rows) flush_run(P, o, t.q, D, lvl_children, stats);
+                if ((o.used - o.run_begin) / CW >= run_rows) flush_run(P, o, t.q, D, lvl_children, run_rows, stats);
             }
         }
         if (final_level && lane == 0 && cnt_acc >= (1ull << 20)) {
@@ -1024,10 +1059,9 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
         }
         __syncthreads();
     }
-    if (!final_level && o.bank != NO_BANK) flush_run(P, o, t.q, D, lvl_children, stats);
-    // hand the output page on at the end of every final-level task, so the
-    // query's stored-row count is complete when its last task finishes
-    if (final_level && P.table && lane == 0) tab_release(P, wt);
+    if (!final_level && o.bank != NO_BANK) flush_run(P, o, t.q, D, lvl_children, run_rows, stats);
+    // the query's stored-row count is complete when its last task finishes
+    if (final_level && P.table && lane == 0) tab_task_end(P, wt);
     if (lane == 0 && cnt_acc) {
         unsigned long long now_cnt = atom_add(&Q.count, cnt_acc) + cnt_acc;
         if (Q.max_count && now_cnt > Q.max_count) query_fail(P, t.q, FM_ERR_COUNT_LIMIT);
@@ -1136,7 +1170,11 @@ __global__ __launch_bounds__(WAVE) void fm_match_kernel(KParams P) {
     uint32_t idle = 0;
     unsigned long long idle_since = 0;  // lane 0: when this wave ran out of work
     if (lane == 0) wt = WaveTab{0, NO_PAGE, 0, 0, 0};
-    unsigned long long stats[ST_N] = {0, 0, 0, 0, 0, 0, 0, 0};
+    // per-wave counters in LDS (lane 0 updates them), flushed at exit
+    __shared__ unsigned long long stats[ST_N];
+    if (lane == 0)
+        for (int i = 0; i < ST_N; i++) stats[i] = 0;
+    const unsigned long long t_wave = wall_clock64();
     unsigned long long starve_since = 0;
     unsigned long long deadline = 0, retire_at = ~0ull;  // retire_at: lane 0 only
     if (lane == 0) {
@@ -1189,7 +1227,9 @@ __global__ __launch_bounds__(WAVE) void fm_match_kernel(KParams P) {
         if (flag == 3) {
             const uint32_t slot = s_task.q;
             __syncthreads();
+            const unsigned long long ta = wall_clock64();
             admit(P, slot, stats);
+            if (lane == 0) stats[ST_BUSY] += wall_clock64() - ta;
             idle_since = 0;
             continue;
         }
@@ -1209,8 +1249,10 @@ __global__ __launch_bounds__(WAVE) void fm_match_kernel(KParams P) {
         __syncthreads();
         if (P.instrument && lane == 0)
             for (uint32_t i = 0; i < 4 + MAXN; i++) s.inst[i] = 0;
+        const unsigned long long tb = wall_clock64();
         process_task(P, t, s, o, wt, stats, starve_since, deadline);
         __syncthreads();
+        if (lane == 0) stats[ST_BUSY] += wall_clock64() - tb;
         if (P.instrument && lane == 0) {
             DQuery &Q = P.qs[t.q];
             unsigned long long *dst[4] = {&Q.row_words, &Q.lookups, &Q.candidates, &Q.probes};
@@ -1233,6 +1275,7 @@ __global__ __launch_bounds__(WAVE) void fm_match_kernel(KParams P) {
     if (lane == 0) {
         if (o.bank != NO_BANK) bank_release(P, o.bank);
         tab_release(P, wt);
+        stats[ST_ALIVE] = wall_clock64() - t_wave;
         for (int i = 0; i < ST_N; i++)
             if (stats[i]) atom_add(&P.ctl->stats[i], stats[i]);
     }
@@ -1657,7 +1700,10 @@ int fm_engine_create(const fm_engine_config *cfg, fm_engine **out) {
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fm_match_kernel, WAVE, 0) != hipSuccess || blocks <= 0)
         blocks = 8;
     e->waves_per_cu = blocks;
-    e->grid_waves = (uint32_t)(blocks * e->cu_count);
+    // Resident waves: at most 8 per CU (2 per SIMD). More waves fit, but
+    // they measured slower on R-MAT workloads: the waves' requirement rows
+    // then overflow the per-XCD L2 and idle waves poll the queues harder.
+    e->grid_waves = (uint32_t)(std::min(blocks, 8) * e->cu_count);
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess)
         return cleanup_fail(fail(FM_ERR_DEVICE, "stream/event creation failed"));
@@ -1912,6 +1958,7 @@ int fm_match_batch(fm_engine *e, const fm_graph *g, const fm_query *const *qs, u
     st.tasks_split = ctl_back.stats[ST_SPLIT];
     st.requeues = ctl_back.stats[ST_REQUEUE];
     st.pages_allocated = ctl_back.stats[ST_PAGES];
+    st.busy_frac = ctl_back.stats[ST_ALIVE] ? (double)ctl_back.stats[ST_BUSY] / (double)ctl_back.stats[ST_ALIVE] : 0.0;
     r->pages_used = std::min(ctl_back.page_bump, e->npages);
     st.status = ctl_back.error;
     if (ctl_back.error) {

@@ -291,6 +291,7 @@ typedef struct fm_launch_stats {
     uint64_t requeues;       /* tasks re-queued by a yielding wave */
     uint32_t grid_waves;
     uint32_t status;
+    double busy_frac;        /* share of the waves' resident time spent in tasks */
 } fm_launch_stats;
 
 uint32_t fm_result_queries(const fm_result *r);

@@ -345,3 +345,21 @@ def test_serving_results_equal_alone_and_in_the_mix(er, slice_ms, plan):
     for k, ref in enumerate(refs):
         assert done[k][0] == ref.count, k
         assert np.array_equal(done[k][1], ref.table), k
+
+
+def test_small_tasks_fill_pages_instead_of_wasting_them():
+    """511,472 rows of 16 bytes need 125 pages of 64 KiB. With 64-parent
+    final tasks a wave keeps its page from task to task, so the table fits a
+    256-page arena; a fresh page per task would need several hundred."""
+    g = fm.Graph.erdos_renyi(1000, 4000, 1, seed=77).upload()
+    og = _oracle_graph(g)
+    q = fm.Query.from_edges(4, [(0, 1, 0), (1, 0, 0), (1, 2, 0), (2, 1, 0), (2, 3, 0), (3, 2, 0)])
+    ref = orc.match(og, 4, q.edges())
+    assert ref.count == 511472
+    with fm.Engine(soln_prealloc=1 << 28, output_prealloc=256 << 16, output_page_bytes=1 << 16) as e:
+        r = e.match(g, q, grid_waves=64, task_rows=64, timeout_ms=TIMEOUT_MS)
+        out = r.query(0)
+        assert out.status == 0 and out.count == ref.count
+        assert out.rows == ref.count and not out.truncated
+        assert np.array_equal(r.table(0), ref.table)
+        r.close()
