@@ -1297,22 +1297,28 @@ __global__ void fm_init_slots_kernel(Slot *ring, uint32_t qcap, uint32_t levels,
 
 // ---------------------------------------------------------------------------
 // Intersection microkernel: |A n B| and the ordered intersection for a batch
-// of pairs of sorted u32 lists. One 256-thread block per pair (grid-stride).
-//  - Similar lengths: merge path. The block walks the merged order of A and
-//    B in tiles of 2048 elements; a tile's A and B slices are staged in LDS
-//    with coalesced loads, each thread finds its 8-element stretch of the
-//    merge path by binary search in LDS and merges it sequentially; a block
-//    scan of the per-thread match counts places the matches in order. Every
-//    element is read from HBM once.
-//  - Skewed lengths (longer >= 32x shorter): the shorter list is read in
-//    chunks of 256; the block narrows the longer list to the chunk's value
-//    range with two binary searches, then every thread binary-searches its
-//    element there. The longer list is only touched along search paths.
+// of pairs of sorted u32 lists (A = the shorter list of a pair).
+//
+// Each pair is cut into tiles, planned on the host from the list lengths:
+//  - merge tiles (lengths within 32x): tile k covers merge-path diagonals
+//    [k*XT, (k+1)*XT) of merge(A, B);
+//  - skew tiles (longer >= 32x shorter): tile k covers A[k*IB, (k+1)*IB) and
+//    the range of B between its first and last value.
+// fm_isect_cut_kernel finds every tile's cut, all tiles in parallel (one
+// global merge-path or range search each). fm_isect_kernel runs persistent
+// 256-thread blocks that claim whole pairs, largest first, and stream their
+// tiles: a merge tile is staged in LDS (plus the next B element, so a match
+// on the tile's last A element is seen), each thread merges a 15-element
+// stretch of the path, a block scan orders the matches, and they leave
+// through LDS in coalesced stores. Every list element is read from HBM once.
 
 constexpr uint32_t IB = 256;              // threads per block
-constexpr uint32_t IVT = 8;               // merge-path elements per thread
-constexpr uint32_t ITILE = IB * IVT;      // merge-path elements per tile
-constexpr uint32_t ISKEW = 32;
+// Merge-path elements per thread. Odd: the threads' stretches start about
+// XVT / 2 words apart in each LDS slice, and an odd stride spreads a wave's
+// accesses over all LDS banks (an even one folds them onto a few).
+constexpr uint32_t XVT = 15;
+constexpr uint32_t XT = IB * XVT;         // merge-path elements per merge tile
+constexpr uint32_t ISKEW = 32;            // longer >= ISKEW x shorter: skew tiles
 
 // Number of A elements among the first `diag` elements of merge(A, B), with
 // A taken first on ties.
@@ -1322,6 +1328,24 @@ __device__ __forceinline__ uint32_t merge_split(const uint32_t *A, uint32_t na, 
     while (lo < hi) {
         uint32_t mid = (lo + hi) >> 1;
         if (A[mid] <= B[diag - 1 - mid]) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t *B, uint32_t lo, uint32_t hi, uint32_t v) {
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (B[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ uint32_t upper_bound_u32(const uint32_t *B, uint32_t lo, uint32_t hi, uint32_t v) {
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (B[mid] <= v) lo = mid + 1;
         else hi = mid;
     }
     return lo;
@@ -1343,140 +1367,213 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *s_wave
     return pre + incl - v;
 }
 
-__global__ __launch_bounds__(IB) void fm_intersect_kernel(const uint32_t *__restrict__ data, const uint64_t *a_off,
+struct IsectPair {
+    const uint32_t *A, *B;
+    uint32_t na, nb;
+};
+
+__device__ __forceinline__ IsectPair isect_pair(const uint32_t *data, const uint64_t *a_off, const uint32_t *a_len,
+                                                const uint64_t *b_off, const uint32_t *b_len, uint32_t p) {
+    IsectPair r{data + a_off[p], data + b_off[p], a_len[p], b_len[p]};
+    if (r.na > r.nb) {
+        const uint32_t *tp = r.A;
+        r.A = r.B;
+        r.B = tp;
+        const uint32_t tn = r.na;
+        r.na = r.nb;
+        r.nb = tn;
+    }
+    return r;
+}
+
+__device__ __forceinline__ bool isect_skewed(uint32_t na, uint32_t nb) {
+    return (unsigned long long)nb >= (unsigned long long)ISKEW * na;
+}
+
+// One thread per tile: the tile's cuts. Merge tiles: cut = A elements before
+// the tile's first diagonal. Skew tiles: [cut, cut2) = the range of B that
+// holds the values of the tile's A chunk.
+__global__ __launch_bounds__(IB) void fm_isect_cut_kernel(const uint32_t *__restrict__ data, const uint64_t *a_off,
                                                           const uint32_t *a_len, const uint64_t *b_off,
-                                                          const uint32_t *b_len, uint32_t npairs, uint32_t *counts,
-                                                          const uint64_t *out_off, uint32_t *out) {
-    __shared__ uint32_t sAB[ITILE + 1];   // a tile's A slice followed by its B slice (+1)
-    __shared__ uint32_t s_cut[IB + 1];    // merge-path cuts of up to IB tiles
-    __shared__ uint32_t s_wave[IB / WAVE];
-    __shared__ uint32_t s_split[4];
-    const uint32_t tid = threadIdx.x;
-    for (uint32_t p = blockIdx.x; p < npairs; p += gridDim.x) {
-        const uint32_t *A = data + a_off[p];
-        const uint32_t *B = data + b_off[p];
-        uint32_t na = a_len[p], nb = b_len[p];
-        if (na > nb) {
-            const uint32_t *tp = A;
-            A = B;
-            B = tp;
-            uint32_t tn = na;
-            na = nb;
-            nb = tn;
-        }
-        uint32_t *o = out_off ? out + out_off[p] : nullptr;
-        uint32_t found = 0;
-        if (na == 0) {
-            // nothing to intersect
-        } else if ((unsigned long long)nb >= (unsigned long long)ISKEW * na) {
-            for (uint32_t c0 = 0; c0 < na; c0 += IB) {
-                const uint32_t cn = min(IB, na - c0);
-                if (tid < 2) {
-                    // B range covering [A[c0], A[c0 + cn - 1]]: thread 0 the
-                    // lower bound of the first, thread 1 the upper bound of the last
-                    const uint32_t v = A[c0 + (tid == 0 ? 0 : cn - 1)];
-                    uint32_t lo = 0, hi = nb;
-                    while (lo < hi) {
-                        uint32_t mid = (lo + hi) >> 1;
-                        if (tid == 0 ? B[mid] < v : B[mid] <= v) lo = mid + 1;
-                        else hi = mid;
-                    }
-                    s_split[tid] = lo;
-                }
-                __syncthreads();
-                const uint32_t blo = s_split[0], bhi = s_split[1];
-                bool hit = false;
-                uint32_t x = 0;
-                if (tid < cn) {
-                    x = A[c0 + tid];
-                    uint32_t lo = blo, hi = bhi;
-                    while (lo < hi) {
-                        uint32_t mid = (lo + hi) >> 1;
-                        if (B[mid] < x) lo = mid + 1;
-                        else hi = mid;
-                    }
-                    hit = lo < bhi && B[lo] == x;
-                }
-                uint32_t tot;
-                const uint32_t at = block_excl_scan(hit ? 1u : 0u, s_wave, tot);
-                if (o && hit) o[found + at] = x;
-                found += tot;
-            }
+                                                          const uint32_t *b_len, const uint32_t *tile_begin,
+                                                          const uint32_t *tile_pair, uint32_t ntiles, uint32_t *cut,
+                                                          uint32_t *cut2) {
+    for (uint32_t t = blockIdx.x * IB + threadIdx.x; t < ntiles; t += gridDim.x * IB) {
+        const uint32_t p = tile_pair[t];
+        const IsectPair q = isect_pair(data, a_off, a_len, b_off, b_len, p);
+        const uint32_t k = t - tile_begin[p];
+        if (isect_skewed(q.na, q.nb)) {
+            const uint32_t first = k * IB, last = min(first + IB, q.na) - 1;
+            const uint32_t lo = lower_bound_u32(q.B, 0, q.nb, q.A[first]);
+            cut[t] = lo;
+            cut2[t] = upper_bound_u32(q.B, lo, q.nb, q.A[last]);
         } else {
-            // Tile boundaries are found for up to IB tiles at once (one
-            // global merge-path search per thread), then the tiles stream
-            // through LDS with the next tile's elements loaded into registers
-            // while the current one is merged.
-            const uint32_t total_path = na + nb;
-            const uint32_t ntiles = (total_path + ITILE - 1) / ITILE;
-            for (uint32_t r0 = 0; r0 < ntiles; r0 += IB) {
-                const uint32_t rn = min(IB, ntiles - r0);
-                // s_cut[i] = A elements before tile r0 + i (i = 0..rn)
-                for (uint32_t i = tid; i <= rn; i += IB) {
-                    const uint32_t d = min((r0 + i) * ITILE, total_path);
-                    s_cut[i] = merge_split(A, na, B, nb, d);
+            cut[t] = merge_split(q.A, q.na, q.B, q.nb, k * XT);
+            cut2[t] = 0;
+        }
+    }
+}
+
+// Persistent blocks claim whole pairs (largest first, `order`) and stream
+// their tiles; the next tile's elements are loaded into registers while the
+// current one is merged, and the matches leave through LDS in coalesced
+// stores at the pair's running offset.
+__global__ __launch_bounds__(IB) void fm_isect_kernel(const uint32_t *__restrict__ data, const uint64_t *a_off,
+                                                      const uint32_t *a_len, const uint64_t *b_off,
+                                                      const uint32_t *b_len, uint32_t *counts,
+                                                      const uint64_t *out_off, uint32_t *out,
+                                                      const uint32_t *tile_begin, const uint32_t *order,
+                                                      uint32_t nwork, const uint32_t *cut, const uint32_t *cut2,
+                                                      uint32_t *ctr) {
+    __shared__ uint32_t sAB[XT + 1];  // a tile's A slice, then its B slice (+ the next B element)
+    // A match takes two merge steps (its A element, then the equal B
+    // element), so a stretch holds at most XM matches. They are parked at
+    // tid * (XM + 1) (an odd stride: conflict-free banks) and then moved to
+    // their block order in the same buffer.
+    constexpr uint32_t XM = (XVT + 1) / 2;
+    __shared__ uint32_t s_out[IB * (XM + 1)];
+    __shared__ uint32_t s_wave[IB / WAVE];
+    __shared__ uint32_t s_w;
+    // cuts of the pair's tiles, XCUT at a time (no dependent global load
+    // before a tile's element loads)
+    constexpr uint32_t XCUT = 2 * IB;
+    __shared__ uint32_t s_cut[XCUT + 1], s_cut2[XCUT];
+    const uint32_t tid = threadIdx.x;
+    while (true) {
+        if (tid == 0) s_w = atomicAdd(ctr, 1u);
+        __syncthreads();
+        const uint32_t w = s_w;
+        if (w >= nwork) break;
+        const uint32_t p = order[w];
+        const IsectPair q = isect_pair(data, a_off, a_len, b_off, b_len, p);
+        const uint32_t tb = tile_begin[p], te = tile_begin[p + 1];
+        const bool skew = isect_skewed(q.na, q.nb);
+        const uint32_t total_path = q.na + q.nb;
+        uint32_t *const o = out ? out + out_off[p] : nullptr;
+        uint32_t cbase = tb;  // tile of s_cut[0]
+        auto fill_cuts = [&](uint32_t from) {
+            cbase = from;
+            for (uint32_t i = tid; i <= XCUT; i += IB) {
+                const uint32_t t = from + i;
+                if (i < XCUT) s_cut2[i] = t < te ? cut2[t] : 0u;
+                s_cut[i] = t < te ? cut[t] : q.na;  // past the pair's last tile: all of A
+            }
+        };
+        fill_cuts(tb);
+        __syncthreads();
+        uint32_t reg[XVT + 1];
+        uint32_t rc0 = 0, rc1 = 0;  // cuts of the tile in reg
+        // t - cbase < XCUT: the caller refills s_cut before a tile past it
+        auto load_tile = [&](uint32_t t) {
+            rc0 = s_cut[t - cbase];
+            if (skew) {
+                rc1 = s_cut2[t - cbase];
+                const uint32_t first = (t - tb) * IB;
+                reg[0] = tid < min(IB, q.na - first) ? q.A[first + tid] : 0u;
+                return;
+            }
+            rc1 = s_cut[t - cbase + 1];
+            const uint32_t d0 = (t - tb) * XT, d1 = min(d0 + XT, total_path);
+            const uint32_t b0 = d0 - rc0, b1 = d1 - rc1;
+            const uint32_t ta = rc1 - rc0, tbx = (b1 - b0) + (b1 < q.nb ? 1u : 0u);
+#pragma unroll
+            for (uint32_t x = 0; x <= XVT; x++) {
+                const uint32_t e = tid + x * IB;
+                reg[x] = e < ta ? q.A[rc0 + e] : (e < ta + tbx ? q.B[b0 + e - ta] : 0u);
+            }
+        };
+        load_tile(tb);
+        uint32_t found = 0;
+        for (uint32_t t = tb; t < te; t++) {
+            const uint32_t c0 = rc0, c1 = rc1;
+            uint32_t c = 0, x_skew = 0;
+            uint32_t ta = 0, tbn = 0, tbx = 0, i0 = 0, j0 = 0, steps = 0;
+            if (skew) {
+                const uint32_t first = (t - tb) * IB, cn = min(IB, q.na - first);
+                x_skew = reg[0];
+                if (t + 1 < te && t + 1 - cbase < XCUT) load_tile(t + 1);
+                if (tid < cn) {
+                    const uint32_t at = lower_bound_u32(q.B, c0, c1, x_skew);
+                    c = at < c1 && q.B[at] == x_skew ? 1u : 0u;
+                }
+            } else {
+                const uint32_t d0 = (t - tb) * XT, d1 = min(d0 + XT, total_path);
+                const uint32_t b0 = d0 - c0, b1 = d1 - c1;
+                ta = c1 - c0;
+                tbn = b1 - b0;
+                tbx = tbn + (b1 < q.nb ? 1u : 0u);
+#pragma unroll
+                for (uint32_t x = 0; x <= XVT; x++) {
+                    const uint32_t e = tid + x * IB;
+                    if (e < ta + tbx) sAB[e] = reg[x];
                 }
                 __syncthreads();
-                // registers holding the next tile (A slice, then B slice + 1)
-                uint32_t reg[IVT + 1];
-                auto load_tile = [&](uint32_t ti) {
-                    const uint32_t d0 = (r0 + ti) * ITILE, d1 = min(d0 + ITILE, total_path);
-                    const uint32_t a0 = s_cut[ti], a1 = s_cut[ti + 1];
-                    const uint32_t b0 = d0 - a0, b1 = d1 - a1;
-                    const uint32_t ta = a1 - a0, tbx = (b1 - b0) + (b1 < nb ? 1u : 0u);
+                if (t + 1 < te && t + 1 - cbase < XCUT) load_tile(t + 1);  // in flight during the merge
+                const uint32_t dl = min(tid * XVT, ta + tbn);
+                steps = min(XVT, ta + tbn - dl);
+                i0 = merge_split(sAB, ta, sAB + ta, tbn, dl);
+                j0 = dl - i0;
+            }
+            // Merges the stretch; emit(x) for each match. Consuming A[i], an
+            // equal B element is the next one on B's side. The current A and
+            // B elements stay in registers: one LDS read per step.
+            auto merge_stretch = [&](auto emit) {
+                const uint32_t *sA = sAB, *sB = sAB + ta;
+                uint32_t ii = i0, jj = j0;
+                uint32_t av = ii < ta ? sA[ii] : 0u, bv = jj < tbx ? sB[jj] : 0u;
 #pragma unroll
-                    for (uint32_t k = 0; k <= IVT; k++) {
-                        const uint32_t e = tid + k * IB;
-                        reg[k] = e < ta ? A[a0 + e] : (e < ta + tbx ? B[b0 + e - ta] : 0u);
-                    }
-                };
-                load_tile(0);
-                for (uint32_t ti = 0; ti < rn; ti++) {
-                    const uint32_t d0 = (r0 + ti) * ITILE, d1 = min(d0 + ITILE, total_path);
-                    const uint32_t a0 = s_cut[ti], a1 = s_cut[ti + 1];
-                    const uint32_t b0 = d0 - a0, b1 = d1 - a1;
-                    const uint32_t ta = a1 - a0, tb = b1 - b0;
-                    const uint32_t tbx = tb + (b1 < nb ? 1u : 0u);
-#pragma unroll
-                    for (uint32_t k = 0; k <= IVT; k++) {
-                        const uint32_t e = tid + k * IB;
-                        if (e < ta + tbx) sAB[e] = reg[k];
-                    }
-                    __syncthreads();
-                    if (ti + 1 < rn) load_tile(ti + 1);  // in flight during the merge
-                    const uint32_t *sA = sAB, *sB = sAB + ta;
-                    const uint32_t dl = min(tid * IVT, ta + tb);
-                    const uint32_t steps = min(IVT, ta + tb - dl);
-                    uint32_t i = merge_split(sA, ta, sB, tb, dl);
-                    uint32_t j = dl - i;
-                    uint32_t m[IVT];
-                    uint32_t c = 0;
-#pragma unroll
-                    for (uint32_t st = 0; st < IVT; st++) {
-                        if (st < steps) {
-                            if (i < ta && (j >= tb || sA[i] <= sB[j])) {
-                                // consuming A[i]; an equal B element is the next one on B's side
-                                if (j < tbx && sB[j] == sA[i]) m[c++] = sA[i];
-                                i++;
-                            } else {
-                                j++;
-                            }
+                for (uint32_t st = 0; st < XVT; st++) {
+                    if (st < steps) {
+                        if (ii < ta && (jj >= tbn || av <= bv)) {
+                            if (jj < tbx && bv == av) emit(av);
+                            ii++;
+                            if (ii < ta) av = sA[ii];
+                        } else {
+                            jj++;
+                            if (jj < tbx) bv = sB[jj];
                         }
                     }
-                    uint32_t tot;
-                    const uint32_t at = block_excl_scan(c, s_wave, tot);
-                    if (o) {
+                }
+            };
+            if (!skew) {
+                // strictly increasing lists give at most XM; the clamp keeps
+                // any input inside the buffers
+                if (o) merge_stretch([&](uint32_t x) {
+                    if (c < XM) s_out[tid * (XM + 1) + c] = x;
+                    c++;
+                });
+                else merge_stretch([&](uint32_t) { c++; });
+                c = min(c, XM);
+            }
+            uint32_t tot;
+            const uint32_t at = block_excl_scan(c, s_wave, tot);  // its barriers order the parking
+            if (o) {
+                // parked matches to block order, then coalesced stores
+                uint32_t m[XM];
+                if (skew) {
+                    m[0] = x_skew;
+                } else {
 #pragma unroll
-                        for (uint32_t k = 0; k < IVT; k++)
-                            if (k < c) o[found + at + k] = m[k];
-                    }
-                    found += tot;
+                    for (uint32_t x = 0; x < XM; x++) m[x] = x < c ? s_out[tid * (XM + 1) + x] : 0u;
                 }
                 __syncthreads();
+#pragma unroll
+                for (uint32_t x = 0; x < XM; x++)
+                    if (x < c) s_out[at + x] = m[x];
+                __syncthreads();
+                for (uint32_t x = tid; x < tot; x += IB) o[found + x] = s_out[x];
+            }
+            found += tot;
+            __syncthreads();  // sAB and s_out are rewritten by the next tile
+            if (t + 1 < te && t + 1 - cbase == XCUT) {
+                // the next tile is past the cut window: refill, then load it
+                fill_cuts(t + 1);
+                __syncthreads();
+                load_tile(t + 1);
             }
         }
         if (tid == 0) counts[p] = found;
-        __syncthreads();
+        __syncthreads();  // s_cut is refilled for the next pair
     }
 }
 
@@ -1526,6 +1623,8 @@ struct fm_engine {
     uint64_t epoch = 0;
     bool dirty = true;  // queues/pool need a full re-initialisation
     fm_server *server = nullptr;  // a server owns the engine while it lives
+    void *isect_buf = nullptr;    // intersection tile plan and look-back state (grow only)
+    size_t isect_cap = 0;
     FSlot *pfree_ring = nullptr;
     uint32_t pfcap = 0;
 };
@@ -1742,6 +1841,7 @@ void fm_engine_destroy(fm_engine *e) {
     (void)hipFree(e->banks);
     (void)hipFree(e->bank_ref);
     (void)hipFree(e->free_ring);
+    (void)hipFree(e->isect_buf);
     (void)hipFree(e->ring);
     (void)hipFree(e->pages);
     (void)hipFree(e->ctl);
@@ -2457,17 +2557,71 @@ int fm_intersect_pairs(fm_engine *e, const uint32_t *data, const uint64_t *a_off
     if (!e || !data || !a_off || !a_len || !b_off || !b_len || !counts)
         return fail(FM_ERR_INVALID_ARG, "null argument");
     if (out_off && !out) return fail(FM_ERR_INVALID_ARG, "out_off without out");
+    if (kernel_ms) *kernel_ms = 0;
     if (npairs == 0) return FM_OK;
     HIP_TRY(hipSetDevice(e->device));
+    // ---- tile plan (host): tiles of each pair, pair of each tile ----
+    std::vector<uint32_t> la(npairs), lb(npairs);
+    HIP_TRY(hipMemcpy(la.data(), a_len, npairs * 4ull, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(lb.data(), b_len, npairs * 4ull, hipMemcpyDeviceToHost));
+    std::vector<uint32_t> tile_begin(npairs + 1);
+    uint64_t ntiles = 0;
+    for (uint32_t p = 0; p < npairs; p++) {
+        tile_begin[p] = (uint32_t)ntiles;
+        const uint64_t na = std::min(la[p], lb[p]), nb = std::max(la[p], lb[p]);
+        if (na == 0) continue;
+        ntiles += nb >= (uint64_t)ISKEW * na ? (na + IB - 1) / IB : (na + nb + XT - 1) / XT;
+        if (ntiles >= (1ull << 31)) return fail(FM_ERR_INVALID_ARG, "too many intersection tiles");
+    }
+    tile_begin[npairs] = (uint32_t)ntiles;
+    std::vector<uint32_t> tile_pair(ntiles);
+    for (uint32_t p = 0; p < npairs; p++)
+        for (uint32_t t = tile_begin[p]; t < tile_begin[p + 1]; t++) tile_pair[t] = p;
+    // pairs with tiles, most tiles first (claimed in this order)
+    std::vector<uint32_t> order;
+    order.reserve(npairs);
+    for (uint32_t p = 0; p < npairs; p++)
+        if (tile_begin[p + 1] > tile_begin[p]) order.push_back(p);
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+        return tile_begin[x + 1] - tile_begin[x] > tile_begin[y + 1] - tile_begin[y];
+    });
+    const uint32_t nwork = (uint32_t)order.size();
+    // ---- scratch: tile_begin | order | tile_pair | cut | cut2 | ctr ----
+    const size_t need = ((size_t)npairs + 1 + npairs + 3 * ntiles + 1) * 4;
+    if (need > e->isect_cap) {
+        (void)hipFree(e->isect_buf);
+        e->isect_buf = nullptr;
+        e->isect_cap = 0;
+        HIP_TRY(hipMalloc((void **)&e->isect_buf, need));
+        e->isect_cap = need;
+    }
+    uint32_t *d_tile_begin = (uint32_t *)e->isect_buf;
+    uint32_t *d_order = d_tile_begin + npairs + 1;
+    uint32_t *d_tile_pair = d_order + npairs;
+    uint32_t *d_cut = d_tile_pair + ntiles;
+    uint32_t *d_cut2 = d_cut + ntiles;
+    uint32_t *d_ctr = d_cut2 + ntiles;
+    HIP_TRY(hipMemcpyAsync(d_tile_begin, tile_begin.data(), (npairs + 1) * 4ull, hipMemcpyHostToDevice, e->stream));
+    if (nwork) HIP_TRY(hipMemcpyAsync(d_order, order.data(), nwork * 4ull, hipMemcpyHostToDevice, e->stream));
+    if (ntiles)
+        HIP_TRY(hipMemcpyAsync(d_tile_pair, tile_pair.data(), ntiles * 4ull, hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipMemsetAsync(d_ctr, 0, 4, e->stream));
+    // pairs without tiles (an empty list) keep count 0
+    HIP_TRY(hipMemsetAsync(counts, 0, npairs * 4ull, e->stream));
     int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fm_intersect_kernel, IB, 0) != hipSuccess ||
-        blocks <= 0)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fm_isect_kernel, IB, 0) != hipSuccess || blocks <= 0)
         blocks = 4;
-    uint32_t grid = std::min<uint32_t>(npairs, (uint32_t)(blocks * e->cu_count));
+    const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(nwork, (uint64_t)blocks * e->cu_count));
+    const uint32_t cut_grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((ntiles + IB - 1) / IB, 65535));
     HIP_TRY(hipEventRecord(e->ev0, e->stream));
-    hipLaunchKernelGGL(fm_intersect_kernel, dim3(grid), dim3(IB), 0, e->stream, data, a_off, a_len, b_off, b_len,
-                       npairs, counts, out_off, out);
-    HIP_TRY(hipGetLastError());
+    if (ntiles) {
+        hipLaunchKernelGGL(fm_isect_cut_kernel, dim3(cut_grid), dim3(IB), 0, e->stream, data, a_off, a_len, b_off,
+                           b_len, d_tile_begin, d_tile_pair, (uint32_t)ntiles, d_cut, d_cut2);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(fm_isect_kernel, dim3(grid), dim3(IB), 0, e->stream, data, a_off, a_len, b_off, b_len,
+                           counts, out_off, out, d_tile_begin, d_order, nwork, d_cut, d_cut2, d_ctr);
+        HIP_TRY(hipGetLastError());
+    }
     HIP_TRY(hipEventRecord(e->ev1, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
     float ms = 0;

@@ -367,10 +367,12 @@ void fm_server_destroy(fm_server *s);
 
 /* ---- intersection microkernel -------------------------------------------
  * Replaces the warp-cooperative intersect<CoopSize,Decomp> family
- * (src/intersect.cuh:31-191). pairs of sorted u32 lists inside one device
- * array `data` (device pointers): list A_i = data[a_off[i] .. +a_len[i]],
- * B_i likewise. counts[i] = |A_i n B_i|. If out_off is not NULL, the
- * intersection of pair i is written ascending at out[out_off[i] ..]. */
+ * (src/intersect.cuh:31-191). pairs of strictly increasing u32 lists inside
+ * one device array `data` (device pointers): list A_i = data[a_off[i] ..
+ * +a_len[i]], B_i likewise. counts[i] = |A_i n B_i|. If out_off is not NULL,
+ * the intersection of pair i is written ascending at out[out_off[i] ..]
+ * (room for min(a_len[i], b_len[i]) words). Lists with repeated values give
+ * unspecified counts, never accesses outside the lists and that room. */
 int fm_intersect_pairs(fm_engine *e, const uint32_t *data,
                        const uint64_t *a_off, const uint32_t *a_len,
                        const uint64_t *b_off, const uint32_t *b_len,

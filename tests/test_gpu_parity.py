@@ -263,10 +263,14 @@ def test_intersection_microkernel(engine):
     lists, a_off, a_len, b_off, b_len = [], [], [], [], []
     pos = 0
     expected = []
-    sizes = [(0, 5), (1, 1), (3, 3000), (2500, 2600), (5000, 70), (64, 64), (4096, 10000), (1, 0)]
+    # one-tile, many-tile (up to ~550 tiles: more than one window of staged
+    # cuts), skewed and empty pairs
+    sizes = [(0, 5), (1, 1), (3, 3000), (2500, 2600), (5000, 70), (64, 64), (4096, 10000), (1, 0),
+             (100000, 100000), (600, 40000), (30000, 25000), (1000000, 1100000)]
     for na, nb in sizes:
-        a = np.unique(rng.integers(0, 20000, na, dtype=np.uint32))
-        b = np.unique(rng.integers(0, 20000, nb, dtype=np.uint32))
+        span = max(20000, 3 * max(na, nb))
+        a = np.unique(rng.integers(0, span, na, dtype=np.uint32))
+        b = np.unique(rng.integers(0, span, nb, dtype=np.uint32))
         a_off.append(pos); a_len.append(len(a)); lists.append(a); pos += len(a)
         b_off.append(pos); b_len.append(len(b)); lists.append(b); pos += len(b)
         expected.append(np.intersect1d(a, b))

@@ -1,13 +1,18 @@
-"""CPU model of the merge-path tiling of fm_intersect_kernel (engine.hip):
-tiles of 2048 merge-path elements, per-thread 8-element stretches found by
-binary search, the match check against the next B element (staged one past
-the tile), and the block-ordered output. Checked against numpy on random and
-edge-case pairs, including matches straddling tile and thread boundaries."""
+"""CPU model of the tiled intersection kernels (engine.hip:
+fm_isect_cut_kernel, fm_isect_kernel): the host tile plan (merge tiles of
+3840 merge-path elements, skew tiles of 256 elements of the shorter list when
+the longer one is >= 32x longer), each tile's cut, the per-thread 15-element
+stretches found by binary search in the tile's LDS slices, the match check
+against the next B element (staged one past the tile), and the output offset
+of each tile as the sum of the counts of the pair's earlier tiles (the running
+offset of the block that streams the pair). Checked against numpy on random
+and edge-case pairs, including matches straddling tile and thread
+boundaries."""
 import numpy as np
 import pytest
 
-IB, IVT = 256, 8
-ITILE = IB * IVT
+IB, XVT, ISKEW = 256, 15, 32
+XT = IB * XVT
 
 
 def merge_split(A, na, B, nb, diag):
@@ -21,48 +26,96 @@ def merge_split(A, na, B, nb, diag):
     return lo
 
 
-def intersect_model(A, B):
-    if len(A) > len(B):
-        A, B = B, A
+def plan(pairs):
+    """tile_begin per pair and the pair of each tile (host plan)."""
+    begin, tile_pair = [], []
+    for p, (A, B) in enumerate(pairs):
+        begin.append(len(tile_pair))
+        na, nb = sorted((len(A), len(B)))
+        if na == 0:
+            continue
+        n = -(-na // IB) if nb >= ISKEW * na else -(-(na + nb) // XT)
+        tile_pair += [p] * n
+    begin.append(len(tile_pair))
+    return begin, tile_pair
+
+
+def tile_matches(A, B, k, cuts, te_minus_tb):
+    """Matches of tile k of a pair, in thread order (the block scan order)."""
     na, nb = len(A), len(B)
+    if nb >= ISKEW * na:
+        first, last = k * IB, min(k * IB + IB, na) - 1
+        lo = int(np.searchsorted(B, A[first], "left"))
+        hi = int(np.searchsorted(B, A[last], "right"))
+        out = []
+        for x in A[first:last + 1]:
+            i = lo + int(np.searchsorted(B[lo:hi], x, "left"))
+            if i < hi and B[i] == x:
+                out.append(int(x))
+        return out
+    d0, d1 = k * XT, min(k * XT + XT, na + nb)
+    a0 = cuts[k]
+    a1 = cuts[k + 1] if k + 1 < te_minus_tb else na
+    b0, b1 = d0 - a0, d1 - a1
+    assert 0 <= a1 - a0 <= XT and 0 <= b1 - b0 <= XT
+    sA = A[a0:a1]
+    sB = B[b0:b1 + (1 if b1 < nb else 0)]
+    ta, tb, tbx = a1 - a0, b1 - b0, len(sB)
     out = []
-    a0 = b0 = 0
-    for d0 in range(0, na + nb, ITILE):
-        d1 = min(d0 + ITILE, na + nb)
-        a1 = merge_split(A, na, B, nb, d1)
-        b1 = d1 - a1
-        assert 0 <= a1 - a0 <= ITILE and 0 <= b1 - b0 <= ITILE
-        sA = A[a0:a1]
-        sB = B[b0:b1 + (1 if b1 < nb else 0)]
-        ta, tb = a1 - a0, b1 - b0
-        tbx = len(sB)
-        per = []
-        for tid in range(IB):
-            dl = min(tid * IVT, ta + tb)
-            steps = min(IVT, ta + tb - dl)
-            i = merge_split(sA, ta, sB, tb, dl)
-            j = dl - i
-            m = []
-            for _ in range(steps):
-                if i < ta and (j >= tb or sA[i] <= sB[j]):
-                    if j < tbx and sB[j] == sA[i]:
-                        m.append(int(sA[i]))
-                    i += 1
-                else:
-                    j += 1
-            per.append(m)
-        for m in per:
+    for tid in range(IB):
+        dl = min(tid * XVT, ta + tb)
+        steps = min(XVT, ta + tb - dl)
+        i = merge_split(sA, ta, sB, tb, dl)
+        j = dl - i
+        for _ in range(steps):
+            if i < ta and (j >= tb or sA[i] <= sB[j]):
+                if j < tbx and sB[j] == sA[i]:
+                    out.append(int(sA[i]))
+                i += 1
+            else:
+                j += 1
+    return out
+
+
+def intersect_model(pairs):
+    begin, tile_pair = plan(pairs)
+    results = []
+    for p, (A, B) in enumerate(pairs):
+        if len(A) > len(B):
+            A, B = B, A
+        ntile = begin[p + 1] - begin[p]
+        cuts = [merge_split(A, len(A), B, len(B), k * XT) for k in range(ntile)]
+        out, pre = [], 0
+        for k in range(ntile):
+            m = tile_matches(A, B, k, cuts, ntile)
+            assert len(out) == pre  # the running offset is the earlier tiles' total
             out += m
-        a0, b0 = a1, b1
-    return np.array(out, dtype=np.uint32)
+            pre += len(m)
+        results.append(np.array(out, dtype=np.uint32))
+    assert all(tile_pair[t] == p for p in range(len(pairs)) for t in range(begin[p], begin[p + 1]))
+    return results
 
 
-@pytest.mark.parametrize("seed", range(12))
-def test_merge_path_model(seed):
+@pytest.mark.parametrize("seed", range(8))
+def test_tiled_intersection_model(seed):
     rng = np.random.default_rng(seed)
-    for na, nb, span in [(0, 10, 50), (1, 1, 2), (5000, 5200, 20000), (3000, 3000, 3000), (2048, 2048, 4096),
-                         (1, 9000, 9000), (4095, 4097, 9000)]:
+    shapes = [(0, 10, 50), (1, 1, 2), (5000, 5200, 20000), (3000, 3000, 3000), (4096, 4096, 8192),
+              (1, 9000, 9000), (8191, 8193, 18000), (300, 12000, 30000), (257, 9000, 9000), (20, 0, 5)]
+    pairs = []
+    for na, nb, span in shapes:
         A = np.unique(rng.integers(0, span, na)).astype(np.uint32)
         B = np.unique(rng.integers(0, span, nb)).astype(np.uint32)
-        got = intersect_model(A, B)
-        assert np.array_equal(got, np.intersect1d(A, B))
+        pairs.append((A, B) if seed % 2 else (B, A))
+    got = intersect_model(pairs)
+    for (A, B), g in zip(pairs, got):
+        assert np.array_equal(g, np.intersect1d(A, B))
+
+
+def test_tile_plan_counts():
+    A = np.arange(10, dtype=np.uint32)
+    pairs = [(A, np.arange(400, dtype=np.uint32)), (np.arange(5000, dtype=np.uint32), np.arange(5000, dtype=np.uint32)),
+             (np.zeros(0, np.uint32), A)]
+    begin, tile_pair = plan(pairs)
+    # 400 >= 32 * 10: one skew tile; 10000 merge-path elements: 3 tiles; empty: none
+    assert begin == [0, 1, 4, 4]
+    assert tile_pair == [0, 1, 1, 1]
