@@ -469,6 +469,14 @@ def main():
                          "kernel": "fm_match_kernel", "kernel_ms_per_step": dom_ms},
         }
         workload = "rmat20_plain_32q"
+        # HBM bytes per launch, measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
+        # on this workload and calibrated against a copy of known size
+        tpath = ROOT / "profiles" / "fm_match_kernel_rmat20_plain_traffic.json"
+        if tpath.exists():
+            tr = json.loads(tpath.read_text())
+            if tr.get("workload") == workload and tr.get("embeddings_per_step") == embeddings:
+                result["roofline"]["traffic"] = tr["hbm_bytes_per_launch"]
+                result["roofline"]["traffic_source"] = str(tpath.relative_to(ROOT))
     elif args.config == 4:
         rng = np.random.default_rng(args.seed)
         # tables of 8 queries matched alone, for the mix check (untimed)
