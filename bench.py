@@ -14,6 +14,11 @@ resident before the timed region).
 (batched), with the plain one-launch-per-query rate measured beside it;
 value = queries/s of the batched form.
 
+--config 5: stress. R-MAT 2^24 vertices, 2^28 undirected edges, edge labels
+uniform over 0..3; 64 dense 10-vertex queries (15 edges each) with at most
+2^27 embeddings each and 2.5e9 in all (~100 GB of tables), matched in one
+batched launch with every table resident; value = embeddings/s.
+
 --config 4: serving. R-MAT 2^22 vertices, 2^26 undirected edges, edge
 labels Zipf(1.2) over 0..127, 4 vertex labels; 8 MiB banks (CurBankSize =
 1 << 20). 512 queries of 4..16 vertices (at most 2^18 embeddings each)
@@ -53,6 +58,8 @@ def build_graph(cfg: int, seed: int):
         return fm.Graph.rmat(20, 1 << 24, 87, zipf_s=1.2, seed=seed, vlabels=2)
     if cfg == 4:
         return fm.Graph.rmat(22, 1 << 26, 128, zipf_s=1.2, seed=seed, vlabels=4)
+    if cfg == 5:
+        return fm.Graph.rmat(24, 1 << 28, 4, zipf_s=0.0, seed=seed, vlabels=8)
     raise SystemExit(f"unsupported config {cfg}")
 
 
@@ -63,6 +70,15 @@ def make_candidates(g, cfg: int, seed: int, count: int):
     i = 0
     while len(out) < count:
         i += 1
+        if cfg == 5:
+            # dense: 10 vertices, 15 edges (a spanning tree plus 6 chords, the
+            # cyclomatic number of a 10-cycle with 5 chords)
+            try:
+                q = fm.Query.random_walk(g, 10, shape=3, min_cycles=6, seed=seed * 1_000_003 + i)
+            except fm.ForkMatchError:
+                continue
+            out.append((3, 10, q))
+            continue
         if cfg == 2:
             # tree-like: a walk tree plus exactly one induced chord (every pure
             # 6-8 vertex tree on this graph has > 4e8 embeddings); the other
@@ -85,22 +101,26 @@ def make_candidates(g, cfg: int, seed: int, count: int):
     return out
 
 
-def screen(engine, g, cands, budget: int, want, per_shape=None):
+def screen(engine, g, cands, budget: int, want, per_shape=None, max_partials=None, total_cap=None):
     """Keeps candidates whose count is in [1, budget], screened in count mode
-    in batched launches (untimed). want = total wanted; per_shape caps."""
+    in batched launches (untimed). want = total wanted; per_shape caps;
+    total_cap bounds the sum of the kept counts."""
     keep, counts = [], []
     taken = {}
     hist = {}
     for b0 in range(0, len(cands), 64):
         chunk = cands[b0:b0 + 64]
         r = engine.match_batch(g, [c[2] for c in chunk], table=False, max_count=budget,
-                               max_partials=budget, timeout_ms=120000, allow_errors=True)
+                               max_partials=budget if max_partials is None else max_partials, timeout_ms=120000,
+                               allow_errors=True)
         for j, c in enumerate(chunk):
             o = r.query(j)
             hist[o.status] = hist.get(o.status, 0) + 1
             if o.status != 0 or o.count == 0 or o.count > budget:
                 continue
             if per_shape is not None and taken.get(c[0], 0) >= per_shape.get(c[0], 0):
+                continue
+            if total_cap is not None and sum(counts) + o.count > total_cap:
                 continue
             taken[c[0]] = taken.get(c[0], 0) + 1
             keep.append(c)
@@ -345,7 +365,7 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=6.0, help="config 4: arrival window per rate")
     args = ap.parse_args()
     if not args.budget:
-        args.budget = {3: 1 << 22, 4: 1 << 18}.get(args.config, 1 << 26)
+        args.budget = {3: 1 << 22, 4: 1 << 18, 5: 1 << 27}.get(args.config, 1 << 26)
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py runs on one MI355X only: this build has no multi-GPU path "
@@ -365,13 +385,16 @@ def main():
     elif args.config == 4:
         # CurBankSize = 1 << 20 (8 MiB banks); 64 GiB of banks, 64 GiB of pages
         engine = fm.Engine(soln_prealloc=64 << 30, output_prealloc=64 << 30, bank_size=1 << 20)
+    elif args.config == 5:
+        # tables up to ~100 GB resident at once
+        engine = fm.Engine(soln_prealloc=64 << 30, output_prealloc=128 << 30)
     else:
         engine = fm.Engine()
     ei = engine.info()
     log(f"engine: {ei.device_name.decode()} CUs={ei.compute_units} waves/CU={ei.waves_per_cu} "
         f"grid={ei.grid_waves} banks={ei.banks}x{ei.bank_bytes}B")
 
-    nq = args.queries or {2: 32, 3: 1024, 4: 512}[args.config]
+    nq = args.queries or {2: 32, 3: 1024, 4: 512, 5: 64}[args.config]
     t1 = time.time()
     manifest = None
     if args.load_queries:
@@ -387,6 +410,16 @@ def main():
             c2, n2 = screen(engine, g, more, args.budget, nq - len(chosen),
                             per_shape={3: nq // 2 - sum(1 for c in chosen if c[0] == 3),
                                        1: nq - nq // 2 - sum(1 for c in chosen if c[0] == 1)})
+            chosen += c2
+            counts += n2
+    elif args.config == 5:
+        # at most 2.5e9 rows in all (100 GB of 10-vertex rows)
+        chosen, counts, rnd = [], [], 0
+        while len(chosen) < nq and rnd < 24:
+            cands = make_candidates(g, 5, args.seed + 7919 * rnd, 128)
+            rnd += 1
+            c2, n2 = screen(engine, g, cands, args.budget, nq - len(chosen), max_partials=1 << 31,
+                            total_cap=int(2.5e9) - sum(counts))
             chosen += c2
             counts += n2
     else:
@@ -524,6 +557,49 @@ def main():
                          "kernel_ms_per_step": 1e3 / top["achieved_queries_per_s"]},
         }
         workload = f"rmat22_serving_{len(queries)}q_poisson"
+    elif args.config == 5:
+        # batched only: all tables resident in the page arena at once
+        for _ in range(args.warmup):
+            engine.match_batch(g, queries, timeout_ms=300000).close()
+        step_times, kms, rb = [], [], None
+        for _ in range(args.steps):
+            if rb is not None:
+                rb.close()
+            ts = time.perf_counter()
+            rb = engine.match_batch(g, queries, timeout_ms=300000)
+            step_times.append(time.perf_counter() - ts)
+            kms.append(rb.stats().kernel_ms)
+            log(f"stress step: {step_times[-1] * 1e3:.0f} ms")
+        st = rb.stats()
+        outs = [rb.query(i) for i in range(len(queries))]
+        ok = all(o.status == 0 and o.count == counts[i] and o.rows == o.count for i, o in enumerate(outs))
+        rng = np.random.default_rng(5)
+        for i in range(0, len(queries), 8):
+            ok = ok and validate_sample(g, queries[i], rb.table(i, sorted=False, max_rows=1 << 16), rng)
+        embeddings = sum(o.count for o in outs)
+        ms_step = 1e3 * float(np.mean(step_times))
+        dom_ms = float(np.mean(kms))
+        achieved = sum(balg) / (dom_ms / 1e3) / 1e9
+        result = {
+            "metric": "embeddings/s (batched stress, tables resident)",
+            "value": embeddings / (ms_step / 1e3),
+            "unit": "embeddings/s",
+            "ms_per_step": ms_step,
+            "queries_per_s": len(queries) / (ms_step / 1e3),
+            "embeddings_per_step": embeddings,
+            "table_bytes_per_step": int(sum(4 * 10 * o.count for o in outs)),
+            "largest_query_embeddings": int(max(o.count for o in outs)),
+            "wave_busy_frac": st.busy_frac,
+            "launch": {"tasks": int(st.tasks_pushed), "banks_fresh": int(st.banks_allocated),
+                       "banks_reused": int(st.banks_reused), "bank_alloc_failures": int(st.alloc_failures),
+                       "requeues": int(st.requeues), "splits": int(st.tasks_split),
+                       "pages": int(st.pages_allocated)},
+            "parity_checks_passed": bool(ok),
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "traffic": None, "alg_bytes_per_step": sum(balg),
+                         "kernel": "fm_match_kernel", "kernel_ms_per_step": dom_ms},
+        }
+        workload = f"rmat24_stress_{len(queries)}q_dense10"
     else:
         from forkmatch_amd.autotune import autotune
         space = {"root_chunk": [256, 4096], "task_rows": [64, 1024], "grid_frac": [0.5, 1.0]}

@@ -50,9 +50,11 @@ struct HostGraph {
 };
 
 // Builds a HostGraph from directed (src, dst, label) lines. Duplicate lines
-// are collapsed. Returns FM_OK or a status.
+// are collapsed. Returns FM_OK or a status. A caller that wrote every line
+// in both directions (the generators) skips the symmetry check.
 int build_graph(uint32_t V, uint64_t n, const uint32_t *src, const uint32_t *dst,
-                const uint32_t *label, HostGraph &g);
+                const uint32_t *label, HostGraph &g,
+                bool symmetric_by_construction = false);
 
 struct Req {
     uint32_t anchor;  // depth of the earlier query vertex (== own depth: self-loop)

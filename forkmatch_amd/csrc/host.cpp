@@ -7,6 +7,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <numeric>
 #include <thread>
 #include <unordered_map>
@@ -120,7 +121,11 @@ static int build_dir(uint32_t V, uint32_t L, uint64_t n, const uint32_t *lid,
                      const uint32_t *from, const uint32_t *to, Csr &c) {
     uint64_t rows = (uint64_t)L * V;
     std::vector<uint32_t> cnt(rows + 1, 0);
-    for (uint64_t i = 0; i < n; i++) cnt[(uint64_t)lid[i] * V + from[i]]++;
+    // counting and scattering run on all threads with atomic counters; the
+    // order inside a row is then fixed by the row sort
+    parallel_for(n, [&](uint64_t b, uint64_t e) {
+        for (uint64_t i = b; i < e; i++) __atomic_fetch_add(&cnt[(uint64_t)lid[i] * V + from[i]], 1u, __ATOMIC_RELAXED);
+    });
     c.ptr.assign(rows + 1, 0);
     uint64_t acc = 0;
     for (uint64_t r = 0; r < rows; r++) {
@@ -132,7 +137,10 @@ static int build_dir(uint32_t V, uint32_t L, uint64_t n, const uint32_t *lid,
     {
         // reuse cnt as the running write cursor
         for (uint64_t r = 0; r < rows; r++) cnt[r] = c.ptr[r];
-        for (uint64_t i = 0; i < n; i++) tmp[cnt[(uint64_t)lid[i] * V + from[i]]++] = to[i];
+        parallel_for(n, [&](uint64_t b, uint64_t e) {
+            for (uint64_t i = b; i < e; i++)
+                tmp[__atomic_fetch_add(&cnt[(uint64_t)lid[i] * V + from[i]], 1u, __ATOMIC_RELAXED)] = to[i];
+        });
     }
     // sort + unique each row (in parallel over row ranges), then compact
     std::vector<uint32_t> keep(rows + 1, 0);
@@ -147,17 +155,20 @@ static int build_dir(uint32_t V, uint32_t L, uint64_t n, const uint32_t *lid,
             }
         }
     });
-    uint64_t total = 0;
-    for (uint64_t r = 0; r < rows; r++) total += keep[r];
-    c.col.resize(total);
+    // compacted offsets (keep[] becomes the new row starts), then the copy
     uint64_t w = 0;
     for (uint64_t r = 0; r < rows; r++) {
-        uint32_t s = c.ptr[r];
-        c.ptr[r] = (uint32_t)w;
-        std::memcpy(c.col.data() + w, tmp.data() + s, (size_t)keep[r] * 4);
-        w += keep[r];
+        const uint32_t k = keep[r];
+        keep[r] = (uint32_t)w;
+        w += k;
     }
-    c.ptr[rows] = (uint32_t)w;
+    keep[rows] = (uint32_t)w;
+    c.col.resize(w);
+    parallel_for(rows, [&](uint64_t b, uint64_t e) {
+        for (uint64_t r = b; r < e; r++)
+            std::memcpy(c.col.data() + keep[r], tmp.data() + c.ptr[r], (size_t)(keep[r + 1] - keep[r]) * 4);
+    });
+    c.ptr.swap(keep);
     // per-label root lists
     c.root_ptr.assign(L + 1, 0);
     c.roots.clear();
@@ -192,7 +203,7 @@ static bool row_has(const Csr &c, uint32_t V, uint32_t l, uint32_t v, uint32_t x
 }
 
 int build_graph(uint32_t V, uint64_t n, const uint32_t *src, const uint32_t *dst,
-                const uint32_t *label, HostGraph &g) {
+                const uint32_t *label, HostGraph &g, bool symmetric_by_construction) {
     if (n >= (1ull << 32) - 1)
         return fail(FM_ERR_NOT_SUPPORTED, "graph has %llu directed edges; the CSR holds < 2^32",
                     (unsigned long long)n);
@@ -202,8 +213,15 @@ int build_graph(uint32_t V, uint64_t n, const uint32_t *src, const uint32_t *dst
                         (unsigned long long)i, src[i], dst[i], V);
     g.V = V;
     // dense label ids
-    uint32_t maxl = 0;
-    for (uint64_t i = 0; i < n; i++) maxl = std::max(maxl, label[i]);
+    std::atomic<uint32_t> maxa{0};
+    parallel_for(n, [&](uint64_t b, uint64_t e) {
+        uint32_t m = 0;
+        for (uint64_t i = b; i < e; i++) m = std::max(m, label[i]);
+        uint32_t cur = maxa.load();
+        while (m > cur && !maxa.compare_exchange_weak(cur, m)) {
+        }
+    });
+    const uint32_t maxl = maxa.load();
     std::vector<uint32_t> lid(n);
     g.label_values.clear();
     if (maxl < (1u << 24)) {
@@ -214,7 +232,9 @@ int build_graph(uint32_t V, uint64_t n, const uint32_t *src, const uint32_t *dst
                 map[v] = (uint32_t)g.label_values.size();
                 g.label_values.push_back(v);
             }
-        for (uint64_t i = 0; i < n; i++) lid[i] = map[label[i]];
+        parallel_for(n, [&](uint64_t b, uint64_t e) {
+            for (uint64_t i = b; i < e; i++) lid[i] = map[label[i]];
+        });
     } else {
         g.label_values.assign(label, label + n);
         std::sort(g.label_values.begin(), g.label_values.end());
@@ -230,7 +250,7 @@ int build_graph(uint32_t V, uint64_t n, const uint32_t *src, const uint32_t *dst
     build_dir(V, L, n, lid.data(), src, dst, g.out);
     // symmetry: every (u, v, l) has (v, u, l)
     std::atomic<bool> sym{true};
-    parallel_for((uint64_t)L * V, [&](uint64_t b, uint64_t e) {
+    if (!symmetric_by_construction) parallel_for((uint64_t)L * V, [&](uint64_t b, uint64_t e) {
         for (uint64_t r = b; r < e && sym.load(std::memory_order_relaxed); r++) {
             uint32_t l = (uint32_t)(r / V), u = (uint32_t)(r % V);
             for (uint32_t k = g.out.ptr[r]; k < g.out.ptr[r + 1]; k++)
@@ -806,7 +826,7 @@ static int finish_generated(uint32_t V, const std::vector<uint64_t> &und, uint32
         d[2 * i + 1] = a;
         l[2 * i + 1] = lab;
     }
-    int rc = build_graph(V, 2 * m, s.data(), d.data(), l.data(), g);
+    int rc = build_graph(V, 2 * m, s.data(), d.data(), l.data(), g, true);
     g.header_edges = (uint32_t)std::min<uint64_t>(2 * m, UINT32_MAX);
     return rc;
 }
@@ -828,6 +848,54 @@ int gen_er(uint32_t V, uint64_t m, uint32_t nlabels, int dist, double zs, uint64
     return finish_generated(V, und, nlabels, dist, zs, rng, g);
 }
 
+// 64-bit finaliser of splitmix64: a keyed hash for seeds, ranks and labels.
+static uint64_t mix64(uint64_t x) {
+    x ^= x >> 30;
+    x *= 0xBF58476D1CE4E5B9ull;
+    x ^= x >> 27;
+    x *= 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+// Open-addressing set of undirected edge keys that several threads insert
+// into at once (compare-and-swap on empty slots).
+struct SharedEdgeSet {
+    std::vector<uint64_t> slots;
+    uint64_t mask;
+    explicit SharedEdgeSet(uint64_t expected) {
+        uint64_t cap = 1;
+        while (cap < expected * 2 + 16) cap <<= 1;
+        slots.resize(cap);
+        mask = cap - 1;
+        parallel_for(cap, [&](uint64_t b, uint64_t e) {
+            std::fill(slots.begin() + b, slots.begin() + e, ~0ull);
+        });
+    }
+    bool insert(uint64_t key) {
+        uint64_t i = mix64(key) & mask;
+        while (true) {
+            uint64_t cur = __atomic_load_n(&slots[i], __ATOMIC_RELAXED);
+            if (cur == key) return false;
+            if (cur == ~0ull) {
+                uint64_t expected = ~0ull;
+                if (__atomic_compare_exchange_n(&slots[i], &expected, key, false, __ATOMIC_RELAXED,
+                                                __ATOMIC_RELAXED))
+                    return true;
+                if (expected == key) return false;
+            }
+            i = (i + 1) & mask;
+        }
+    }
+};
+
+// R-MAT: each edge picks one of the four quadrants (a, b, c, d) per bit of
+// the vertex ids; self-loops are dropped and duplicates collapsed. The
+// candidate stream is cut into fixed chunks, each with its own seeded
+// generator, and chunks run on all host threads. The edge set is the union
+// of the candidates (rounds are added until it holds m edges); when it holds
+// more, the m edges of smallest keyed hash are kept. Labels are a keyed
+// function of the edge. So the graph depends on the seed only, not on the
+// thread count or timing.
 int gen_rmat(uint32_t scale, uint64_t m, double a, double b, double c, uint32_t nlabels, int dist,
              double zs, uint64_t seed, HostGraph &g) {
     if (scale == 0 || scale > 31) return fail(FM_ERR_INVALID_ARG, "R-MAT scale %u out of range", scale);
@@ -835,33 +903,85 @@ int gen_rmat(uint32_t scale, uint64_t m, double a, double b, double c, uint32_t 
     if (m > (uint64_t)V * (V - 1) / 4)
         return fail(FM_ERR_INVALID_ARG, "R-MAT scale %u cannot hold %llu edges", scale,
                     (unsigned long long)m);
-    Rng rng(seed);
-    EdgeSet set(m);
-    std::vector<uint64_t> und;
-    und.reserve(m);
-    double ab = a + b, abc = a + b + c;
-    while (und.size() < m) {
-        uint32_t u = 0, v = 0;
-        for (uint32_t bit = 0; bit < scale; bit++) {
-            double r = rng.uniform();
-            uint32_t du = 0, dv = 0;
-            if (r < a) {
-            } else if (r < ab) {
-                dv = 1;
-            } else if (r < abc) {
-                du = 1;
-            } else {
-                du = 1;
-                dv = 1;
-            }
-            u = (u << 1) | du;
-            v = (v << 1) | dv;
+    const double ab = a + b, abc = a + b + c;
+    std::unique_ptr<SharedEdgeSet> set(new SharedEdgeSet(m + m / 8));
+    std::atomic<uint64_t> uniq{0};
+    for (uint64_t round = 0; uniq.load() < m; round++) {
+        const uint64_t need = m - uniq.load();
+        const uint64_t gen = round == 0 ? m + m / 8 : need + need / 2 + 1024;
+        // chunk size from the round size only, so the stream does not
+        // depend on the thread count
+        const uint64_t CH = std::min<uint64_t>(1ull << 16, std::max<uint64_t>(1024, gen / 64));
+        const uint64_t nchunks = (gen + CH - 1) / CH;
+        // room for every candidate of the round at load <= 1/2
+        const uint64_t bound = uniq.load() + nchunks * CH;
+        if (2 * bound + 16 > set->slots.size()) {
+            std::unique_ptr<SharedEdgeSet> bigger(new SharedEdgeSet(bound));
+            for (uint64_t k : set->slots)
+                if (k != ~0ull) bigger->insert(k);
+            set.swap(bigger);
         }
-        if (u == v) continue;
-        uint64_t key = ((uint64_t)std::min(u, v) << 32) | std::max(u, v);
-        if (set.insert(key)) und.push_back(key);
+        std::atomic<uint64_t> next{0};
+        auto worker = [&]() {
+            for (uint64_t k; (k = next.fetch_add(1)) < nchunks;) {
+                Rng rng(mix64(seed ^ mix64((round << 40) ^ k)));
+                uint64_t added = 0;
+                for (uint64_t i = 0; i < CH; i++) {
+                    uint32_t u = 0, v = 0;
+                    for (uint32_t bit = 0; bit < scale; bit++) {
+                        const double r = rng.uniform();
+                        const uint32_t du = r >= ab, dv = (r >= a && r < ab) || r >= abc;
+                        u = (u << 1) | du;
+                        v = (v << 1) | dv;
+                    }
+                    if (u == v) continue;
+                    added += set->insert(((uint64_t)std::min(u, v) << 32) | std::max(u, v));
+                }
+                uniq.fetch_add(added);
+            }
+        };
+        const unsigned T = nchunks > 1 ? hw_threads() : 1;
+        std::vector<std::thread> th;
+        for (unsigned t = 1; t < T; t++) th.emplace_back(worker);
+        worker();
+        for (auto &x : th) x.join();
     }
-    return finish_generated(V, und, nlabels, dist, zs, rng, g);
+    // the set's edges (slot order varies with timing; nothing below depends on it)
+    std::vector<uint64_t> und;
+    und.reserve(uniq.load());
+    for (uint64_t k : set->slots)
+        if (k != ~0ull) und.push_back(k);
+    set.reset();
+    if (und.size() > m) {
+        const uint64_t key2 = mix64(seed ^ 0x52414E4Bull);
+        auto rank_less = [&](uint64_t x, uint64_t y) {
+            const uint64_t hx = mix64(x ^ key2), hy = mix64(y ^ key2);
+            return hx != hy ? hx < hy : x < y;
+        };
+        std::nth_element(und.begin(), und.begin() + m, und.end(), rank_less);
+        und.resize(m);
+    }
+    // labels: a keyed draw per undirected edge, written on both lines
+    LabelSampler ls(nlabels, dist, zs);
+    std::vector<uint32_t> s(2 * m), d(2 * m), l(2 * m);
+    const uint64_t key3 = mix64(seed ^ 0x4C4142454Cull);
+    parallel_for(m, [&](uint64_t lo, uint64_t hi) {
+        for (uint64_t i = lo; i < hi; i++) {
+            const uint32_t x = (uint32_t)(und[i] >> 32), y = (uint32_t)und[i];
+            Rng lr(mix64(und[i] ^ key3));
+            const uint32_t lab = ls(lr);
+            s[2 * i] = x;
+            d[2 * i] = y;
+            l[2 * i] = lab;
+            s[2 * i + 1] = y;
+            d[2 * i + 1] = x;
+            l[2 * i + 1] = lab;
+        }
+    });
+    std::vector<uint64_t>().swap(und);
+    int rc = build_graph(V, 2 * m, s.data(), d.data(), l.data(), g, true);
+    g.header_edges = (uint32_t)std::min<uint64_t>(2 * m, UINT32_MAX);
+    return rc;
 }
 
 // Query extraction on the data graph (undirected view over all labels),

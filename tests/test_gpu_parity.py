@@ -190,9 +190,9 @@ def test_cost_and_reference_plans_give_the_same_table(engine):
     g = fm.Graph.rmat(11, 12000, 12, zipf_s=1.2, seed=21).upload()
     og = _oracle_graph(g)
     # seeds whose cost-based order differs from the reference order, with
-    # 875 .. 480,028 embeddings
-    for i, (n, shape, seed) in enumerate([(5, 0, 401), (6, 0, 404), (6, 1, 401), (7, 1, 400), (7, 2, 405),
-                                          (9, 2, 405)]):
+    # 768 .. 394,342 embeddings
+    for i, (n, shape, seed) in enumerate([(5, 0, 401), (6, 0, 404), (6, 1, 401), (7, 1, 401), (7, 2, 401),
+                                          (9, 2, 401)]):
         q = fm.Query.random_walk(g, n, shape=shape, min_cycles=1 if shape == 1 else 0, seed=seed)
         assert q.plan(g).order != q.plan().order
         ref = orc.match(og, n, q.edges(), threads=4)
