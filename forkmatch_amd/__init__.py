@@ -254,6 +254,20 @@ class Query:
         k = info.nreqs
         return Plan(order.tolist(), rb.tolist(), an[:k].tolist(), lb[:k].tolist(), dr[:k].tolist())
 
+    def symmetry(self, max_auts: int = 48):
+        """(automorphisms, constraints): auts[k][u] = p_k(u) (identity
+        first; empty when there are more than max_auts) and pairs (a, b)
+        meaning m(a) < m(b) (symmetry breaking)."""
+        na, nl = C.c_uint32(), C.c_uint32()
+        _check(L.lib().fm_query_symmetry(self._h, max_auts, C.byref(na), None, 0, C.byref(nl), None, 0))
+        n = self.vertices
+        auts = np.zeros(max(na.value * n, 1), np.uint32)
+        less = np.zeros(max(2 * nl.value, 1), np.uint32)
+        _check(L.lib().fm_query_symmetry(self._h, max_auts, C.byref(na), _p32(auts), len(auts), C.byref(nl),
+                                         _p32(less), len(less)))
+        return (auts[: na.value * n].reshape(na.value, n).tolist(),
+                [tuple(x) for x in less[: 2 * nl.value].reshape(nl.value, 2).tolist()])
+
     def write_g(self, path: str) -> None:
         _check(L.lib().fm_query_write_g(self._h, str(path).encode()))
 
@@ -355,10 +369,11 @@ class Engine:
 
     @staticmethod
     def _opts(mode, instrument, max_rows, max_count, timeout_ms, grid_waves, task_rows, root_chunk,
-              max_partials=0, plan=0):
+              max_partials=0, plan=0, symmetry=False):
         o = L.MatchOpts()
         o.max_partials = max_partials
         o.plan = plan
+        o.symmetry = 1 if symmetry else 0
         o.mode = mode
         o.instrument = 1 if instrument else 0
         o.max_rows = max_rows
@@ -375,9 +390,9 @@ class Engine:
     def match_batch(self, graph: Graph, queries, table: bool = True, instrument: bool = False,
                     max_rows: int = 0, max_count: int = 0, timeout_ms: int = 0, grid_waves: int = 0,
                     task_rows: int = 0, root_chunk: int = 0, max_partials: int = 0,
-                    allow_errors: bool = False, plan: int = 0) -> Result:
+                    allow_errors: bool = False, plan: int = 0, symmetry: bool = False) -> Result:
         o = self._opts(FM_MODE_TABLE if table else FM_MODE_COUNT, instrument, max_rows, max_count, timeout_ms,
-                       grid_waves, task_rows, root_chunk, max_partials, plan)
+                       grid_waves, task_rows, root_chunk, max_partials, plan, symmetry)
         arr = (C.c_void_p * len(queries))(*[q._h for q in queries])
         h = C.c_void_p()
         rc = L.lib().fm_match_batch(self._h, graph._h, arr, len(queries), C.byref(o), C.byref(h))
@@ -421,11 +436,12 @@ class Server:
 
     def __init__(self, engine: Engine, graph: Graph, slots: int = 256, slot_pages: int = 256, bank_budget: int = 0,
                  idle_ms: int = 20, root_chunk: int = 0, task_rows: int = 0, table: bool = True,
-                 max_launch_ms: int = 0, plan: int = 0, slice_ms: int = 0):
+                 max_launch_ms: int = 0, plan: int = 0, slice_ms: int = 0, symmetry: bool = False):
         cfg = L.ServerConfig()
         cfg.slots, cfg.slot_pages, cfg.bank_budget, cfg.idle_ms = slots, slot_pages, bank_budget, idle_ms
         cfg.root_chunk, cfg.task_rows, cfg.table = root_chunk, task_rows, 1 if table else 0
         cfg.max_launch_ms, cfg.plan, cfg.slice_ms = max_launch_ms, plan, slice_ms
+        cfg.symmetry = 1 if symmetry else 0
         h = C.c_void_p()
         _check(L.lib().fm_server_create(engine._h, graph._h, C.byref(cfg), C.byref(h)))
         self._h = h

@@ -42,7 +42,7 @@ class EngineInfo(C.Structure):
 class MatchOpts(C.Structure):
     _fields_ = [("mode", u32), ("instrument", u32), ("max_rows", u64), ("max_count", u64),
                 ("timeout_ms", u32), ("grid_waves", u32), ("task_rows", u32), ("root_chunk", u32),
-                ("plan", u32), ("max_partials", u64), ("reserved", u32 * 4)]
+                ("plan", u32), ("max_partials", u64), ("symmetry", u32), ("reserved", u32 * 3)]
 
 
 class QueryResult(C.Structure):
@@ -61,7 +61,7 @@ class LaunchStats(C.Structure):
 class ServerConfig(C.Structure):
     _fields_ = [("slots", u32), ("slot_pages", u32), ("bank_budget", u32), ("idle_ms", u32), ("root_chunk", u32),
                 ("task_rows", u32), ("table", u32), ("max_launch_ms", u32), ("plan", u32), ("slice_ms", u32),
-                ("reserved", u32 * 6)]
+                ("symmetry", u32), ("reserved", u32 * 5)]
 
 
 class ServerCompletion(C.Structure):
@@ -100,6 +100,7 @@ SIGNATURES = {
     "fm_query_edges": (C.c_int, [vp, P(u32), P(u32), P(u32)]),
     "fm_query_plan": (C.c_int, [vp, P(PlanInfo), P(u32), P(u32), P(u32), P(u32), P(u32)]),
     "fm_query_plan_on": (C.c_int, [vp, vp, u32, P(PlanInfo), P(u32), P(u32), P(u32), P(u32), P(u32)]),
+    "fm_query_symmetry": (C.c_int, [vp, u32, P(u32), P(u32), u32, P(u32), P(u32), u32]),
     "fm_query_destroy": (None, [vp]),
     "fm_engine_create": (C.c_int, [P(EngineConfig), P(vp)]),
     "fm_engine_destroy": (None, [vp]),

@@ -53,15 +53,23 @@ __host__ __device__ inline uint32_t req_anchor(uint32_t r) { return r & 63u; }
 __host__ __device__ inline uint32_t req_dir(uint32_t r) { return (r >> 7) & 1u; }
 __host__ __device__ inline uint32_t req_label(uint32_t r) { return r >> 8; }
 
+constexpr uint32_t MAXAUT = FM_MAX_AUTOMORPHISMS;
+constexpr uint32_t MAXSC = 64;  // symmetry-breaking constraints per plan
+
 struct DPlan {
     uint32_t n;
     uint32_t root_base;  // ALL_VERTICES or offset into the combined root list
     uint32_t nroots;
-    uint32_t pad;
+    uint32_t naut;       // > 1: symmetry breaking, every embedding found is expanded naut ways
     uint32_t order[MAXN];
     uint32_t req_begin[MAXN + 1];
     uint32_t nself[MAXN];  // number of self-loop requirements per depth
     uint32_t req[FM_MAX_REQS_TOTAL];  // per depth: non-self first, then self
+    // ordering constraints of each depth: anchor depth, bit 7 set when the
+    // candidate must be greater than the anchor's vertex (else smaller)
+    uint32_t sc_begin[MAXN + 1];
+    uint32_t sc[MAXSC];
+    uint8_t autcol[MAXAUT][MAXN];  // expansion k: column u takes the vertex of depth autcol[k][u]
 };
 
 struct DQuery {
@@ -1010,6 +1018,10 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
                 ok = true;
                 const uint32_t *prow = &par[pi * W];
                 for (uint32_t a = 0; a < W; a++) ok &= prow[a] != x;
+                for (uint32_t i = pl.sc_begin[D]; i < pl.sc_begin[D + 1] && ok; i++) {
+                    const uint32_t c = pl.sc[i], v = prow[c & 63u];
+                    ok = (c & 128u) ? x > v : x < v;
+                }
                 const uint32_t dx = s.didx[pi];
                 for (uint32_t i = 0; i < k && ok; i++)
                     if (i != dx) {
@@ -1032,13 +1044,25 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
             const uint32_t rank = __popcll(mask & ((1ull << lane) - 1ull));
             if (P.instrument && lane == 0) s.inst[4 + D] += cnt;
             if (final_level) {
-                cnt_acc += cnt;
+                const uint32_t naut = pl.naut > 1 ? pl.naut : 1u;
+                cnt_acc += (unsigned long long)cnt * naut;
                 if (P.table) {
                     const uint32_t *prow = &par[pi * W];
-                    emit_rows(P, wt, t.q, n, ok, rank, cnt, stats, [&](uint32_t *dst) {
-                        for (uint32_t j = 0; j < W; j++) dst[pl.order[j]] = prow[j];
-                        dst[pl.order[D]] = x;
-                    });
+                    if (naut == 1) {
+                        emit_rows(P, wt, t.q, n, ok, rank, cnt, stats, [&](uint32_t *dst) {
+                            for (uint32_t j = 0; j < W; j++) dst[pl.order[j]] = prow[j];
+                            dst[pl.order[D]] = x;
+                        });
+                    } else {
+                        // one row per automorphism p: column u holds m(p(u))
+                        for (uint32_t k = 0; k < naut; k++)
+                            emit_rows(P, wt, t.q, n, ok, rank, cnt, stats, [&](uint32_t *dst) {
+                                for (uint32_t u = 0; u < n; u++) {
+                                    const uint32_t dd = pl.autcol[k][u];
+                                    dst[u] = dd == D ? x : prow[dd];
+                                }
+                            });
+                    }
                 }
             } else {
                 const uint32_t CW = D + 1;
@@ -1871,7 +1895,8 @@ int fm_engine_info_get(fm_engine *e, fm_engine_info *out) {
 
 // Device plan of a query against a graph: dense label ids, symmetric
 // de-duplication, root list selection.
-static int make_dplan(const fm_graph *g, const HostQuery &q, bool all_roots, uint32_t plan, DPlan &dp) {
+static int make_dplan(const fm_graph *g, const HostQuery &q, bool all_roots, uint32_t plan, uint32_t symmetry,
+                      DPlan &dp) {
     const HostGraph &h = g->h;
     if (plan != FM_PLAN_COST && plan != FM_PLAN_REFERENCE) return fail(FM_ERR_INVALID_ARG, "unknown plan %u", plan);
     Plan cost_plan;
@@ -1901,6 +1926,28 @@ static int make_dplan(const fm_graph *g, const HostQuery &q, bool all_roots, uin
         dp.nself[d] = (uint32_t)self.size();
     }
     dp.req_begin[p.n] = nr;
+    // symmetry breaking: constraints placed at the later of their two depths
+    if (symmetry && p.n > 1) {
+        Symmetry sym;
+        query_symmetry(q, MAXAUT, sym);
+        if (sym.complete && sym.auts.size() > 1 && sym.less.size() <= MAXSC) {
+            std::vector<std::vector<uint32_t>> per(p.n);
+            for (auto &c : sym.less) {
+                const uint32_t da = p.depth_of[c.first], db = p.depth_of[c.second];
+                if (da > db) per[da].push_back(db);            // candidate a < m(b)
+                else per[db].push_back(da | 128u);             // candidate b > m(a)
+            }
+            uint32_t k = 0;
+            for (uint32_t d = 0; d < p.n; d++) {
+                dp.sc_begin[d] = k;
+                for (uint32_t c : per[d]) dp.sc[k++] = c;
+            }
+            dp.sc_begin[p.n] = k;
+            dp.naut = (uint32_t)sym.auts.size();
+            for (uint32_t a = 0; a < dp.naut; a++)
+                for (uint32_t u = 0; u < p.n; u++) dp.autcol[a][u] = (uint8_t)p.depth_of[sym.auts[a][u]];
+        }
+    }
     if (p.n == 1 || all_roots) {
         dp.root_base = ALL_VERTICES;
         dp.nroots = h.V;
@@ -1954,7 +2001,7 @@ int fm_match_batch(fm_engine *e, const fm_graph *g, const fm_query *const *qs, u
     uint32_t maxn = 1;
     for (uint32_t i = 0; i < nq; i++) {
         if (!qs[i]) return fail(FM_ERR_INVALID_ARG, "null query %u", i);
-        int rc = make_dplan(g, qs[i]->h, o.instrument != 0, o.plan, plans[i]);
+        int rc = make_dplan(g, qs[i]->h, o.instrument != 0, o.plan, o.symmetry, plans[i]);
         if (rc) return rc;
         DQuery &Q = qh[i];
         std::memset(&Q, 0, sizeof Q);
@@ -2390,7 +2437,7 @@ int fm_server_submit(fm_server *s, const fm_query *q, uint64_t max_rows, uint32_
     }
     uint32_t slot = s->free_slots.back();
     DPlan dp;
-    int rc = make_dplan(s->g, q->h, false, s->cfg.plan, dp);
+    int rc = make_dplan(s->g, q->h, false, s->cfg.plan, s->cfg.symmetry, dp);
     if (rc) return rc;
     const uint32_t n = dp.n;
     SrvQInit qi{};

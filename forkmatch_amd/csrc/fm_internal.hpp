@@ -87,6 +87,17 @@ int plan_from_order(const HostQuery &q, const std::vector<uint32_t> &order, Plan
 // whose estimated number of examined candidates is smallest.
 int make_cost_plan(const HostQuery &q, const HostGraph &g, Plan &p);
 
+// Symmetry of a query: its automorphisms (permutations p of the query
+// vertices with (p(u), p(w), l) a line for every line (u, w, l)) and
+// ordering constraints m(a) < m(b) that keep exactly one embedding of every
+// orbit of the automorphism group acting on embeddings.
+struct Symmetry {
+    std::vector<std::vector<uint32_t>> auts;              // auts[k][u] = p_k(u); auts[0] = identity
+    std::vector<std::pair<uint32_t, uint32_t>> less;      // (a, b): m(a) < m(b)
+    bool complete = true;                                 // false: more than the cap, none listed
+};
+void query_symmetry(const HostQuery &q, uint32_t max_auts, Symmetry &s);
+
 // Reads a whole file. Returns false if it cannot be opened.
 bool read_file(const std::string &path, std::string &buf);
 

@@ -7,6 +7,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <numeric>
 #include <thread>
@@ -656,6 +657,94 @@ int make_cost_plan(const HostQuery &q, const HostGraph &g, Plan &p) {
         }
     }
     return plan_from_order(q, best_order, p);
+}
+
+// Automorphisms by backtracking over the vertices in id order: a vertex may
+// only go to a vertex with the same (label, direction) incidence signature,
+// and every line between assigned vertices must map onto a line. The
+// constraints follow Grochow and Kellis ("Network motif discovery using
+// subgraph enumeration and symmetry-breaking", RECOMB 2007): while the group
+// G is not trivial, take the smallest vertex v with a non-trivial orbit,
+// require m(v) < m(u) for every other u in the orbit, and replace G by the
+// stabiliser of v. Every orbit of embeddings {m o p : p in Aut} then has
+// exactly one member that satisfies all constraints.
+void query_symmetry(const HostQuery &q, uint32_t max_auts, Symmetry &s) {
+    const uint32_t n = q.n;
+    s.auts.clear();
+    s.less.clear();
+    s.complete = true;
+    std::unordered_set<uint64_t> lines;
+    auto key = [](uint32_t u, uint32_t w, uint32_t l) { return ((uint64_t)l << 32) | ((uint64_t)u << 16) | w; };
+    std::vector<std::vector<uint64_t>> sig(n);
+    for (auto &e : q.edges) {
+        lines.insert(key(e[0], e[1], e[2]));
+        sig[e[0]].push_back(((uint64_t)e[2] << 2) | (e[0] == e[1] ? 2u : 0u));
+        if (e[0] != e[1]) sig[e[1]].push_back(((uint64_t)e[2] << 2) | 1u);
+    }
+    for (auto &x : sig) std::sort(x.begin(), x.end());
+    std::vector<uint32_t> p(n, UINT32_MAX);
+    std::vector<char> used(n, 0);
+    bool over = false;
+    std::function<void(uint32_t)> rec = [&](uint32_t v) {
+        if (over) return;
+        if (v == n) {
+            if (s.auts.size() >= max_auts) {
+                over = true;
+                return;
+            }
+            s.auts.push_back(p);
+            return;
+        }
+        for (uint32_t x = 0; x < n && !over; x++) {
+            if (used[x] || sig[x] != sig[v]) continue;
+            p[v] = x;
+            bool ok = true;
+            for (auto &e : q.edges) {
+                const uint32_t a = e[0], b = e[1];
+                if (a > v || b > v || (a != v && b != v)) continue;  // both assigned, one of them v
+                if (!lines.count(key(p[a], p[b], e[2]))) {
+                    ok = false;
+                    break;
+                }
+            }
+            if (ok) {
+                used[x] = 1;
+                rec(v + 1);
+                used[x] = 0;
+            }
+        }
+        p[v] = UINT32_MAX;
+    };
+    rec(0);
+    if (over) {
+        s.auts.clear();
+        s.complete = false;
+        return;
+    }
+    // the identity first (the search meets it first already: vertex order)
+    std::vector<uint32_t> group(s.auts.size());
+    std::iota(group.begin(), group.end(), 0);
+    while (group.size() > 1) {
+        uint32_t v = UINT32_MAX;
+        std::vector<char> orbit(n, 0);
+        for (uint32_t u = 0; u < n && v == UINT32_MAX; u++) {
+            std::fill(orbit.begin(), orbit.end(), 0);
+            uint32_t size = 0;
+            for (uint32_t k : group)
+                if (!orbit[s.auts[k][u]]) {
+                    orbit[s.auts[k][u]] = 1;
+                    size++;
+                }
+            if (size > 1) v = u;
+        }
+        if (v == UINT32_MAX) break;
+        for (uint32_t u = 0; u < n; u++)
+            if (orbit[u] && u != v) s.less.push_back({v, u});
+        std::vector<uint32_t> stab;
+        for (uint32_t k : group)
+            if (s.auts[k][v] == v) stab.push_back(k);
+        group.swap(stab);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1361,6 +1450,29 @@ int fm_query_plan_on(const fm_graph *g, const fm_query *q, uint32_t plan, fm_pla
     int rc = make_cost_plan(q->h, g->h, p);
     if (rc) return rc;
     export_plan(p, info, order, req_begin, req_anchor, req_label, req_dir);
+    return FM_OK;
+}
+
+int fm_query_symmetry(const fm_query *q, uint32_t max_auts, uint32_t *n_aut, uint32_t *auts, uint32_t aut_cap,
+                      uint32_t *n_less, uint32_t *less, uint32_t less_cap) {
+    if (!q || !n_aut || !n_less) return fail(FM_ERR_INVALID_ARG, "null argument");
+    Symmetry sym;
+    query_symmetry(q->h, max_auts, sym);
+    const uint32_t n = q->h.n;
+    *n_aut = (uint32_t)sym.auts.size();
+    *n_less = (uint32_t)sym.less.size();
+    if (auts) {
+        if ((uint64_t)aut_cap < (uint64_t)*n_aut * n) return fail(FM_ERR_INVALID_ARG, "auts holds %u entries", aut_cap);
+        for (size_t k = 0; k < sym.auts.size(); k++)
+            for (uint32_t u = 0; u < n; u++) auts[k * n + u] = sym.auts[k][u];
+    }
+    if (less) {
+        if (less_cap < 2 * *n_less) return fail(FM_ERR_INVALID_ARG, "less holds %u entries", less_cap);
+        for (size_t i = 0; i < sym.less.size(); i++) {
+            less[2 * i] = sym.less[i].first;
+            less[2 * i + 1] = sym.less[i].second;
+        }
+    }
     return FM_OK;
 }
 

@@ -184,6 +184,15 @@ typedef struct fm_plan_info {
 int fm_query_plan(const fm_query *q, fm_plan_info *info, uint32_t *order,
                   uint32_t *req_begin, uint32_t *req_anchor,
                   uint32_t *req_label, uint32_t *req_dir);
+/* Automorphisms of q and the symmetry-breaking constraints used by
+ * opts.symmetry: auts (n_aut x vertices entries, auts[k*n + u] = p_k(u),
+ * the identity first) and less (2 x n_less entries: m(less[2i]) <
+ * m(less[2i+1])). Arrays may be NULL to query the sizes; when q has more
+ * than max_auts automorphisms, *n_aut = 0 and none are listed. */
+#define FM_MAX_AUTOMORPHISMS 48
+int fm_query_symmetry(const fm_query *q, uint32_t max_auts, uint32_t *n_aut,
+                      uint32_t *auts, uint32_t aut_cap, uint32_t *n_less,
+                      uint32_t *less, uint32_t less_cap);
 /* The plan fm_match uses for q on graph g under `plan` (FM_PLAN_*), in the
  * layout of fm_query_plan. */
 int fm_query_plan_on(const fm_graph *g, const fm_query *q, uint32_t plan,
@@ -248,7 +257,15 @@ typedef struct fm_match_opts {
     /* abort a query once it has stored more than this many intermediate
      * partial embeddings (0 = none); its status becomes COUNT_LIMIT */
     uint64_t max_partials;
-    uint32_t reserved[4];
+    /* 1: exact symmetry breaking. Embeddings are enumerated once per orbit
+     * of the query's automorphism group (ordering constraints on the
+     * matched vertices) and every one found is expanded by the group, so the
+     * count and the table are those of full enumeration. Queries with more
+     * than FM_MAX_AUTOMORPHISMS automorphisms are enumerated in full. No
+     * reference counterpart: the reference's rule (ccsr.cuh:342-676,
+     * smsm.cu:187-227) is inexact for C5, C6 and the bowtie. */
+    uint32_t symmetry;
+    uint32_t reserved[3];
 } fm_match_opts;
 
 /* One query in one launch (the reference's match()). */
@@ -331,7 +348,8 @@ typedef struct fm_server_config {
     uint32_t slice_ms;       /* a launch retires after this long and the next
                                 submit/poll relaunches it; queued work stays
                                 on the device (default 2000) */
-    uint32_t reserved[6];
+    uint32_t symmetry;       /* as fm_match_opts.symmetry */
+    uint32_t reserved[5];
 } fm_server_config;
 typedef struct fm_server_completion {
     uint32_t ticket;         /* returned by fm_server_submit */
