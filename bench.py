@@ -190,13 +190,16 @@ def validate_sample(g, q, table: np.ndarray, rng, k=256) -> bool:
 def cpu_baseline(g, queries, target_s: float, threads: int):
     """The oracle (C backtracking) on a strided root sample of the same queries."""
     import oracle as orc
+    t0 = time.perf_counter()
     s, d, l = g.edge_list()
     og = orc.OracleGraph(g.vertices, s, d, l)
+    log(f"cpu baseline: oracle graph built in {time.perf_counter() - t0:.1f} s")
     V = g.vertices
     # calibrate the stride on a coarse sample so the run takes ~target_s
     stride = 4096
     probe = sum(orc.match(og, q.vertices, q.edges(), threads=threads, table=False, stride=stride).seconds
                 for _, _, q in queries)
+    log(f"cpu baseline: probe at stride {stride} took {probe:.2f} s")
     if probe > 0:
         stride = int(max(1, min(V, stride * probe / target_s)))
     for _ in range(4):
@@ -205,6 +208,7 @@ def cpu_baseline(g, queries, target_s: float, threads: int):
             r = orc.match(og, q.vertices, q.edges(), threads=threads, table=False, stride=stride)
             total_cnt += r.count
             total_s += r.seconds
+        log(f"cpu baseline: stride {stride}: {total_cnt} embeddings in {total_s:.2f} s")
         # thread start-up dominates a tiny probe: refine until the sample is
         # at least a third of the target
         if total_s >= target_s / 3 or stride == 1:
@@ -318,28 +322,47 @@ def serve_rate(srv, pool, counts, rate: float, seconds: float, rng, balg=None, d
 def serve_mix_check(srv, pool, alone, rng, extra: int = 56) -> bool:
     """The first len(alone) pool queries submitted together with `extra`
     others: each table must equal the table of the same query matched alone."""
+    import faulthandler
     reqs = list(range(len(alone))) + [int(i) for i in rng.integers(0, len(pool), extra)]
     rng.shuffle(reqs)
     t2q, got = {}, {}
     pending = list(reqs)
-    t0 = time.perf_counter()
-    while pending or t2q:
-        while pending:
-            t = srv.submit(pool[pending[0]])
-            if t is None:
-                break
-            t2q[t] = pending.pop(0)
-        for c in srv.poll():
-            qi = t2q.pop(c.ticket)
-            if qi < len(alone) and qi not in got:
-                got[qi] = (c.status, c.count, srv.table(c.ticket, c.rows, c.vertices, sorted=True))
-            srv.release(c.ticket)
-        if time.perf_counter() - t0 > 120:
-            raise RuntimeError("mix check did not drain in 120 s")
+    t0 = last_log = time.perf_counter()
+    log(f"mix check: {len(reqs)} requests, {len(alone)} tables compared")
+    # a call that blocks inside the library shows up as a stack dump
+    faulthandler.dump_traceback_later(100, exit=True)
+    try:
+        while pending or t2q:
+            while pending:
+                t = srv.submit(pool[pending[0]])
+                if t is None:
+                    break
+                t2q[t] = pending.pop(0)
+            for c in srv.poll():
+                qi = t2q.pop(c.ticket)
+                if qi < len(alone) and qi not in got:
+                    got[qi] = (c.status, c.count, srv.table(c.ticket, c.rows, c.vertices, sorted=True), c.flags)
+                srv.release(c.ticket)
+            tn = time.perf_counter()
+            if tn - last_log > 10:
+                last_log = tn
+                st = srv.stats()
+                log(f"mix check: {len(reqs) - len(pending) - len(t2q)}/{len(reqs)} done, {len(t2q)} in flight "
+                    f"(tickets {sorted(t2q)[:8]}), live banks {st.live_banks}, launches {st.launches}, "
+                    f"dropped {st.dropped_tasks}")
+            if tn - t0 > 60:
+                raise RuntimeError(f"mix check did not drain in 60 s ({len(t2q)} in flight)")
+    finally:
+        faulthandler.cancel_dump_traceback_later()
+    log(f"mix check: drained in {time.perf_counter() - t0:.1f} s")
     ok = True
     for qi, (count, table) in enumerate(alone):
-        st, cnt, tab = got[qi]
-        ok = ok and st == 0 and cnt == count and tab.shape == table.shape and bool(np.array_equal(tab, table))
+        st, cnt, tab, flags = got[qi]
+        same = st == 0 and cnt == count and tab.shape == table.shape and bool(np.array_equal(tab, table))
+        if not same:
+            log(f"mix check: query {qi}: status {st} flags {flags:#x} count {cnt} (alone {count}), "
+                f"rows {tab.shape[0]} (alone {table.shape[0]})")
+        ok = ok and same
     return ok
 
 
@@ -533,7 +556,7 @@ def main():
             mix_ok = serve_mix_check(srv, queries, alone, rng)
             sst = srv.stats()
         top = per_rate[-1]
-        ok = mix_ok and all(pr["mismatches"] == 0 for pr in per_rate)
+        ok = mix_ok and all(pr["mismatches"] == 0 for pr in per_rate) and sst.dropped_tasks == 0
         achieved = top["alg_gbs_wall"]
         result = {
             "metric": f"queries/s (serving, Poisson arrivals at {top['offered_req_per_s']:.0f} req/s)",
@@ -545,7 +568,7 @@ def main():
             "rates": per_rate,
             "server": {"slots": 256, "bank_bytes": engine.info().bank_bytes, "banks": banks,
                        "bank_budget": max(1, banks * 3 // 4), "launches": int(sst.launches),
-                       "refused": int(sst.refused)},
+                       "refused": int(sst.refused), "dropped_tasks": int(sst.dropped_tasks)},
             "mix_equals_alone": bool(mix_ok),
             "parity_checks_passed": bool(ok),
             "shape_mix": {name: sum(1 for c in chosen if c[0] == k)

@@ -71,7 +71,7 @@ class ServerCompletion(C.Structure):
 
 class ServerStats(C.Structure):
     _fields_ = [("submitted", u64), ("completed", u64), ("refused", u64), ("launches", u64), ("in_flight", u32),
-                ("live_banks", u32)]
+                ("live_banks", u32), ("dropped_tasks", u64)]
 
 
 # name -> (restype, argtypes); every function declared in include/forkmatch.h

@@ -46,6 +46,8 @@ constexpr uint32_t ROOT_BANK = 0xFFFFFFFFu;
 constexpr uint32_t NO_BANK = 0xFFFFFFFFu;
 constexpr uint32_t ALL_VERTICES = 0xFFFFFFFFu;
 constexpr uint32_t NO_PAGE = 0xFFFFFFFFu;
+// serving: a table page whose allocation failed (its rows are not stored)
+constexpr uint32_t FAILED_PAGE = 0xFFFFFFFEu;
 
 // Packed requirement: bits 0-5 anchor depth (== own depth: self-loop),
 // bit 7 direction (0 out-row, 1 in-row), bits 8-31 dense label id.
@@ -86,6 +88,9 @@ struct DQuery {
     unsigned long long stored;  // intermediate partial embeddings written
     unsigned long long spare;  // partly filled page handed between waves: (id + 1) << 32 | fill
     long long qpending;        // this query's unfinished tasks (serving completion)
+    // serving tables are dense: row r sits in page r / page_rows of the slot
+    unsigned long long cursor;    // rows claimed
+    unsigned long long trunc_at;  // rows from here on are not stored (budget or arena exhausted)
     uint32_t status;
     uint32_t flags;
     uint32_t page_rows;
@@ -107,6 +112,7 @@ enum StatIdx {
     ST_SPLIT,
     ST_BUSY,   // wall-clock ticks waves spent inside tasks and admissions
     ST_ALIVE,  // wall-clock ticks waves were resident
+    ST_STALE,  // serving: tasks of already completed queries, dropped
     ST_N
 };
 
@@ -440,6 +446,11 @@ __device__ void bank_release(const KParams &P, uint32_t id) {
 // candidate `skip` of its driver row and the last one stops at `last_end`
 // (ALL_CANDIDATES: its whole row), which lets a task hold part of a row.
 constexpr uint32_t ALL_CANDIDATES = 0xFFFFFFFFu;
+// TaskDesc::q = query slot | tag << 24. Serving tags a query's tasks with the
+// low byte of its ticket, so a task that outlives its query is recognised
+// (the slot is reused by later queries); batch launches use tag 0.
+constexpr uint32_t QSLOT_MASK = 0xFFFFFFu;
+__device__ __forceinline__ uint32_t qslot(uint32_t q) { return q & QSLOT_MASK; }
 struct TaskDesc {
     uint32_t q, depth, bank, begin, count, skip, last_end;
 };
@@ -454,7 +465,7 @@ __device__ void push_tasks(const KParams &P, uint32_t lvl, uint32_t m, const Tas
     uint32_t pos0 = 0, full = 0;
     if (lane == 0) {
         atom_add(&c->pending, (long long)m);
-        atom_add(&P.qs[t.q].qpending, (long long)m);
+        atom_add(&P.qs[qslot(t.q)].qpending, (long long)m);
         if (bank != ROOT_BANK) atom_add(&P.bank_ref[bank], (int32_t)m);
         pos0 = atom_add(&c->lq[lvl].tail, m);
         uint32_t head = ld_rlx(&c->lq[lvl].head);
@@ -576,12 +587,10 @@ __device__ void tab_sync(const KParams &P, WaveTab &w) {
 
 // Lane 0, at the end of a final-level task. A batch launch keeps the page
 // for the wave's next task of the same query, so a query uses about
-// rows / page_rows + waves pages however small its tasks are. A serving
-// launch hands the page on: a completed query's slot is recycled with its
-// pages, so no wave may hold one past the query's last task.
+// rows / page_rows + waves pages however small its tasks are. Serving
+// tables are dense (emit_rows_dense) and hold no page in the wave.
 __device__ void tab_task_end(const KParams &P, WaveTab &w) {
-    if (P.serving) tab_release(P, w);
-    else tab_sync(P, w);
+    if (!P.serving) tab_sync(P, w);
 }
 
 // Lane 0 only. Takes the query's spare page (if allowed) or claims a fresh
@@ -600,46 +609,10 @@ __device__ bool tab_acquire(const KParams &P, WaveTab &w, uint32_t q, uint32_t c
             return true;
         }
     }
-    if (ld_rlx(&Q.npages) >= Q.max_pages) return false;
-    uint32_t id = NO_PAGE;
-    if (P.serving) {
-        // a recycled page first, then a fresh one
-        Ctl *c = P.ctl;
-        if (ld_rlx(&c->pf_avail) > 0) {
-            if (atom_add(&c->pf_avail, -1) > 0) {
-                uint32_t pos = atom_add(&c->pf_head, 1u);
-                FSlot *fs = &P.pfree_ring[pos & (P.pfcap - 1)];
-                if (!wait_eq(&fs->seq, pos + 1, c, 15)) return false;
-                id = fs->id;
-                st_rel(&fs->seq, pos + P.pfcap);
-            } else {
-                atom_add(&c->pf_avail, 1);
-            }
-        }
-        if (id == NO_PAGE) {
-            if (ld_rlx(&c->srv_page_bump) >= P.npages) return false;
-            id = atom_add(&c->srv_page_bump, 1u);
-            if (id >= P.npages) return false;
-        }
-        uint32_t idx = atom_add(&Q.npages, 1u);
-        if (idx >= Q.max_pages) {
-            // over budget: give the page back
-            uint32_t pos = atom_add(&c->pf_tail, 1u);
-            FSlot *fs = &P.pfree_ring[pos & (P.pfcap - 1)];
-            if (wait_eq(&fs->seq, pos, c, 16)) {
-                fs->id = id;
-                st_rel(&fs->seq, pos + 1);
-                atom_add(&c->pf_avail, 1);
-            }
-            return false;
-        }
-        st_rlx(&P.slot_pages[(size_t)q * P.max_slot_pages + idx], id);
-    } else {
-        if (ld_rlx(&P.ctl->page_bump) >= P.npages) return false;
-        if (atom_add(&Q.npages, 1u) >= Q.max_pages) return false;
-        id = atom_add(&P.ctl->page_bump, 1u);
-        if (id >= P.npages) return false;
-    }
+    if (ld_rlx(&P.ctl->page_bump) >= P.npages) return false;
+    if (atom_add(&Q.npages, 1u) >= Q.max_pages) return false;
+    const uint32_t id = atom_add(&P.ctl->page_bump, 1u);
+    if (id >= P.npages) return false;
     st_rlx(&P.page_owner[id], q);
     w.page = id;
     w.fill = w.fill0 = 0;
@@ -647,11 +620,107 @@ __device__ bool tab_acquire(const KParams &P, WaveTab &w, uint32_t q, uint32_t c
     return true;
 }
 
+// Lane 0, serving: a page of the arena, a recycled one first; NO_PAGE when
+// the arena is exhausted.
+__device__ uint32_t srv_page_alloc(const KParams &P, unsigned long long *stats) {
+    Ctl *c = P.ctl;
+    uint32_t id = NO_PAGE;
+    if (ld_rlx(&c->pf_avail) > 0) {
+        if (atom_add(&c->pf_avail, -1) > 0) {
+            const uint32_t pos = atom_add(&c->pf_head, 1u);
+            FSlot *fs = &P.pfree_ring[pos & (P.pfcap - 1)];
+            if (!wait_eq(&fs->seq, pos + 1, c, 15)) return NO_PAGE;
+            id = fs->id;
+            st_rel(&fs->seq, pos + P.pfcap);
+        } else {
+            atom_add(&c->pf_avail, 1);
+        }
+    }
+    if (id == NO_PAGE && ld_rlx(&c->srv_page_bump) < P.npages) {
+        id = atom_add(&c->srv_page_bump, 1u);
+        if (id >= P.npages) id = NO_PAGE;
+    }
+    if (id != NO_PAGE) stats[ST_PAGES]++;
+    return id;
+}
+
+// Lane 0, serving: the page of slot entry k once its owner has resolved it
+// (a page id or FAILED_PAGE).
+__device__ uint32_t wait_page(const KParams &P, uint32_t q, uint32_t k) {
+    const uint32_t *e = &P.slot_pages[(size_t)q * P.max_slot_pages + k];
+    uint32_t it = 0, v;
+    while ((v = __hip_atomic_load(e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) == NO_PAGE)
+        if (!spin_ok(P.ctl, it, 18, v, k)) return FAILED_PAGE;
+    return v;
+}
+
+// All lanes, serving. A query's table is dense: a wave claims cnt rows at
+// the query's cursor, so row r lands in slot page r / page_rows at
+// r % page_rows, and the query holds ceil(rows / page_rows) pages however
+// many waves emit its rows. The wave whose rows include the first row of a
+// page allocates it; waves with rows further into that page wait for the
+// slot entry. When the arena runs out the query's table ends at the start
+// of the page that failed (trunc_at): every row below trunc_at is stored.
+template <class F>
+__device__ void emit_rows_dense(const KParams &P, uint32_t q, uint32_t n, bool ok, uint32_t rank, uint32_t cnt,
+                                unsigned long long *stats, F write_row) {
+    const uint32_t lane = lane_id();
+    DQuery &Q = P.qs[q];
+    const uint32_t pr = Q.page_rows;
+    uint32_t off = 0, keep = 0, k0 = 0, pA = FAILED_PAGE, pB = FAILED_PAGE;
+    if (lane == 0) {
+        const unsigned long long r0 = atom_add(&Q.cursor, (unsigned long long)cnt);
+        // the page whose first row is ours, if any: resolved whatever we keep,
+        // since waves that read an older trunc_at may wait for it
+        const unsigned long long kf = (r0 + pr - 1) / pr;
+        if (kf * pr < r0 + cnt && kf < Q.max_pages) {
+            uint32_t id = FAILED_PAGE;
+            if (kf * pr < ld_rlx(&Q.trunc_at)) {
+                id = srv_page_alloc(P, stats);
+                if (id == NO_PAGE) {
+                    id = FAILED_PAGE;
+                    __hip_atomic_fetch_min(&Q.trunc_at, kf * pr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+            }
+            __hip_atomic_fetch_max(&Q.npages, (uint32_t)kf + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            st_rel(&P.slot_pages[(size_t)q * P.max_slot_pages + kf], id);
+        }
+        // trunc_at only falls: rows dropped here lie at or above its final value
+        const unsigned long long lim = ld_rlx(&Q.trunc_at);
+        keep = r0 >= lim ? 0u : (uint32_t)min((unsigned long long)cnt, lim - r0);
+        if (keep < cnt)
+            __hip_atomic_fetch_or(&Q.flags, FM_RESULT_TRUNCATED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (keep) {
+            k0 = (uint32_t)(r0 / pr);
+            off = (uint32_t)(r0 - (unsigned long long)k0 * pr);
+            const uint32_t k1 = (uint32_t)((r0 + keep - 1) / pr);
+            pA = wait_page(P, q, k0);
+            pB = k1 != k0 ? wait_page(P, q, k1) : pA;
+        }
+    }
+    keep = bcast(keep, 0);
+    if (keep == 0) return;
+    off = bcast(off, 0);
+    pA = bcast(pA, 0);
+    pB = bcast(pB, 0);
+    if (ok && rank < keep) {
+        // rows fit a page or straddle two
+        const uint32_t at = off + rank;
+        const uint32_t page = at < pr ? pA : pB;
+        const uint32_t row = at < pr ? at : at - pr;
+        if (page < P.npages) write_row(P.pages + (size_t)page * P.page_words + (size_t)row * n);
+    }
+}
+
 // All lanes. Appends the rows of the lanes with ok set (rank = their order)
 // to query q's table; write_row(dst) stores one row of n words at dst.
 template <class F>
 __device__ void emit_rows(const KParams &P, WaveTab &wt, uint32_t q, uint32_t n, bool ok, uint32_t rank, uint32_t cnt,
                           unsigned long long *stats, F write_row) {
+    if (P.serving) {
+        emit_rows_dense(P, q, n, ok, rank, cnt, stats, write_row);
+        return;
+    }
     const uint32_t lane = lane_id();
     uint32_t pA = NO_PAGE, fA = 0, nA = 0, pB = NO_PAGE, fB = 0;
     if (lane == 0) {
@@ -719,9 +788,9 @@ __device__ void flush_run(const KParams &P, WaveOut &o, uint32_t q, uint32_t D, 
     uint32_t rows = (o.used - o.run_begin) / W;
     if (rows == 0) return;
     if (lane_id() == 0) {
-        DQuery &Q = P.qs[q];
+        DQuery &Q = P.qs[qslot(q)];
         unsigned long long stored = atom_add(&Q.stored, (unsigned long long)rows) + rows;
-        if (Q.max_partials && stored > Q.max_partials) query_fail(P, q, FM_ERR_COUNT_LIMIT);
+        if (Q.max_partials && stored > Q.max_partials) query_fail(P, qslot(q), FM_ERR_COUNT_LIMIT);
     }
     uint32_t ntasks = (rows + per - 1) / per;
     const uint32_t lane = lane_id();
@@ -748,8 +817,9 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
                              unsigned long long *stats, unsigned long long &starve_since,
                              unsigned long long deadline) {
     const uint32_t lane = lane_id();
-    const DPlan &pl = P.plans[t.q];
-    DQuery &Q = P.qs[t.q];
+    const uint32_t qi = qslot(t.q);
+    const DPlan &pl = P.plans[qi];
+    DQuery &Q = P.qs[qi];
     if (ld_rlx(&Q.status) != 0) return;
     const uint32_t n = pl.n;
     const uint32_t d = t.depth;
@@ -778,6 +848,8 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
                     dg[6] = t.last_end;
                     dg[7] = n;
                     dg[8] = pl.nroots;
+                    dg[9] = ld_rlx(&Q.pad);
+                    dg[10] = (uint32_t)__hip_atomic_load(&Q.qpending, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 }
                 st_rlx(&P.ctl->abort, 1u);
             }
@@ -804,7 +876,7 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
             cnt_acc += cnt;
             if (P.instrument && lane == 0) s.inst[4 + 0] += cnt;
             if (P.table && cnt)
-                emit_rows(P, wt, t.q, 1, ok, rank, cnt, stats, [&](uint32_t *dst) { dst[0] = v; });
+                emit_rows(P, wt, qi, 1, ok, rank, cnt, stats, [&](uint32_t *dst) { dst[0] = v; });
         }
         if (lane == 0) {
             atom_add(&Q.count, cnt_acc);
@@ -1049,14 +1121,14 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
                 if (P.table) {
                     const uint32_t *prow = &par[pi * W];
                     if (naut == 1) {
-                        emit_rows(P, wt, t.q, n, ok, rank, cnt, stats, [&](uint32_t *dst) {
+                        emit_rows(P, wt, qi, n, ok, rank, cnt, stats, [&](uint32_t *dst) {
                             for (uint32_t j = 0; j < W; j++) dst[pl.order[j]] = prow[j];
                             dst[pl.order[D]] = x;
                         });
                     } else {
                         // one row per automorphism p: column u holds m(p(u))
                         for (uint32_t k = 0; k < naut; k++)
-                            emit_rows(P, wt, t.q, n, ok, rank, cnt, stats, [&](uint32_t *dst) {
+                            emit_rows(P, wt, qi, n, ok, rank, cnt, stats, [&](uint32_t *dst) {
                                 for (uint32_t u = 0; u < n; u++) {
                                     const uint32_t dd = pl.autcol[k][u];
                                     dst[u] = dd == D ? x : prow[dd];
@@ -1079,7 +1151,7 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
         if (final_level && lane == 0 && cnt_acc >= (1ull << 20)) {
             unsigned long long now_cnt = atom_add(&Q.count, cnt_acc) + cnt_acc;
             cnt_acc = 0;
-            if (Q.max_count && now_cnt > Q.max_count) query_fail(P, t.q, FM_ERR_COUNT_LIMIT);
+            if (Q.max_count && now_cnt > Q.max_count) query_fail(P, qi, FM_ERR_COUNT_LIMIT);
         }
         __syncthreads();
     }
@@ -1088,7 +1160,7 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
     if (final_level && P.table && lane == 0) tab_task_end(P, wt);
     if (lane == 0 && cnt_acc) {
         unsigned long long now_cnt = atom_add(&Q.count, cnt_acc) + cnt_acc;
-        if (Q.max_count && now_cnt > Q.max_count) query_fail(P, t.q, FM_ERR_COUNT_LIMIT);
+        if (Q.max_count && now_cnt > Q.max_count) query_fail(P, qi, FM_ERR_COUNT_LIMIT);
     }
 }
 
@@ -1106,7 +1178,10 @@ __device__ __forceinline__ void st_sys(uint32_t *p, uint32_t v) {
 __device__ void complete_query(const KParams &P, uint32_t q) {
     DQuery &Q = P.qs[q];
     SrvDone *d = &P.srv_done[q];
-    unsigned long long count = ld_rlx(&Q.count), rows = ld_rlx(&Q.rows);
+    const unsigned long long count = ld_rlx(&Q.count);
+    // the table: rows [0, rows) in slot pages 0, 1, ... (dense)
+    const unsigned long long rows = min(ld_rlx(&Q.cursor), ld_rlx(&Q.trunc_at));
+    const uint32_t pages = Q.page_rows ? (uint32_t)((rows + Q.page_rows - 1) / Q.page_rows) : 0u;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");  // table rows visible to the host
     st_sys(&d->count_lo, (uint32_t)count);
     st_sys(&d->count_hi, (uint32_t)(count >> 32));
@@ -1114,7 +1189,7 @@ __device__ void complete_query(const KParams &P, uint32_t q) {
     st_sys(&d->rows_hi, (uint32_t)(rows >> 32));
     st_sys(&d->status, ld_rlx(&Q.status));
     st_sys(&d->flags, __hip_atomic_load(&Q.flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    st_sys(&d->npages, min(ld_rlx(&Q.npages), Q.max_pages));
+    st_sys(&d->npages, pages);
     __hip_atomic_store(&d->ticket, Q.pad, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     atom_add(&P.ctl->n_active, -1);
     st_sys(&P.srv->live_banks, (uint32_t)max(0, ld_rlx(&P.ctl->live_banks)));
@@ -1123,8 +1198,11 @@ __device__ void complete_query(const KParams &P, uint32_t q) {
 // Lane 0: claims the next submission; returns its slot or NO_PAGE.
 __device__ uint32_t claim_submission(const KParams &P) {
     Ctl *c = P.ctl;
+    // The two relaxed loads may complete in either order, so the tail read
+    // can be older than the head: the head may already be past it. Only a
+    // head strictly behind the tail names a written entry.
     uint32_t h = ld_rlx(&c->sub_head);
-    if (h == ld_sys(&P.srv->sub_tail)) return NO_PAGE;
+    if ((int32_t)(ld_sys(&P.srv->sub_tail) - h) <= 0) return NO_PAGE;
     if (!cas_u32(&c->sub_head, h, h + 1)) return NO_PAGE;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
     return ld_sys(&P.srv->sub[h % SUB_CAP].slot);
@@ -1138,9 +1216,12 @@ __device__ void admit(const KParams &P, uint32_t slot, unsigned long long *stats
     DQuery &Q = P.qs[slot];
     Ctl *c = P.ctl;
     // pages of the previous query in this slot go back to the free ring
+    // (each entry is taken with an exchange, so no page is freed twice)
     const uint32_t old = min(Q.npages, Q.max_pages);
     for (uint32_t i = lane; i < old; i += WAVE) {
-        uint32_t id = P.slot_pages[(size_t)slot * P.max_slot_pages + i];
+        const uint32_t id = __hip_atomic_exchange(&P.slot_pages[(size_t)slot * P.max_slot_pages + i], NO_PAGE,
+                                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (id == NO_PAGE || id >= P.npages) continue;
         uint32_t pos = atom_add(&c->pf_tail, 1u);
         FSlot *fs = &P.pfree_ring[pos & (P.pfcap - 1)];
         if (wait_eq(&fs->seq, pos, c, 17)) {
@@ -1160,21 +1241,29 @@ __device__ void admit(const KParams &P, uint32_t slot, unsigned long long *stats
                      ld_sys((const uint32_t *)&qi->max_rows);
         z.page_rows = ld_sys(&qi->page_rows);
         z.max_pages = ld_sys(&qi->max_pages);
+        z.trunc_at = z.max_rows;
         z.pad = ld_sys(&qi->ticket);  // ticket, echoed in the completion
         z.qpending = 1;               // held until every root chunk is queued
         Q = z;
         atom_add(&c->n_active, 1);
     }
     __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    const DPlan &pl = P.plans[slot];
-    const uint32_t n = pl.n;
-    const uint32_t chunks = (pl.nroots + P.root_chunk - 1) / P.root_chunk;
+    // publish the plan and the query state; also drop this CU's L1 copies of
+    // the slot's old plan before any task of the new query runs here
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+    // The plan's size fields come from the pinned source, not from the copy
+    // just stored: the copy's lines may sit stale in this CU's vector L1
+    // from the slot's previous query.
+    const uint32_t n = ld_sys(&P.srv_plans[slot].n);
+    const uint32_t nroots = ld_sys(&P.srv_plans[slot].nroots);
+    const uint32_t chunks = (nroots + P.root_chunk - 1) / P.root_chunk;
     const uint32_t lvl = max(n, 2u) - 2;
     for (uint32_t c0 = 0; c0 < chunks; c0 += WAVE) {
         const uint32_t m = min(WAVE, chunks - c0);
         const uint32_t first = (c0 + lane) * P.root_chunk;
-        TaskDesc t{slot, 0, ROOT_BANK, first, lane < m ? min(P.root_chunk, pl.nroots - first) : 0, 0, ALL_CANDIDATES};
+        const uint32_t tag = ld_sys(&P.srv_qinit[slot].ticket) & 0xFFu;
+        TaskDesc t{slot | tag << 24, 0, ROOT_BANK, first, lane < m ? min(P.root_chunk, nroots - first) : 0, 0,
+                   ALL_CANDIDATES};
         push_tasks(P, lvl, m, t, ROOT_BANK, stats);
     }
     if (lane == 0) {
@@ -1274,11 +1363,17 @@ __global__ __launch_bounds__(WAVE) void fm_match_kernel(KParams P) {
         if (P.instrument && lane == 0)
             for (uint32_t i = 0; i < 4 + MAXN; i++) s.inst[i] = 0;
         const unsigned long long tb = wall_clock64();
-        process_task(P, t, s, o, wt, stats, starve_since, deadline);
+        // serving: a task whose tag is not the slot's current query belongs
+        // to a query that has already completed; it is dropped (counted)
+        uint32_t stale = 0;
+        if (P.serving && lane == 0) stale = (t.q >> 24) != (ld_rlx(&P.qs[qslot(t.q)].pad) & 0xFFu);
+        stale = bcast(stale, 0);
+        if (!stale) process_task(P, t, s, o, wt, stats, starve_since, deadline);
+        else if (lane == 0) stats[ST_STALE]++;
         __syncthreads();
         if (lane == 0) stats[ST_BUSY] += wall_clock64() - tb;
         if (P.instrument && lane == 0) {
-            DQuery &Q = P.qs[t.q];
+            DQuery &Q = P.qs[qslot(t.q)];
             unsigned long long *dst[4] = {&Q.row_words, &Q.lookups, &Q.candidates, &Q.probes};
             for (uint32_t i = 0; i < 4; i++)
                 if (s.inst[i]) atom_add(dst[i], s.inst[i]);
@@ -1288,11 +1383,12 @@ __global__ __launch_bounds__(WAVE) void fm_match_kernel(KParams P) {
         if (lane == 0) {
             if (t.bank != ROOT_BANK && t.bank < P.nbanks) bank_release(P, t.bank);
             atom_add(&P.ctl->pending, -1ll);
-            if (P.serving) {
+            if (P.serving && !stale) {
                 // rows of this task reach memory before the pending decrement
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-                if (__hip_atomic_fetch_add(&P.qs[t.q].qpending, -1ll, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == 1)
-                    complete_query(P, t.q);
+                if (__hip_atomic_fetch_add(&P.qs[qslot(t.q)].qpending, -1ll, __ATOMIC_ACQ_REL,
+                                           __HIP_MEMORY_SCOPE_AGENT) == 1)
+                    complete_query(P, qslot(t.q));
             }
         }
     }
@@ -1986,7 +2082,7 @@ int fm_match_batch(fm_engine *e, const fm_graph *g, const fm_query *const *qs, u
     fm_match_opts o{};
     if (opts) o = *opts;
     if (o.timeout_ms == 0) o.timeout_ms = 120000;
-    if (o.task_rows == 0) o.task_rows = SubMaxMulSpill;
+    if (o.task_rows == 0) o.task_rows = nq == 1 ? SubMaxMulSpill : FM_BATCH_TASK_ROWS;
     if (o.root_chunk == 0) o.root_chunk = TooLargeThreshold;
     HIP_TRY(hipSetDevice(e->device));
     if (e->dirty) {
@@ -2236,13 +2332,13 @@ struct fm_server {
     hipStream_t copy_stream = nullptr;
     hipEvent_t kernel_done = nullptr;
     bool launched = false;
+    int failed = 0;  // status of the launch that stopped the server (0: running)
     std::chrono::steady_clock::time_point t0;
     std::vector<uint32_t> free_slots;
     std::vector<uint32_t> slot_ticket;   // ticket in the slot (0 = none)
     std::vector<uint8_t> slot_state;     // 0 free, 1 in flight, 2 done
     std::vector<double> slot_submit_ms;
     std::vector<uint32_t> slot_n;
-    std::vector<uint32_t> page_fill_cache;
     uint32_t next_ticket = 1;
     uint32_t sub_tail = 0;
     fm_server_stats st{};
@@ -2327,17 +2423,33 @@ static bool server_has_work(const fm_server *s) {
 // Relaunches the persistent kernel when it has gone idle or retired and
 // work waits.
 static int server_pump(fm_server *s) {
+    // After an aborted launch the device queues, pools and submission head
+    // are no longer trustworthy: the server stays stopped.
+    if (s->failed) return fail(s->failed, "the server stopped after an earlier error (status %d)", s->failed);
     if (s->launched) {
         hipError_t q = hipEventQuery(s->kernel_done);
         if (q == hipErrorNotReady) return FM_OK;
-        if (q != hipSuccess) return fail(FM_ERR_DEVICE, "serving kernel failed: %s", hipGetErrorString(q));
+        if (q != hipSuccess) {
+            s->launched = false;
+            s->failed = FM_ERR_DEVICE;
+            return fail(FM_ERR_DEVICE, "serving kernel failed: %s", hipGetErrorString(q));
+        }
         s->launched = false;
         Ctl back{};
         HIP_TRY(hipMemcpy(&back, s->e->ctl, CTL_RESET_BYTES, hipMemcpyDeviceToHost));
+        s->st.dropped_tasks += back.stats[ST_STALE];
         if (back.error) {
             s->e->dirty = true;
+            s->failed = (int)back.error;
             if (back.error == FM_ERR_TIMEOUT)
                 return fail(FM_ERR_TIMEOUT, "serving kernel passed its deadline (%s)", timeout_site(back).c_str());
+            if (back.error == FM_ERR_INTERNAL)
+                return fail(FM_ERR_INTERNAL,
+                            "serving kernel caught an inconsistent task q=%u tag=%u depth=%u bank=%u begin=%u "
+                            "count=%u skip=%u last_end=%u (n=%u nroots=%u; slot ticket %u, pending %d)",
+                            back.diag[0] & 0xFFFFFF, back.diag[0] >> 24, back.diag[1], back.diag[2], back.diag[3],
+                            back.diag[4], back.diag[5], back.diag[6], back.diag[7], back.diag[8], back.diag[9],
+                            (int)back.diag[10]);
             return fail((int)back.error, "serving kernel aborted with status %u", back.error);
         }
     }
@@ -2359,7 +2471,7 @@ int fm_server_create(fm_engine *e, const fm_graph *g, const fm_server_config *cf
     if (!s->cfg.slot_pages) s->cfg.slot_pages = 256;
     if (!s->cfg.idle_ms) s->cfg.idle_ms = 20;
     if (!s->cfg.root_chunk) s->cfg.root_chunk = TooLargeThreshold;
-    if (!s->cfg.task_rows) s->cfg.task_rows = SubMaxMulSpill;
+    if (!s->cfg.task_rows) s->cfg.task_rows = FM_BATCH_TASK_ROWS;
     if (!s->cfg.bank_budget) s->cfg.bank_budget = e->nbanks;
     if (!s->cfg.max_launch_ms) s->cfg.max_launch_ms = 120000;
     if (!s->cfg.slice_ms) s->cfg.slice_ms = 2000;
@@ -2389,7 +2501,9 @@ int fm_server_create(fm_engine *e, const fm_graph *g, const fm_server_config *cf
         hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&s->kernel_done, hipEventDisableTiming) != hipSuccess)
         return fail_out(fail(FM_ERR_OUT_OF_MEMORY, "serving device buffers"));
-    // slot state on the device starts zeroed (no pages to recycle)
+    // slot state on the device starts zeroed (no pages to recycle); page
+    // lists start empty
+    HIP_TRY(hipMemset(s->slot_pages, 0xFF, (size_t)S * s->cfg.slot_pages * 4));
     if (S > e->q_cap) {
         (void)hipFree(e->d_plans);
         (void)hipFree(e->d_qs);
@@ -2504,19 +2618,17 @@ int fm_server_copy_table(fm_server *s, uint32_t ticket, uint32_t *dst, uint64_t 
     if (rows == 0 || npg == 0) return FM_OK;
     if (!dst) return fail(FM_ERR_INVALID_ARG, "null destination");
     HIP_TRY(hipSetDevice(e->device));
+    // dense table: row r in slot page r / page_rows
+    const uint64_t pr = s->hqinit[slot].page_rows;
     std::vector<uint32_t> ids(npg);
     HIP_TRY(hipMemcpyAsync(ids.data(), s->slot_pages + (size_t)slot * s->cfg.slot_pages, npg * 4ull,
                            hipMemcpyDeviceToHost, s->copy_stream));
-    s->page_fill_cache.resize(e->npages);
-    HIP_TRY(hipMemcpyAsync(s->page_fill_cache.data(), e->page_fill, e->npages * 4ull, hipMemcpyDeviceToHost,
-                           s->copy_stream));
     HIP_TRY(hipStreamSynchronize(s->copy_stream));
     uint64_t done = 0;
     for (uint32_t k = 0; k < npg && done < rows; k++) {
-        uint32_t p = ids[k];
-        if (p >= e->npages) continue;
-        uint64_t take = std::min<uint64_t>(s->page_fill_cache[p], rows - done);
-        if (!take) continue;
+        const uint32_t p = ids[k];
+        if (p >= e->npages) return fail(FM_ERR_INTERNAL, "ticket %u: table page %u is missing", ticket, k);
+        const uint64_t take = std::min<uint64_t>(pr, rows - done);
         HIP_TRY(hipMemcpyAsync(dst + done * n, e->pages + (size_t)p * e->page_words, take * n * 4ull,
                                hipMemcpyDeviceToHost, s->copy_stream));
         done += take;
@@ -2562,7 +2674,7 @@ int fm_server_stats_get(fm_server *s, fm_server_stats *out) {
 void fm_server_destroy(fm_server *s) {
     if (!s) return;
     if (s->sh) __atomic_store_n(&s->sh->shutdown, 1u, __ATOMIC_RELEASE);
-    if (s->sh && s->e) {
+    if (s->sh && s->e && !s->failed) {
         // give in-flight queries 10 s to drain (relaunching retired
         // launches), then stop the launch
         auto t_stop = std::chrono::steady_clock::now() + std::chrono::seconds(10);

@@ -369,6 +369,55 @@ def test_small_tasks_fill_pages_instead_of_wasting_them():
         r.close()
 
 
+def _rows_subset(rows, table):
+    """Every row of `rows` is a row of `table` and no row repeats."""
+    v = np.dtype((np.void, rows.dtype.itemsize * rows.shape[1]))
+    a = np.ascontiguousarray(rows).view(v).ravel()
+    b = np.ascontiguousarray(table).view(v).ravel()
+    return len(np.unique(a)) == len(a) and bool(np.isin(a, b).all())
+
+
+def _serve_one(srv, q, max_rows=0):
+    import time
+    t = srv.submit(q, max_rows)
+    assert t is not None
+    t_end = time.time() + 60
+    while time.time() < t_end:
+        for c in srv.poll():
+            assert c.ticket == t
+            tab = srv.table(c.ticket, c.rows, c.vertices)
+            srv.release(c.ticket)
+            return c, tab
+        time.sleep(0.001)
+    raise AssertionError("query did not complete")
+
+
+def test_serving_tables_are_dense():
+    """A serving table holds ceil(rows / page_rows) pages however many waves
+    emit its rows: 511,472 rows of 16 bytes (125 pages of 64 KiB) fit a
+    130-page arena with every wave of the chip emitting. A row budget or an
+    arena that runs out stores a prefix of whole pages (each row a distinct
+    embedding) and flags the result truncated."""
+    g = fm.Graph.erdos_renyi(1000, 4000, 1, seed=77).upload()
+    og = _oracle_graph(g)
+    q = fm.Query.from_edges(4, [(0, 1, 0), (1, 0, 0), (1, 2, 0), (2, 1, 0), (2, 3, 0), (3, 2, 0)])
+    ref = orc.match(og, 4, q.edges())
+    assert ref.count == 511472
+    page_rows = (1 << 16) // 16
+    for pages, max_rows, want_rows in [(130, 0, ref.count), (130, 10000, 10000), (64, 0, 64 * page_rows)]:
+        with fm.Engine(soln_prealloc=1 << 28, output_prealloc=pages << 16, output_page_bytes=1 << 16) as e:
+            with fm.Server(e, g, slots=4, slot_pages=256, max_launch_ms=60000) as srv:
+                for _ in range(2):  # the second run reuses the recycled pages
+                    c, tab = _serve_one(srv, q, max_rows)
+                    assert c.status == 0 and c.count == ref.count, (pages, max_rows)
+                    assert c.rows == want_rows, (pages, max_rows, c.rows)
+                    assert bool(c.flags & fm.FM_RESULT_TRUNCATED) == (want_rows < ref.count)
+                    if want_rows == ref.count:
+                        assert np.array_equal(tab, ref.table)
+                    else:
+                        assert tab.shape == (want_rows, 4) and _rows_subset(tab, ref.table)
+
+
 @pytest.mark.parametrize("table", [True, False])
 def test_symmetry_breaking_gives_the_full_table(engine, table):
     """Symmetric patterns (including C5, C6 and the bowtie, where the
