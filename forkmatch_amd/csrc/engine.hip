@@ -1436,7 +1436,10 @@ constexpr uint32_t IB = 256;              // threads per block
 // Merge-path elements per thread. Odd: the threads' stretches start about
 // XVT / 2 words apart in each LDS slice, and an odd stride spreads a wave's
 // accesses over all LDS banks (an even one folds them onto a few).
-constexpr uint32_t XVT = 15;
+#ifndef FM_ISECT_VT
+#define FM_ISECT_VT 15
+#endif
+constexpr uint32_t XVT = FM_ISECT_VT;
 constexpr uint32_t XT = IB * XVT;         // merge-path elements per merge tile
 constexpr uint32_t ISKEW = 32;            // longer >= ISKEW x shorter: skew tiles
 
@@ -1535,9 +1538,12 @@ __global__ __launch_bounds__(IB) void fm_isect_cut_kernel(const uint32_t *__rest
 }
 
 // Persistent blocks claim whole pairs (largest first, `order`) and stream
-// their tiles; the next tile's elements are loaded into registers while the
-// current one is merged, and the matches leave through LDS in coalesced
-// stores at the pair's running offset.
+// their tiles. A merge tile's elements are loaded into registers while the
+// previous tile is merged and stored to LDS at the end of that iteration, so
+// no register is carried from one iteration to the next (a loop-carried
+// prefetch register makes the compiler wait for the loads at the loop join,
+// exposing their latency on every tile). The matches leave through LDS in
+// coalesced stores at the pair's running offset.
 __global__ __launch_bounds__(IB) void fm_isect_kernel(const uint32_t *__restrict__ data, const uint64_t *a_off,
                                                       const uint32_t *a_len, const uint64_t *b_off,
                                                       const uint32_t *b_len, uint32_t *counts,
@@ -1567,7 +1573,6 @@ __global__ __launch_bounds__(IB) void fm_isect_kernel(const uint32_t *__restrict
         const uint32_t p = order[w];
         const IsectPair q = isect_pair(data, a_off, a_len, b_off, b_len, p);
         const uint32_t tb = tile_begin[p], te = tile_begin[p + 1];
-        const bool skew = isect_skewed(q.na, q.nb);
         const uint32_t total_path = q.na + q.nb;
         uint32_t *const o = out ? out + out_off[p] : nullptr;
         uint32_t cbase = tb;  // tile of s_cut[0]
@@ -1581,81 +1586,101 @@ __global__ __launch_bounds__(IB) void fm_isect_kernel(const uint32_t *__restrict
         };
         fill_cuts(tb);
         __syncthreads();
-        uint32_t reg[XVT + 1];
-        uint32_t rc0 = 0, rc1 = 0;  // cuts of the tile in reg
-        // t - cbase < XCUT: the caller refills s_cut before a tile past it
-        auto load_tile = [&](uint32_t t) {
-            rc0 = s_cut[t - cbase];
-            if (skew) {
-                rc1 = s_cut2[t - cbase];
-                const uint32_t first = (t - tb) * IB;
-                reg[0] = tid < min(IB, q.na - first) ? q.A[first + tid] : 0u;
-                return;
-            }
-            rc1 = s_cut[t - cbase + 1];
-            const uint32_t d0 = (t - tb) * XT, d1 = min(d0 + XT, total_path);
-            const uint32_t b0 = d0 - rc0, b1 = d1 - rc1;
-            const uint32_t ta = rc1 - rc0, tbx = (b1 - b0) + (b1 < q.nb ? 1u : 0u);
-#pragma unroll
-            for (uint32_t x = 0; x <= XVT; x++) {
-                const uint32_t e = tid + x * IB;
-                reg[x] = e < ta ? q.A[rc0 + e] : (e < ta + tbx ? q.B[b0 + e - ta] : 0u);
-            }
-        };
-        load_tile(tb);
         uint32_t found = 0;
-        for (uint32_t t = tb; t < te; t++) {
-            const uint32_t c0 = rc0, c1 = rc1;
-            uint32_t c = 0, x_skew = 0;
-            uint32_t ta = 0, tbn = 0, tbx = 0, i0 = 0, j0 = 0, steps = 0;
-            if (skew) {
-                const uint32_t first = (t - tb) * IB, cn = min(IB, q.na - first);
-                x_skew = reg[0];
-                if (t + 1 < te && t + 1 - cbase < XCUT) load_tile(t + 1);
-                if (tid < cn) {
-                    const uint32_t at = lower_bound_u32(q.B, c0, c1, x_skew);
-                    c = at < c1 && q.B[at] == x_skew ? 1u : 0u;
+        if (isect_skewed(q.na, q.nb)) {
+            // skew tiles: each thread looks one A element up in the tile's
+            // range of B
+            for (uint32_t t = tb; t < te; t++) {
+                if (t - cbase == XCUT) {
+                    fill_cuts(t);
+                    __syncthreads();
                 }
-            } else {
-                const uint32_t d0 = (t - tb) * XT, d1 = min(d0 + XT, total_path);
-                const uint32_t b0 = d0 - c0, b1 = d1 - c1;
-                ta = c1 - c0;
-                tbn = b1 - b0;
-                tbx = tbn + (b1 < q.nb ? 1u : 0u);
+                const uint32_t c0 = s_cut[t - cbase], c1 = s_cut2[t - cbase];
+                const uint32_t first = (t - tb) * IB, cn = min(IB, q.na - first);
+                uint32_t c = 0, x = 0;
+                if (tid < cn) {
+                    x = q.A[first + tid];
+                    const uint32_t at = lower_bound_u32(q.B, c0, c1, x);
+                    c = at < c1 && q.B[at] == x ? 1u : 0u;
+                }
+                uint32_t tot;
+                const uint32_t at = block_excl_scan(c, s_wave, tot);  // its barriers order s_out
+                if (o) {
+                    if (c) s_out[at] = x;
+                    __syncthreads();
+                    for (uint32_t i = tid; i < tot; i += IB) o[found + i] = s_out[i];
+                }
+                found += tot;
+                __syncthreads();  // s_out and s_cut are rewritten by the next tile
+            }
+        } else {
+            uint32_t reg[XVT + 1];
+            uint32_t nc0 = 0, nc1 = 0;  // cuts of the tile in reg
+            // Tile t's elements into reg: every load is unconditional and in
+            // bounds (lanes past the tile read B's last element; the store
+            // below skips them).
+            auto load_tile = [&](uint32_t t) {
+                nc0 = s_cut[t - cbase];
+                nc1 = s_cut[t - cbase + 1];
+                const uint32_t d0 = (t - tb) * XT;
+                const uint32_t b0 = d0 - nc0, ta = nc1 - nc0;
 #pragma unroll
                 for (uint32_t x = 0; x <= XVT; x++) {
                     const uint32_t e = tid + x * IB;
-                    if (e < ta + tbx) sAB[e] = reg[x];
-                }
-                __syncthreads();
-                if (t + 1 < te && t + 1 - cbase < XCUT) load_tile(t + 1);  // in flight during the merge
-                const uint32_t dl = min(tid * XVT, ta + tbn);
-                steps = min(XVT, ta + tbn - dl);
-                i0 = merge_split(sAB, ta, sAB + ta, tbn, dl);
-                j0 = dl - i0;
-            }
-            // Merges the stretch; emit(x) for each match. Consuming A[i], an
-            // equal B element is the next one on B's side. The current A and
-            // B elements stay in registers: one LDS read per step.
-            auto merge_stretch = [&](auto emit) {
-                const uint32_t *sA = sAB, *sB = sAB + ta;
-                uint32_t ii = i0, jj = j0;
-                uint32_t av = ii < ta ? sA[ii] : 0u, bv = jj < tbx ? sB[jj] : 0u;
-#pragma unroll
-                for (uint32_t st = 0; st < XVT; st++) {
-                    if (st < steps) {
-                        if (ii < ta && (jj >= tbn || av <= bv)) {
-                            if (jj < tbx && bv == av) emit(av);
-                            ii++;
-                            if (ii < ta) av = sA[ii];
-                        } else {
-                            jj++;
-                            if (jj < tbx) bv = sB[jj];
-                        }
-                    }
+                    const uint32_t *src = e < ta ? q.A + (nc0 + e) : q.B + min(b0 + (e - ta), q.nb - 1);
+                    reg[x] = *src;
                 }
             };
-            if (!skew) {
+            // reg (tile t, cuts c0 and c1) into sAB
+            auto store_tile = [&](uint32_t t, uint32_t c0, uint32_t c1) {
+                const uint32_t d0 = (t - tb) * XT, d1 = min(d0 + XT, total_path);
+                const uint32_t b0 = d0 - c0, b1 = d1 - c1;
+                const uint32_t n = (c1 - c0) + (b1 - b0) + (b1 < q.nb ? 1u : 0u);
+#pragma unroll
+                for (uint32_t x = 0; x <= XVT; x++) {
+                    const uint32_t e = tid + x * IB;
+                    if (e < n) sAB[e] = reg[x];
+                }
+            };
+            load_tile(tb);
+            uint32_t c0 = nc0, c1 = nc1;
+            store_tile(tb, c0, c1);
+            __syncthreads();
+            for (uint32_t t = tb; t < te; t++) {
+                // t + 1 - cbase <= XCUT: the window holds the next tile's
+                // cuts unless it starts past it (then it is refilled below)
+                const bool has_next = t + 1 < te && t + 1 - cbase < XCUT;
+                load_tile(has_next ? t + 1 : t);  // in flight during the merge
+                const uint32_t d0 = (t - tb) * XT, d1 = min(d0 + XT, total_path);
+                const uint32_t b0 = d0 - c0, b1 = d1 - c1;
+                const uint32_t ta = c1 - c0, tbn = b1 - b0, tbx = tbn + (b1 < q.nb ? 1u : 0u);
+                const uint32_t dl = min(tid * XVT, ta + tbn);
+                const uint32_t steps = min(XVT, ta + tbn - dl);
+                const uint32_t i0 = merge_split(sAB, ta, sAB + ta, tbn, dl);
+                const uint32_t j0 = dl - i0;
+                // Merges the stretch; emit(x) for each match. Consuming A[i], an
+                // equal B element is the next one on B's side. Branch-free
+                // steps: the lanes of a wave take A or B steps in any mix, so
+                // each step advances one side by a predicate and reloads both
+                // current elements (clamped indices, always in the tile's LDS
+                // slice; a value past a side's end is never compared).
+                auto merge_stretch = [&](auto emit) {
+                    const uint32_t *sA = sAB, *sB = sAB + ta;
+                    const uint32_t alast = ta ? ta - 1 : 0u, blast = tbx ? tbx - 1 : 0u;
+                    uint32_t ii = i0, jj = j0;
+                    uint32_t av = sA[min(ii, alast)], bv = sB[min(jj, blast)];
+#pragma unroll
+                    for (uint32_t st = 0; st < XVT; st++) {
+                        const bool live = st < steps;
+                        const bool take_a = ii < ta && (jj >= tbn || av <= bv);
+                        if (live && take_a && jj < tbx && bv == av) emit(av);
+                        ii += (live && take_a) ? 1u : 0u;
+                        jj += (live && !take_a) ? 1u : 0u;
+                        av = sA[min(ii, alast)];
+                        bv = sB[min(jj, blast)];
+                    }
+                };
+                uint32_t c = 0;
                 // strictly increasing lists give at most XM; the clamp keeps
                 // any input inside the buffers
                 if (o) merge_stretch([&](uint32_t x) {
@@ -1664,32 +1689,36 @@ __global__ __launch_bounds__(IB) void fm_isect_kernel(const uint32_t *__restrict
                 });
                 else merge_stretch([&](uint32_t) { c++; });
                 c = min(c, XM);
-            }
-            uint32_t tot;
-            const uint32_t at = block_excl_scan(c, s_wave, tot);  // its barriers order the parking
-            if (o) {
-                // parked matches to block order, then coalesced stores
-                uint32_t m[XM];
-                if (skew) {
-                    m[0] = x_skew;
-                } else {
+                uint32_t tot;
+                const uint32_t at = block_excl_scan(c, s_wave, tot);  // its barriers order the parking
+                if (o) {
+                    // parked matches to block order, then coalesced stores
+                    uint32_t m[XM];
 #pragma unroll
                     for (uint32_t x = 0; x < XM; x++) m[x] = x < c ? s_out[tid * (XM + 1) + x] : 0u;
+                    __syncthreads();
+#pragma unroll
+                    for (uint32_t x = 0; x < XM; x++)
+                        if (x < c) s_out[at + x] = m[x];
+                    __syncthreads();
+                    for (uint32_t x = tid; x < tot; x += IB) o[found + x] = s_out[x];
+                }
+                found += tot;
+                __syncthreads();  // sAB and s_out are rewritten
+                if (has_next) {
+                    c0 = nc0;
+                    c1 = nc1;
+                    store_tile(t + 1, c0, c1);
+                } else if (t + 1 < te) {
+                    // the next tile is past the cut window: refill, then load it
+                    fill_cuts(t + 1);
+                    __syncthreads();
+                    load_tile(t + 1);
+                    c0 = nc0;
+                    c1 = nc1;
+                    store_tile(t + 1, c0, c1);
                 }
                 __syncthreads();
-#pragma unroll
-                for (uint32_t x = 0; x < XM; x++)
-                    if (x < c) s_out[at + x] = m[x];
-                __syncthreads();
-                for (uint32_t x = tid; x < tot; x += IB) o[found + x] = s_out[x];
-            }
-            found += tot;
-            __syncthreads();  // sAB and s_out are rewritten by the next tile
-            if (t + 1 < te && t + 1 - cbase == XCUT) {
-                // the next tile is past the cut window: refill, then load it
-                fill_cuts(t + 1);
-                __syncthreads();
-                load_tile(t + 1);
             }
         }
         if (tid == 0) counts[p] = found;

@@ -61,19 +61,36 @@ def tile_matches(A, B, k, cuts, te_minus_tb):
     sA = A[a0:a1]
     sB = B[b0:b1 + (1 if b1 < nb else 0)]
     ta, tb, tbx = a1 - a0, b1 - b0, len(sB)
+    # the element loads of the tile: thread tid, register x reads element
+    # e = tid + x * IB from A, or else from B at a clamped index
+    for x in range(XVT + 1):
+        e = np.arange(IB) + x * IB
+        ia = a0 + e[e < ta]
+        ib = np.minimum(b0 + e[e >= ta] - ta, nb - 1)
+        assert ((ia >= 0) & (ia < na)).all() and ((ib >= 0) & (ib < nb)).all()
+    # the LDS slice: A part, B part, then stale words up to XT + 1
+    sAB = np.full(XT + 1, 0xFFFFFFFF, dtype=np.int64)
+    sAB[:ta] = sA
+    sAB[ta:ta + tbx] = sB
+    alast, blast = max(ta - 1, 0), max(tbx - 1, 0)
     out = []
     for tid in range(IB):
         dl = min(tid * XVT, ta + tb)
         steps = min(XVT, ta + tb - dl)
         i = merge_split(sA, ta, sB, tb, dl)
         j = dl - i
-        for _ in range(steps):
-            if i < ta and (j >= tb or sA[i] <= sB[j]):
-                if j < tbx and sB[j] == sA[i]:
-                    out.append(int(sA[i]))
-                i += 1
-            else:
-                j += 1
+        # branch-free steps as in the kernel: both current elements are read
+        # at clamped indices after every step
+        av, bv = sAB[min(i, alast)], sAB[ta + min(j, blast)]
+        for st in range(XVT):
+            live = st < steps
+            take_a = i < ta and (j >= tb or av <= bv)
+            if live and take_a and j < tbx and bv == av:
+                out.append(int(av))
+            i += 1 if live and take_a else 0
+            j += 1 if live and not take_a else 0
+            assert min(i, alast) <= XT and ta + min(j, blast) <= XT
+            av, bv = sAB[min(i, alast)], sAB[ta + min(j, blast)]
     return out
 
 
