@@ -195,11 +195,17 @@ def cpu_baseline(g, queries, target_s: float, threads: int):
     og = orc.OracleGraph(g.vertices, s, d, l)
     log(f"cpu baseline: oracle graph built in {time.perf_counter() - t0:.1f} s")
     V = g.vertices
-    # calibrate the stride on a coarse sample so the run takes ~target_s
-    stride = 4096
-    probe = sum(orc.match(og, q.vertices, q.edges(), threads=threads, table=False, stride=stride).seconds
-                for _, _, q in queries)
-    log(f"cpu baseline: probe at stride {stride} took {probe:.2f} s")
+    # calibrate the stride on ever denser samples (1, 16, 256, ... roots per
+    # query) until one takes a twentieth of the target, so the probe of a
+    # heavy query set stays short; then scale the stride to the target
+    stride = max(1, V)
+    while True:
+        probe = sum(orc.match(og, q.vertices, q.edges(), threads=threads, table=False, stride=stride).seconds
+                    for _, _, q in queries)
+        log(f"cpu baseline: probe at stride {stride} took {probe:.2f} s")
+        if probe >= target_s / 20 or stride == 1:
+            break
+        stride = max(1, stride // 16)
     if probe > 0:
         stride = int(max(1, min(V, stride * probe / target_s)))
     for _ in range(4):

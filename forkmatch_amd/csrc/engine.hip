@@ -660,7 +660,7 @@ __device__ uint32_t wait_page(const KParams &P, uint32_t q, uint32_t k) {
 // many waves emit its rows. The wave whose rows include the first row of a
 // page allocates it; waves with rows further into that page wait for the
 // slot entry. When the arena runs out the query's table ends at the start
-// of the page that failed (trunc_at): every row below trunc_at is stored.
+// of the first page that failed (trunc_at): every row below it is stored.
 template <class F>
 __device__ void emit_rows_dense(const KParams &P, uint32_t q, uint32_t n, bool ok, uint32_t rank, uint32_t cnt,
                                 unsigned long long *stats, F write_row) {
@@ -671,11 +671,16 @@ __device__ void emit_rows_dense(const KParams &P, uint32_t q, uint32_t n, bool o
     if (lane == 0) {
         const unsigned long long r0 = atom_add(&Q.cursor, (unsigned long long)cnt);
         // the page whose first row is ours, if any: resolved whatever we keep,
-        // since waves that read an older trunc_at may wait for it
+        // since waves that read an older trunc_at may wait for it. Pages
+        // resolve in index order (the owner of page k waits for page k - 1,
+        // whose owner claimed its rows earlier; the chain ends at page 0), so
+        // an exhausted arena truncates at the first page it cannot hold and
+        // no page past it takes one of the last free pages.
         const unsigned long long kf = (r0 + pr - 1) / pr;
         if (kf * pr < r0 + cnt && kf < Q.max_pages) {
+            const uint32_t prev = kf ? wait_page(P, q, (uint32_t)kf - 1) : 0u;
             uint32_t id = FAILED_PAGE;
-            if (kf * pr < ld_rlx(&Q.trunc_at)) {
+            if (prev != FAILED_PAGE && kf * pr < ld_rlx(&Q.trunc_at)) {
                 id = srv_page_alloc(P, stats);
                 if (id == NO_PAGE) {
                     id = FAILED_PAGE;
@@ -979,9 +984,10 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
         // ---- candidates, 64 per step ----
         for (uint32_t base = 0, step = 0; base < T; base += WAVE, step++) {
             // Work splitting: when waves are idle, donate half of what is
-            // left, either the upper half of the remaining parents or, in a
-            // task's last batch, the upper half of its remaining candidates
-            // (a hub parent's long driver row is split across waves).
+            // left: the upper half of the remaining parents when at least
+            // two batches remain, else the upper half of this batch's
+            // remaining candidates together with the parents after them (a
+            // hub parent's long driver row is split across waves).
             if ((step & 3) == 0) {
                 uint32_t want = 0;
                 if (lane == 0) want = ld_rlx(&P.ctl->hungry) > 0 ? 1u : 0u;
@@ -998,7 +1004,7 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
                         cnt_local = first;
                         last_end = ALL_CANDIDATES;
                         if (lane == 0) stats[ST_SPLIT]++;
-                    } else if (rem_par == 0 && T - base >= 8 * WAVE) {
+                    } else if (T - base >= 8 * WAVE) {
                         const uint32_t mid = base + ((T - base) / (2 * WAVE)) * WAVE;
                         uint32_t lo = 0, hi = nb - 1;
                         while (lo < hi) {
@@ -2111,7 +2117,7 @@ int fm_match_batch(fm_engine *e, const fm_graph *g, const fm_query *const *qs, u
     fm_match_opts o{};
     if (opts) o = *opts;
     if (o.timeout_ms == 0) o.timeout_ms = 120000;
-    if (o.task_rows == 0) o.task_rows = nq == 1 ? SubMaxMulSpill : FM_BATCH_TASK_ROWS;
+    if (o.task_rows == 0) o.task_rows = SubMaxMulSpill;
     if (o.root_chunk == 0) o.root_chunk = TooLargeThreshold;
     HIP_TRY(hipSetDevice(e->device));
     if (e->dirty) {
@@ -2500,7 +2506,7 @@ int fm_server_create(fm_engine *e, const fm_graph *g, const fm_server_config *cf
     if (!s->cfg.slot_pages) s->cfg.slot_pages = 256;
     if (!s->cfg.idle_ms) s->cfg.idle_ms = 20;
     if (!s->cfg.root_chunk) s->cfg.root_chunk = TooLargeThreshold;
-    if (!s->cfg.task_rows) s->cfg.task_rows = FM_BATCH_TASK_ROWS;
+    if (!s->cfg.task_rows) s->cfg.task_rows = SubMaxMulSpill;
     if (!s->cfg.bank_budget) s->cfg.bank_budget = e->nbanks;
     if (!s->cfg.max_launch_ms) s->cfg.max_launch_ms = 120000;
     if (!s->cfg.slice_ms) s->cfg.slice_ms = 2000;

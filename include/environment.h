@@ -40,15 +40,13 @@
 /* Maximum parents carried by one queued task. This replaces the reference's
  * soft fork threshold (outputMax - BlockSize * SubMaxMulSpill,
  * src/smsm.cu:451-454): a bank segment larger than this is split into
- * several tasks so idle waves can take part of it. The default for a
- * single-query launch is one 64-parent batch: on R-MAT workloads it keeps
- * ~80% of the waves busy, against ~40% with 1024-parent tasks. */
-#define SubMaxMulSpill 64
-
-/* The same for launches that carry several queries (batches, serving):
- * there the queries themselves keep the waves busy, and 1024-parent tasks
- * halve the queue traffic (measured 2.3x faster than 64 on 1,024 queries). */
-#define FM_BATCH_TASK_ROWS 1024
+ * several tasks so idle waves can take part of it. While waves are idle a
+ * wave cuts its children into 64-parent tasks instead, and donates half of
+ * its remaining parents or candidates. Measured on R-MAT 2^20 with 16
+ * queries of 6 vertices, one launch each: 1024-parent tasks take 100 ms per
+ * step, 64-parent tasks 215 ms (per-task queue and staging overhead, and
+ * siblings of one parent kept together share their neighbour rows in L2). */
+#define SubMaxMulSpill 1024
 
 /* Number of root vertices claimed by one wave at a time (the reference gives
  * each block a fixed root range, src/smsm.cu:803-811). */

@@ -251,8 +251,7 @@ typedef struct fm_match_opts {
     uint64_t max_count;       /* abort a query past this count (0 = none) */
     uint32_t timeout_ms;      /* launch deadline (0 = 120000) */
     uint32_t grid_waves;      /* persistent waves (0 = full residency) */
-    uint32_t task_rows;       /* parents per queued task (0: SubMaxMulSpill for one
-                                 query, FM_BATCH_TASK_ROWS for several) */
+    uint32_t task_rows;       /* parents per queued task (0: SubMaxMulSpill) */
     uint32_t root_chunk;      /* roots per claim (TooLargeThreshold) */
     uint32_t plan;            /* FM_PLAN_COST (0) or FM_PLAN_REFERENCE */
     /* abort a query once it has stored more than this many intermediate
@@ -342,7 +341,7 @@ typedef struct fm_server_config {
     uint32_t bank_budget;    /* admit while live banks < budget (0 = pool) */
     uint32_t idle_ms;        /* launch lifetime without work (default 20) */
     uint32_t root_chunk;     /* roots per root task (default TooLargeThreshold) */
-    uint32_t task_rows;      /* parents per task (default FM_BATCH_TASK_ROWS) */
+    uint32_t task_rows;      /* parents per task (default SubMaxMulSpill) */
     uint32_t table;          /* 1: store tables (default), 0: count only */
     uint32_t max_launch_ms;  /* deadline of one launch (default 120000) */
     uint32_t plan;           /* FM_PLAN_COST (0) or FM_PLAN_REFERENCE */

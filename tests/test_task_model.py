@@ -60,7 +60,7 @@ def process(task, lens, rng, queue, seen, p_split, p_yield):
                     f = b0 + nb + keep
                     queue.append((first + f, u32(rem_par - keep), 0, last))
                     cnt_local, last = f, ALL
-                elif rem_par == 0 and T - base >= 8 * WAVE:
+                elif T - base >= 8 * WAVE:
                     mid = base + ((T - base) // (2 * WAVE)) * WAVE
                     lo = parent_of(mid)
                     excl = cum[lo - 1] if lo else 0
