@@ -27,9 +27,9 @@ class TuneResult:
 
 
 DEFAULT_SPACE = {
-    "root_chunk": [64, 256, 1024, 4096],
-    "task_rows": [64, 256, 1024],
-    "grid_frac": [0.5, 1.0],
+    "root_chunk": [256, 1024, 4096],
+    "task_rows": [256, 1024, 4096],
+    "grid_frac": [0.5, 1.0, 1.5],
 }
 
 
@@ -43,6 +43,8 @@ def autotune(engine, graph, queries, space: dict | None = None, table: bool = Fa
     keys = list(space)
     for combo in itertools.product(*(space[k] for k in keys)):
         cfg = dict(zip(keys, combo))
+        # grid_frac is relative to the engine's default grid (8 waves per CU);
+        # the library caps it at the resident waves
         opts = {"root_chunk": cfg.get("root_chunk", 0), "task_rows": cfg.get("task_rows", 0),
                 "grid_waves": max(1, int(grid * cfg.get("grid_frac", 1.0)))}
         ms = []

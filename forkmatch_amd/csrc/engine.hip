@@ -126,19 +126,34 @@ struct LevelQ {
     alignas(128) int32_t avail;
 };
 
+// Idle-wave count in shards on separate lines: waves going idle and busy
+// update one shard each, readers sum them.
+constexpr uint32_t HUNGRY_SHARDS = 8;
+struct HungryShard {
+    alignas(128) int32_t n;
+};
+
+constexpr uint32_t STAT_SHARDS = 64;
+constexpr uint32_t STAT_STRIDE = 16;  // u64 per shard: one 128-byte line
+static_assert(ST_N <= STAT_STRIDE, "counters fit a shard");
+
 struct Ctl {
     alignas(128) long long pending;  // tasks pushed or root chunks not yet finished
     alignas(128) unsigned long long root_cursor;
-    alignas(128) int32_t hungry;  // waves that found no work
+    HungryShard hungry[HUNGRY_SHARDS];  // waves that found no work, by workgroup id % HUNGRY_SHARDS
     alignas(128) uint32_t abort;
     uint32_t error;
     unsigned long long t0;  // wall clock of the first wave (deadline origin)
     unsigned long long timeout_ticks;
     uint32_t diag[12];  // FM_ERR_INTERNAL: the task; FM_ERR_TIMEOUT: where the launch ran out of time
     alignas(128) uint32_t page_bump;
-    alignas(128) unsigned long long stats[ST_N];
+    // per-launch counters, summed by the host over the shards: each wave adds
+    // its counters into shard (workgroup id % STAT_SHARDS) at exit, so
+    // thousands of waves leaving together do not queue on one cache line
+    alignas(128) unsigned long long stats[STAT_SHARDS][STAT_STRIDE];
     // persistent
     alignas(128) LevelQ lq[NLEVELS];
+    alignas(128) int32_t queued;  // tasks in the queues, all levels (sum of lq[].avail)
     alignas(128) uint32_t f_head;
     alignas(128) uint32_t f_tail;
     alignas(128) int32_t f_avail;
@@ -318,14 +333,38 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
     return acc;
 }
 
+// Words after the last neighbour that a window load of row_contains may
+// read; the device column array is allocated (and zeroed) with this padding.
+constexpr uint32_t COL_PAD = 16;
+
+// Membership of x in the sorted row col[st, st + len). Binary steps narrow
+// the position where x would sit to at most 13 places; those lie in one
+// 16-word window starting at a 16-byte boundary, read with four independent
+// 16-byte loads and compared in registers. The search is a chain of
+// dependent loads, so this replaces its last 3-4 round trips by one.
 __device__ __forceinline__ bool row_contains(const uint32_t *col, uint32_t st, uint32_t len, uint32_t x) {
-    uint32_t lo = st, hi = st + len;
-    while (lo < hi) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (col[mid] < x) lo = mid + 1;
-        else hi = mid;
+    // invariant: the first position whose value is >= x lies in [lo, lo + n]
+    uint32_t lo = st, n = len;
+    while (n > 12) {
+        const uint32_t half = n >> 1;
+        if (col[lo + half] < x) {
+            lo += half + 1;
+            n -= half + 1;
+        } else {
+            n = half;
+        }
     }
-    return lo < st + len && col[lo] == x;
+    // [lo, lo + 12] lies in [w0, w0 + 16); lo <= st + len, so the window ends
+    // within COL_PAD words past the array
+    const uint32_t w0 = lo & ~3u;
+    const uint4 *wp = reinterpret_cast<const uint4 *>(col + w0);
+    const uint4 q0 = wp[0], q1 = wp[1], q2 = wp[2], q3 = wp[3];
+    const uint32_t v[16] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w,
+                            q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
+    bool hit = false;
+#pragma unroll
+    for (uint32_t j = 0; j < 16; j++) hit |= (w0 + j - st < len) && v[j] == x;
+    return hit;
 }
 
 __device__ __forceinline__ void row_of(const DGraph &g, uint32_t req, uint32_t v, uint32_t &st, uint32_t &len) {
@@ -449,6 +488,14 @@ constexpr uint32_t ALL_CANDIDATES = 0xFFFFFFFFu;
 // TaskDesc::q = query slot | tag << 24. Serving tags a query's tasks with the
 // low byte of its ticket, so a task that outlives its query is recognised
 // (the slot is reused by later queries); batch launches use tag 0.
+// Idle waves back off exponentially between polls of the queues, up to
+// 2^FM_IDLE_BACKOFF_LOG2 sleeps of 8 x 64 cycles.
+#ifndef FM_IDLE_BACKOFF_LOG2
+#define FM_IDLE_BACKOFF_LOG2 5
+#endif
+#ifndef FM_IDLE_BACKOFF_LOG2_UPPER
+#define FM_IDLE_BACKOFF_LOG2_UPPER 10
+#endif
 constexpr uint32_t QSLOT_MASK = 0xFFFFFFu;
 __device__ __forceinline__ uint32_t qslot(uint32_t q) { return q & QSLOT_MASK; }
 struct TaskDesc {
@@ -496,7 +543,10 @@ __device__ void push_tasks(const KParams &P, uint32_t lvl, uint32_t m, const Tas
         st_rel(&s->seq, pos + 1);
         }
     }
-    if (lane == 0) atom_add(&c->lq[lvl].avail, (int32_t)m);
+    if (lane == 0) {
+        atom_add(&c->lq[lvl].avail, (int32_t)m);
+        atom_add(&c->queued, (int32_t)m);
+    }
 }
 
 // Lane 0 only: takes the task of the lowest level (fewest remaining depths).
@@ -508,6 +558,7 @@ __device__ bool pop_task(const KParams &P, TaskDesc &t) {
             atom_add(&c->lq[lvl].avail, 1);
             continue;
         }
+        atom_add(&c->queued, -1);
         uint32_t pos = atom_add(&c->lq[lvl].head, 1u);
         Slot *s = &P.ring[(size_t)lvl * P.qcap + (pos & (P.qcap - 1))];
         if (!wait_eq(&s->seq, pos + 1, c, 14)) return false;
@@ -525,10 +576,17 @@ __device__ bool pop_task(const KParams &P, TaskDesc &t) {
 }
 
 // Lane 0 only: claims the next chunk of root vertices.
-__device__ bool claim_root(const KParams &P, TaskDesc &t, unsigned long long *stats) {
+// A wave's first claim is chunk (workgroup id), without an atomic: the
+// host starts root_cursor at the grid size, so later claims take the chunks
+// past the first round (thousands of waves starting together would otherwise
+// queue on the cursor).
+__device__ bool claim_root(const KParams &P, TaskDesc &t, unsigned long long *stats, bool first_claim) {
     Ctl *c = P.ctl;
-    if (ld_rlx(&c->root_cursor) >= P.total_root_chunks) return false;
-    unsigned long long k = atom_add(&c->root_cursor, 1ull);
+    unsigned long long k = blockIdx.x;
+    if (!first_claim) {
+        if (ld_rlx(&c->root_cursor) >= P.total_root_chunks) return false;
+        k = atom_add(&c->root_cursor, 1ull);
+    }
     if (k >= P.total_root_chunks) return false;
     // the query whose chunk range holds k
     uint32_t lo = 0, hi = P.nq - 1;
@@ -990,7 +1048,20 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
             // hub parent's long driver row is split across waves).
             if ((step & 3) == 0) {
                 uint32_t want = 0;
-                if (lane == 0) want = ld_rlx(&P.ctl->hungry) > 0 ? 1u : 0u;
+                // donate only while idle waves outnumber the waiting work
+                // (queued tasks and unclaimed root chunks): with thousands
+                // of waves idle, every busy wave would otherwise cut its
+                // work into pieces at every check
+                if (lane == 0) {
+                    long long idle_waves = 0;
+                    for (uint32_t h = 0; h < HUNGRY_SHARDS; h++) idle_waves += ld_rlx(&P.ctl->hungry[h].n);
+                    if (idle_waves > 0) {
+                        const unsigned long long rc = ld_rlx(&P.ctl->root_cursor);
+                        const long long roots_left =
+                            rc < P.total_root_chunks ? (long long)(P.total_root_chunks - rc) : 0;
+                        want = idle_waves > max(0, ld_rlx(&P.ctl->queued)) + roots_left ? 1u : 0u;
+                    }
+                }
                 want = bcast(want, 0);
                 run_rows = want ? min(P.task_rows, WAVE) : P.task_rows;
                 if (want) {
@@ -1241,16 +1312,20 @@ __device__ void admit(const KParams &P, uint32_t slot, unsigned long long *stats
     uint32_t *dst = (uint32_t *)&P.plans_rw[slot];
     for (uint32_t i = lane; i < sizeof(DPlan) / 4; i += WAVE) dst[i] = ld_sys(src + i);
     const SrvQInit *qi = &P.srv_qinit[slot];
+    // query state: zero, then the fields the submission sets (lane 0 stores
+    // after the wave's zero stores to the same words)
+    static_assert(sizeof(DQuery) % 4 == 0, "DQuery is cleared by words");
+    uint32_t *qw = reinterpret_cast<uint32_t *>(&Q);
+    for (uint32_t i = lane; i < sizeof(DQuery) / 4; i += WAVE) qw[i] = 0;
     if (lane == 0) {
-        DQuery z{};
-        z.max_rows = ((unsigned long long)ld_sys((const uint32_t *)&qi->max_rows + 1) << 32) |
-                     ld_sys((const uint32_t *)&qi->max_rows);
-        z.page_rows = ld_sys(&qi->page_rows);
-        z.max_pages = ld_sys(&qi->max_pages);
-        z.trunc_at = z.max_rows;
-        z.pad = ld_sys(&qi->ticket);  // ticket, echoed in the completion
-        z.qpending = 1;               // held until every root chunk is queued
-        Q = z;
+        const unsigned long long max_rows = ((unsigned long long)ld_sys((const uint32_t *)&qi->max_rows + 1) << 32) |
+                                            ld_sys((const uint32_t *)&qi->max_rows);
+        Q.max_rows = max_rows;
+        Q.trunc_at = max_rows;
+        Q.page_rows = ld_sys(&qi->page_rows);
+        Q.max_pages = ld_sys(&qi->max_pages);
+        Q.pad = ld_sys(&qi->ticket);  // ticket, echoed in the completion
+        Q.qpending = 1;               // held until every root chunk is queued
         atom_add(&c->n_active, 1);
     }
     __syncthreads();
@@ -1296,12 +1371,17 @@ __global__ __launch_bounds__(WAVE) void fm_match_kernel(KParams P) {
     const unsigned long long t_wave = wall_clock64();
     unsigned long long starve_since = 0;
     unsigned long long deadline = 0, retire_at = ~0ull;  // retire_at: lane 0 only
+    bool first_claim = true;                              // lane 0 only
     if (lane == 0) {
-        unsigned long long now = wall_clock64(), zero = 0;
-        __hip_atomic_compare_exchange_strong(&P.ctl->t0, &zero, now, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT);
-        deadline = ld_rlx(&P.ctl->t0) + P.timeout_ticks;
-        if (P.serving && P.slice_ticks) retire_at = ld_rlx(&P.ctl->t0) + P.slice_ticks;
+        // Each wave times its deadline and slice from its own start (the
+        // waves of a launch start within microseconds); only workgroup 0
+        // publishes the launch start for the spin-wait deadline, so the
+        // waves do not queue on one word.
+        const unsigned long long now = wall_clock64();
+        if (blockIdx.x == 0)
+            __hip_atomic_store(&P.ctl->t0, now, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        deadline = now + P.timeout_ticks;
+        if (P.serving && P.slice_ticks) retire_at = now + P.slice_ticks;
     }
     deadline = ((unsigned long long)bcast((uint32_t)(deadline >> 32), 0) << 32) | bcast((uint32_t)deadline, 0);
     while (true) {
@@ -1319,7 +1399,9 @@ __global__ __launch_bounds__(WAVE) void fm_match_kernel(KParams P) {
                 flag = 2;
             } else {
                 TaskDesc t;
-                if (pop_task(P, t) || (!P.serving && claim_root(P, t, stats))) {
+                const bool first = first_claim;
+                first_claim = false;
+                if (pop_task(P, t) || (!P.serving && claim_root(P, t, stats, first))) {
                     s_task = t;
                     flag = 0;
                 } else if (P.serving) {
@@ -1354,15 +1436,20 @@ __global__ __launch_bounds__(WAVE) void fm_match_kernel(KParams P) {
         }
         if (flag == 1) {
             if (idle == 0 && lane == 0) {
-                atom_add(&P.ctl->hungry, 1);
+                atom_add(&P.ctl->hungry[blockIdx.x % HUNGRY_SHARDS].n, 1);
                 idle_since = wall_clock64();
             }
-            // exponential back-off keeps idle waves off the queue counters
-            for (uint32_t i = 0, lim = 1u << min(idle, 5u); i < lim; i++) __builtin_amdgcn_s_sleep(8);
+            // Exponential back-off keeps idle waves off the queue counters.
+            // The upper half of the grid backs off further: it joins work
+            // that persists and otherwise stays off the counters that the
+            // busy waves update.
+            const uint32_t cap = blockIdx.x * 2 >= gridDim.x ? (uint32_t)FM_IDLE_BACKOFF_LOG2_UPPER
+                                                             : (uint32_t)FM_IDLE_BACKOFF_LOG2;
+            for (uint32_t i = 0, lim = 1u << min(idle, cap); i < lim; i++) __builtin_amdgcn_s_sleep(8);
             idle++;
             continue;
         }
-        if (idle && lane == 0) atom_add(&P.ctl->hungry, -1);
+        if (idle && lane == 0) atom_add(&P.ctl->hungry[blockIdx.x % HUNGRY_SHARDS].n, -1);
         idle = 0;
         const TaskDesc t = s_task;
         __syncthreads();
@@ -1402,9 +1489,10 @@ __global__ __launch_bounds__(WAVE) void fm_match_kernel(KParams P) {
         if (o.bank != NO_BANK) bank_release(P, o.bank);
         tab_release(P, wt);
         stats[ST_ALIVE] = wall_clock64() - t_wave;
-        for (int i = 0; i < ST_N; i++)
-            if (stats[i]) atom_add(&P.ctl->stats[i], stats[i]);
     }
+    __syncthreads();
+    // one lane per counter, into this wave's shard
+    if (lane < ST_N && stats[lane]) atom_add(&P.ctl->stats[blockIdx.x % STAT_SHARDS][lane], stats[lane]);
 }
 
 __global__ void fm_init_slots_kernel(Slot *ring, uint32_t qcap, uint32_t levels, FSlot *fring, uint32_t fcap) {
@@ -1784,6 +1872,13 @@ struct fm_engine {
     uint32_t pfcap = 0;
 };
 
+// A launch counter summed over its shards.
+static unsigned long long stat_sum(const Ctl &c, uint32_t i) {
+    unsigned long long v = 0;
+    for (uint32_t k = 0; k < STAT_SHARDS; k++) v += c.stats[k][i];
+    return v;
+}
+
 struct fm_result {
     fm_engine *e = nullptr;
     uint64_t epoch = 0;
@@ -1859,8 +1954,9 @@ int fm_graph_upload(fm_graph *g) {
     if ((rc = dalloc(&g->d_ptr[0], rows))) return rc;
     HIP_TRY(hipMemcpy(g->d_ptr[0], h.out.ptr.data(), rows * 4, hipMemcpyHostToDevice));
     uint64_t ncol = h.symmetric ? E : 2 * E;
-    if ((rc = dalloc(&g->d_col[0], ncol))) return rc;
+    if ((rc = dalloc(&g->d_col[0], ncol + COL_PAD))) return rc;
     HIP_TRY(hipMemcpy(g->d_col[0], h.out.col.data(), E * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(g->d_col[0] + ncol, 0, COL_PAD * 4));
     uint64_t nroots = h.out.roots.size() + (h.symmetric ? 0 : h.in.roots.size());
     if ((rc = dalloc(&g->d_roots[0], nroots))) return rc;
     HIP_TRY(hipMemcpy(g->d_roots[0], h.out.roots.data(), h.out.roots.size() * 4, hipMemcpyHostToDevice));
@@ -1954,9 +2050,9 @@ int fm_engine_create(const fm_engine_config *cfg, fm_engine **out) {
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fm_match_kernel, WAVE, 0) != hipSuccess || blocks <= 0)
         blocks = 8;
     e->waves_per_cu = blocks;
-    // Resident waves: at most 8 per CU (2 per SIMD). More waves fit, but
-    // they measured slower on R-MAT workloads: the waves' requirement rows
-    // then overflow the per-XCD L2 and idle waves poll the queues harder.
+    // Default grid: 8 waves per CU. The serving launch uses it as is; a
+    // one-query launch uses half of it and a batch every resident wave
+    // (fm_match_batch gives the measurements).
     e->grid_waves = (uint32_t)(std::min(blocks, 8) * e->cu_count);
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess)
@@ -2117,7 +2213,7 @@ int fm_match_batch(fm_engine *e, const fm_graph *g, const fm_query *const *qs, u
     fm_match_opts o{};
     if (opts) o = *opts;
     if (o.timeout_ms == 0) o.timeout_ms = 120000;
-    if (o.task_rows == 0) o.task_rows = SubMaxMulSpill;
+    if (o.task_rows == 0) o.task_rows = nq == 1 ? SubMaxMulSpill : FM_BATCH_TASK_ROWS;
     if (o.root_chunk == 0) o.root_chunk = TooLargeThreshold;
     HIP_TRY(hipSetDevice(e->device));
     if (e->dirty) {
@@ -2129,6 +2225,16 @@ int fm_match_batch(fm_engine *e, const fm_graph *g, const fm_query *const *qs, u
     std::vector<DPlan> plans(nq);
     std::vector<DQuery> qh(nq);
     uint64_t chunks = 0;
+    // Persistent waves. One query: 4 per CU. Measured on R-MAT 2^20 with 16
+    // queries of 6 vertices, one launch each: 79 ms of kernel time per step
+    // at 4 waves per CU, 97 ms at 8 (every wave's first bank and page claims
+    // and its idle polls contend on shared counters, and one query seldom
+    // keeps 2,048 waves busy). Several queries: every resident wave (12 per
+    // CU on gfx950); 1,024 queries of 4-12 vertices on the same graph take
+    // 340 ms at 12 per CU against 445 ms at 8.
+    const uint32_t resident = std::max<uint32_t>(1, (uint32_t)e->waves_per_cu * e->cu_count);
+    const uint32_t grid = o.grid_waves ? std::min<uint32_t>(o.grid_waves, resident)
+                                       : (nq == 1 ? std::max<uint32_t>(1, e->grid_waves / 2) : resident);
     uint32_t maxn = 1;
     for (uint32_t i = 0; i < nq; i++) {
         if (!qs[i]) return fail(FM_ERR_INVALID_ARG, "null query %u", i);
@@ -2143,7 +2249,7 @@ int fm_match_batch(fm_engine *e, const fm_graph *g, const fm_query *const *qs, u
         Q.max_rows = (o.mode == FM_MODE_TABLE) ? (o.max_rows ? std::min<uint64_t>(o.max_rows, cap_rows) : cap_rows) : 0;
         // pages are filled by several waves at once, so a query may hold one
         // partly filled page per wave beyond the full ones
-        uint64_t want_pages = (Q.max_rows + Q.page_rows - 1) / Q.page_rows + (o.max_rows ? e->grid_waves : 0);
+        uint64_t want_pages = (Q.max_rows + Q.page_rows - 1) / Q.page_rows + (o.max_rows ? grid : 0);
         Q.max_pages = (o.mode == FM_MODE_TABLE) ? (uint32_t)std::min<uint64_t>(want_pages, e->npages) : 0;
         Q.max_count = o.max_count;
         Q.max_partials = o.max_partials;
@@ -2167,6 +2273,7 @@ int fm_match_batch(fm_engine *e, const fm_graph *g, const fm_query *const *qs, u
     HIP_TRY(hipMemcpyAsync(e->d_qs, qh.data(), nq * sizeof(DQuery), hipMemcpyHostToDevice, e->stream));
     Ctl reset{};
     reset.pending = (long long)chunks;
+    reset.root_cursor = grid;  // chunks below the grid size are each wave's first claim
     reset.timeout_ticks = (unsigned long long)((double)o.timeout_ms * (e->wall_hz / 1000.0));
     HIP_TRY(hipMemcpyAsync(e->ctl, &reset, CTL_RESET_BYTES, hipMemcpyHostToDevice, e->stream));
     // ---- launch ----
@@ -2201,7 +2308,6 @@ int fm_match_batch(fm_engine *e, const fm_graph *g, const fm_query *const *qs, u
     P.npages = e->npages;
     P.nlevels = std::max<uint32_t>(1, maxn - 1);
     P.starve_ticks = (unsigned long long)(e->wall_hz * 2.0);  // 2 s without a bank
-    uint32_t grid = o.grid_waves ? std::min(o.grid_waves, e->grid_waves) : e->grid_waves;
     P.timeout_ticks = (unsigned long long)((double)o.timeout_ms * (e->wall_hz / 1000.0));
     fm_launch_stats st{};
     st.grid_waves = grid;
@@ -2228,15 +2334,15 @@ int fm_match_batch(fm_engine *e, const fm_graph *g, const fm_query *const *qs, u
     r->n.resize(nq);
     for (uint32_t i = 0; i < nq; i++) r->n[i] = plans[i].n;
     st.kernel_ms = ms;
-    st.tasks_pushed = ctl_back.stats[ST_TASKS];
-    st.root_claims = ctl_back.stats[ST_ROOTS];
-    st.banks_allocated = ctl_back.stats[ST_BANKS_FRESH];
-    st.banks_reused = ctl_back.stats[ST_BANKS_REUSED];
-    st.alloc_failures = ctl_back.stats[ST_ALLOC_FAIL];
-    st.tasks_split = ctl_back.stats[ST_SPLIT];
-    st.requeues = ctl_back.stats[ST_REQUEUE];
-    st.pages_allocated = ctl_back.stats[ST_PAGES];
-    st.busy_frac = ctl_back.stats[ST_ALIVE] ? (double)ctl_back.stats[ST_BUSY] / (double)ctl_back.stats[ST_ALIVE] : 0.0;
+    st.tasks_pushed = stat_sum(ctl_back, ST_TASKS);
+    st.root_claims = stat_sum(ctl_back, ST_ROOTS);
+    st.banks_allocated = stat_sum(ctl_back, ST_BANKS_FRESH);
+    st.banks_reused = stat_sum(ctl_back, ST_BANKS_REUSED);
+    st.alloc_failures = stat_sum(ctl_back, ST_ALLOC_FAIL);
+    st.tasks_split = stat_sum(ctl_back, ST_SPLIT);
+    st.requeues = stat_sum(ctl_back, ST_REQUEUE);
+    st.pages_allocated = stat_sum(ctl_back, ST_PAGES);
+    st.busy_frac = stat_sum(ctl_back, ST_ALIVE) ? (double)stat_sum(ctl_back, ST_BUSY) / (double)stat_sum(ctl_back, ST_ALIVE) : 0.0;
     r->pages_used = std::min(ctl_back.page_bump, e->npages);
     st.status = ctl_back.error;
     if (ctl_back.error) {
@@ -2472,7 +2578,7 @@ static int server_pump(fm_server *s) {
         s->launched = false;
         Ctl back{};
         HIP_TRY(hipMemcpy(&back, s->e->ctl, CTL_RESET_BYTES, hipMemcpyDeviceToHost));
-        s->st.dropped_tasks += back.stats[ST_STALE];
+        s->st.dropped_tasks += stat_sum(back, ST_STALE);
         if (back.error) {
             s->e->dirty = true;
             s->failed = (int)back.error;
