@@ -327,10 +327,13 @@ def serve_rate(srv, pool, counts, rate: float, seconds: float, rng, balg=None, d
                                f"({ndone}/{n_req} done)")
     lat = (done - sched) * 1e3
     span = float(np.nanmax(done) - sched[0])
+    slow = np.argsort(lat)[::-1][:5]
     out = {"offered_req_per_s": rate, "requests": n_req, "achieved_queries_per_s": n_req / span,
            "latency_ms_p50": float(np.percentile(lat, 50)), "latency_ms_p99": float(np.percentile(lat, 99)),
            "latency_ms_max": float(lat.max()), "embeddings_per_s": emb / span, "max_in_flight": inflight_max,
-           "refused_submits": retries, "mismatches": bad, "seconds": span}
+           "refused_submits": retries, "mismatches": bad, "seconds": span,
+           "slowest": [{"arrival_s": round(float(sched[k]), 4), "latency_ms": round(float(lat[k]), 2),
+                        "query": int(qidx[k])} for k in slow]}
     if balg is not None:
         out["alg_bytes"] = int(sum(balg[i] for i in qidx))
         out["alg_gbs_wall"] = out["alg_bytes"] / span / 1e9

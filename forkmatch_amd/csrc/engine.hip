@@ -133,6 +133,13 @@ struct HungryShard {
     alignas(128) int32_t n;
 };
 
+constexpr unsigned long long SRV_ACTIVE_ONE = 1ull << 32;
+constexpr unsigned long long SRV_CLOSING = 1ull << 63;
+__host__ __device__ inline uint32_t srv_head(unsigned long long st) { return (uint32_t)st; }
+__host__ __device__ inline uint32_t srv_active(unsigned long long st) {
+    return (uint32_t)((st & ~SRV_CLOSING) >> 32);
+}
+
 constexpr uint32_t STAT_SHARDS = 64;
 constexpr uint32_t STAT_STRIDE = 16;  // u64 per shard: one 128-byte line
 static_assert(ST_N <= STAT_STRIDE, "counters fit a shard");
@@ -159,8 +166,12 @@ struct Ctl {
     alignas(128) int32_t f_avail;
     alignas(128) uint32_t bank_bump;
     // serving (persistent across serving launches)
-    alignas(128) uint32_t sub_head;      // next submission to admit
-    alignas(128) int32_t n_active;       // admitted, not completed
+    // Serving state in one word, so that a claim, the count of queries in
+    // flight and the decision to end the launch change together: bits 0-31
+    // the next submission to admit, bits 32-62 the admitted queries not yet
+    // completed, bit 63 set once the launch is closing (no more claims; all
+    // waves leave). The host clears bit 63 before it relaunches.
+    alignas(128) unsigned long long srv_state;
     alignas(128) int32_t live_banks;     // banks out of the pool
     alignas(128) uint32_t srv_page_bump;
     alignas(128) uint32_t pf_head;       // recycled output pages
@@ -1268,7 +1279,7 @@ __device__ void complete_query(const KParams &P, uint32_t q) {
     st_sys(&d->flags, __hip_atomic_load(&Q.flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
     st_sys(&d->npages, pages);
     __hip_atomic_store(&d->ticket, Q.pad, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-    atom_add(&P.ctl->n_active, -1);
+    __hip_atomic_fetch_add(&P.ctl->srv_state, -SRV_ACTIVE_ONE, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     st_sys(&P.srv->live_banks, (uint32_t)max(0, ld_rlx(&P.ctl->live_banks)));
 }
 
@@ -1278,9 +1289,16 @@ __device__ uint32_t claim_submission(const KParams &P) {
     // The two relaxed loads may complete in either order, so the tail read
     // can be older than the head: the head may already be past it. Only a
     // head strictly behind the tail names a written entry.
-    uint32_t h = ld_rlx(&c->sub_head);
+    unsigned long long st = ld_rlx(&c->srv_state);
+    if (st & SRV_CLOSING) return NO_PAGE;
+    const uint32_t h = srv_head(st);
     if ((int32_t)(ld_sys(&P.srv->sub_tail) - h) <= 0) return NO_PAGE;
-    if (!cas_u32(&c->sub_head, h, h + 1)) return NO_PAGE;
+    // the head advances and the query counts as active in one step; a
+    // launch that started closing refuses (its waves are leaving)
+    const unsigned long long nst = (st & ~0xFFFFFFFFull) + SRV_ACTIVE_ONE + (h + 1u);
+    if (!__hip_atomic_compare_exchange_strong(&c->srv_state, &st, nst, __ATOMIC_ACQ_REL, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT))
+        return NO_PAGE;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
     return ld_sys(&P.srv->sub[h % SUB_CAP].slot);
 }
@@ -1326,7 +1344,6 @@ __device__ void admit(const KParams &P, uint32_t slot, unsigned long long *stats
         Q.max_pages = ld_sys(&qi->max_pages);
         Q.pad = ld_sys(&qi->ticket);  // ticket, echoed in the completion
         Q.qpending = 1;               // held until every root chunk is queued
-        atom_add(&c->n_active, 1);
     }
     __syncthreads();
     // publish the plan and the query state; also drop this CU's L1 copies of
@@ -1348,7 +1365,7 @@ __device__ void admit(const KParams &P, uint32_t slot, unsigned long long *stats
         push_tasks(P, lvl, m, t, ROOT_BANK, stats);
     }
     if (lane == 0) {
-        st_sys(&P.srv->admitted, ld_rlx(&c->sub_head));
+        st_sys(&P.srv->admitted, srv_head(ld_rlx(&c->srv_state)));
         st_sys(&P.srv->live_banks, (uint32_t)max(0, ld_rlx(&c->live_banks)));
         if (atom_add(&Q.qpending, -1ll) == 1) complete_query(P, slot);
     }
@@ -1391,11 +1408,14 @@ __global__ __launch_bounds__(WAVE) void fm_match_kernel(KParams P) {
             if (ld_rlx(&P.ctl->abort)) {
                 flag = 2;
             } else if (wall_clock64() > deadline) {
-                timeout_at(P.ctl, 1, (uint32_t)ld_rlx(&P.ctl->n_active), (uint32_t)ld_rlx(&P.ctl->pending), idle);
+                timeout_at(P.ctl, 1, srv_active(ld_rlx(&P.ctl->srv_state)), (uint32_t)ld_rlx(&P.ctl->pending), idle);
                 flag = 2;
             } else if (wall_clock64() > retire_at) {
                 // serving: leave between tasks; queued tasks, admitted
-                // queries and banks stay on the device for the next launch
+                // queries and banks stay on the device for the next launch.
+                // Closing stops admissions, so no query starts on the waves
+                // still finishing their tasks.
+                __hip_atomic_fetch_or(&P.ctl->srv_state, SRV_CLOSING, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 flag = 2;
             } else {
                 TaskDesc t;
@@ -1410,11 +1430,27 @@ __global__ __launch_bounds__(WAVE) void fm_match_kernel(KParams P) {
                         s_task.q = slot;
                         flag = 3;
                     } else if (ld_sys(&P.srv->abort)) {
-                        timeout_at(P.ctl, 3, (uint32_t)ld_rlx(&P.ctl->n_active), 0, 0);
+                        timeout_at(P.ctl, 3, srv_active(ld_rlx(&P.ctl->srv_state)), 0, 0);
                         flag = 2;
-                    } else if (ld_rlx(&P.ctl->n_active) <= 0 &&
-                               (ld_sys(&P.srv->shutdown) || (idle_since && wall_clock64() - idle_since > P.idle_ticks))) {
-                        flag = 2;
+                    } else {
+                        // The launch ends as a whole: a wave that has been
+                        // idle long enough closes it, but only while no query
+                        // is active and no submission waits (one CAS on the
+                        // state word). Waves leave once it is closed, so a
+                        // query is never admitted into a launch whose other
+                        // waves have already left.
+                        unsigned long long st = ld_rlx(&P.ctl->srv_state);
+                        if (st & SRV_CLOSING) {
+                            flag = 2;
+                        } else if (srv_active(st) == 0 &&
+                                   (ld_sys(&P.srv->shutdown) ||
+                                    (idle_since && wall_clock64() - idle_since > P.idle_ticks)) &&
+                                   ld_sys(&P.srv->sub_tail) == srv_head(st) &&
+                                   __hip_atomic_compare_exchange_strong(&P.ctl->srv_state, &st, st | SRV_CLOSING,
+                                                                        __ATOMIC_ACQ_REL, __ATOMIC_RELAXED,
+                                                                        __HIP_MEMORY_SCOPE_AGENT)) {
+                            flag = 2;
+                        }
                     }
                 } else if (ld_rlx(&P.ctl->pending) <= 0) {
                     flag = 2;
@@ -2499,6 +2535,16 @@ static int server_launch(fm_server *s) {
     Ctl reset{};
     reset.timeout_ticks = (unsigned long long)(s->cfg.max_launch_ms * (e->wall_hz / 1000.0));
     HIP_TRY(hipMemcpyAsync(e->ctl, &reset, CTL_RESET_BYTES, hipMemcpyHostToDevice, e->stream));
+    // the previous launch (finished: its event was reached) left the state
+    // word closed; reopen it for this one
+    unsigned long long st = 0;
+    HIP_TRY(hipMemcpyAsync(&st, &e->ctl->srv_state, sizeof st, hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    if (st & SRV_CLOSING) {
+        st &= ~SRV_CLOSING;
+        HIP_TRY(hipMemcpyAsync(&e->ctl->srv_state, &st, sizeof st, hipMemcpyHostToDevice, e->stream));
+        HIP_TRY(hipStreamSynchronize(e->stream));
+    }
     KParams P{};
     const fm_graph *g = s->g;
     P.g.V = g->h.V;
