@@ -229,6 +229,7 @@ struct DGraph {
     uint32_t L;
     const uint32_t *ptr[2];  // row pointers into col (in-rows offset by E)
     const uint32_t *col;
+    const uint32_t *skip;  // col[b * 32] for every block b (membership probes)
     const uint32_t *roots;
 };
 
@@ -344,37 +345,74 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
     return acc;
 }
 
-// Words after the last neighbour that a window load of row_contains may
-// read; the device column array is allocated (and zeroed) with this padding.
-constexpr uint32_t COL_PAD = 16;
+// Membership probes read fixed windows past their row: the device column
+// array and its skip index are allocated (and zeroed) with this many words
+// of padding.
+constexpr uint32_t COL_PAD = 64;
+// Skip index: skip[b] = col[b * 16], the first word of every 16-word block of
+// the column array (1/16 of its size).
+constexpr uint32_t SKIP_SHIFT = 4;
+constexpr uint32_t SKIP_BLOCK = 1u << SKIP_SHIFT;
 
-// Membership of x in the sorted row col[st, st + len). Binary steps narrow
-// the position where x would sit to at most 13 places; those lie in one
-// 16-word window starting at a 16-byte boundary, read with four independent
-// 16-byte loads and compared in registers. The search is a chain of
-// dependent loads, so this replaces its last 3-4 round trips by one.
-__device__ __forceinline__ bool row_contains(const uint32_t *col, uint32_t st, uint32_t len, uint32_t x) {
-    // invariant: the first position whose value is >= x lies in [lo, lo + n]
-    uint32_t lo = st, n = len;
-    while (n > 12) {
-        const uint32_t half = n >> 1;
-        if (col[lo + half] < x) {
-            lo += half + 1;
-            n -= half + 1;
-        } else {
-            n = half;
+// The five 16-byte words of a[w0, w0 + 20) for w0 = lo rounded down to 4
+// words: a window that holds any 17 consecutive positions from lo. All five
+// loads are issued before any is used (one round trip).
+struct Window20 {
+    uint4 q[5];
+};
+__device__ __forceinline__ Window20 load_window(const uint32_t *a, uint32_t lo) {
+    const uint4 *wp = reinterpret_cast<const uint4 *>(a + (lo & ~3u));
+    Window20 w;
+#pragma unroll
+    for (uint32_t i = 0; i < 5; i++) w.q[i] = wp[i];
+    return w;
+}
+__device__ __forceinline__ uint32_t window_word(const Window20 &w, uint32_t j) {
+    const uint4 &q = w.q[j >> 2];
+    return (j & 3) == 0 ? q.x : (j & 3) == 1 ? q.y : (j & 3) == 2 ? q.z : q.w;
+}
+
+// Membership of x in the sorted, strictly increasing row col[st, st + len).
+// A row of at most 16 words is one window. A longer row first counts, in one
+// window of the skip index, the blocks starting inside the row whose first
+// word is <= x (binary steps on the skip index first narrow rows of more
+// than 17 blocks): the last such block, or the row's first block, is the
+// only one that can hold x, and a second window reads it. So a probe costs
+// two round trips up to 272 words, plus one per doubling past that on the
+// small skip index, instead of a chain of ~log2(len) dependent loads.
+__device__ __forceinline__ bool row_contains(const uint32_t *col, const uint32_t *skip, uint32_t st, uint32_t len,
+                                             uint32_t x) {
+    if (len == 0) return false;
+    uint32_t lo = st, end = st + len;
+    if (len > SKIP_BLOCK) {
+        const uint32_t b0 = st >> SKIP_SHIFT, b1 = (end - 1) >> SKIP_SHIFT;
+        // blocks b0+1 .. b1 start inside the row; those below blo hold a
+        // first word <= x, those at or past blo + bn one > x
+        uint32_t blo = b0 + 1, bn = b1 - b0;
+        while (bn > 16) {
+            const uint32_t half = bn >> 1;
+            if (skip[blo + half] <= x) {
+                blo += half + 1;
+                bn -= half + 1;
+            } else {
+                bn = half;
+            }
         }
+        const Window20 w = load_window(skip, blo);
+        const uint32_t s0 = blo & ~3u;
+        uint32_t c = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < 20; j++) c += (s0 + j - blo < bn && window_word(w, j) <= x) ? 1u : 0u;
+        const uint32_t bucket = blo - 1 + c;
+        lo = max(st, bucket << SKIP_SHIFT);
+        end = min(end, (bucket + 1) << SKIP_SHIFT);
     }
-    // [lo, lo + 12] lies in [w0, w0 + 16); lo <= st + len, so the window ends
-    // within COL_PAD words past the array
+    // x, if present, lies in [lo, end), at most 16 positions
+    const Window20 w = load_window(col, lo);
     const uint32_t w0 = lo & ~3u;
-    const uint4 *wp = reinterpret_cast<const uint4 *>(col + w0);
-    const uint4 q0 = wp[0], q1 = wp[1], q2 = wp[2], q3 = wp[3];
-    const uint32_t v[16] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w,
-                            q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
     bool hit = false;
 #pragma unroll
-    for (uint32_t j = 0; j < 16; j++) hit |= (w0 + j - st < len) && v[j] == x;
+    for (uint32_t j = 0; j < 20; j++) hit |= (w0 + j - lo < end - lo) && window_word(w, j) == x;
     return hit;
 }
 
@@ -942,7 +980,7 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
             for (uint32_t i = 0; i < pl.nself[0] && ok; i++) {
                 uint32_t st, len;
                 row_of(g, pl.req[pl.req_begin[0] + i], v, st, len);
-                ok = row_contains(g.col, st, len, v);
+                ok = row_contains(g.col, g.skip, st, len, v);
             }
             unsigned long long mask = __ballot(ok);
             uint32_t cnt = __popcll(mask);
@@ -1010,7 +1048,7 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
             for (uint32_t i = 0; i < pl.nself[0] && alive; i++) {
                 uint32_t st, len;
                 row_of(g, pl.req[pl.req_begin[0] + i], v, st, len);
-                alive = row_contains(g.col, st, len, v);
+                alive = row_contains(g.col, g.skip, st, len, v);
             }
         }
         if (lane < nb) {
@@ -1185,13 +1223,13 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
                 const uint32_t dx = s.didx[pi];
                 for (uint32_t i = 0; i < k && ok; i++)
                     if (i != dx) {
-                        ok = row_contains(g.col, rows_st[pi * k + i], rows_len[pi * k + i], x);
+                        ok = row_contains(g.col, g.skip, rows_st[pi * k + i], rows_len[pi * k + i], x);
                         nprobe++;
                     }
                 for (uint32_t i = 0; i < nself && ok; i++) {
                     uint32_t st, len;
                     row_of(g, pl.req[rb + k + i], x, st, len);
-                    ok = row_contains(g.col, st, len, x);
+                    ok = row_contains(g.col, g.skip, st, len, x);
                 }
             }
             if (P.instrument) {
@@ -1993,6 +2031,17 @@ int fm_graph_upload(fm_graph *g) {
     if ((rc = dalloc(&g->d_col[0], ncol + COL_PAD))) return rc;
     HIP_TRY(hipMemcpy(g->d_col[0], h.out.col.data(), E * 4, hipMemcpyHostToDevice));
     HIP_TRY(hipMemset(g->d_col[0] + ncol, 0, COL_PAD * 4));
+    {
+        // skip index over the device column array (out-rows, then in-rows)
+        const uint64_t nskip = (ncol + (1u << SKIP_SHIFT) - 1) >> SKIP_SHIFT;
+        std::vector<uint32_t> skip(nskip + COL_PAD, 0u);
+        for (uint64_t b = 0; b < nskip; b++) {
+            const uint64_t i = b << SKIP_SHIFT;
+            skip[b] = i < E ? h.out.col[i] : h.in.col[i - E];
+        }
+        if ((rc = dalloc(&g->d_skip, skip.size()))) return rc;
+        HIP_TRY(hipMemcpy(g->d_skip, skip.data(), skip.size() * 4, hipMemcpyHostToDevice));
+    }
     uint64_t nroots = h.out.roots.size() + (h.symmetric ? 0 : h.in.roots.size());
     if ((rc = dalloc(&g->d_roots[0], nroots))) return rc;
     HIP_TRY(hipMemcpy(g->d_roots[0], h.out.roots.data(), h.out.roots.size() * 4, hipMemcpyHostToDevice));
@@ -2047,6 +2096,7 @@ void fm_graph_destroy(fm_graph *g) {
         (void)hipFree(g->d_ptr[0]);
         if (g->d_ptr[1] != g->d_ptr[0]) (void)hipFree(g->d_ptr[1]);
         (void)hipFree(g->d_col[0]);
+        (void)hipFree(g->d_skip);
         (void)hipFree(g->d_roots[0]);
     }
     delete g;
@@ -2319,6 +2369,7 @@ int fm_match_batch(fm_engine *e, const fm_graph *g, const fm_query *const *qs, u
     P.g.ptr[0] = g->d_ptr[0];
     P.g.ptr[1] = g->d_ptr[1];
     P.g.col = g->d_col[0];
+    P.g.skip = g->d_skip;
     P.g.roots = g->d_roots[0];
     P.plans = e->d_plans;
     P.qs = e->d_qs;
@@ -2552,6 +2603,7 @@ static int server_launch(fm_server *s) {
     P.g.ptr[0] = g->d_ptr[0];
     P.g.ptr[1] = g->d_ptr[1];
     P.g.col = g->d_col[0];
+    P.g.skip = g->d_skip;
     P.g.roots = g->d_roots[0];
     P.plans = e->d_plans;
     P.plans_rw = e->d_plans;
