@@ -1089,6 +1089,26 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
         }
         __syncthreads();
         // ---- candidates, 64 per step ----
+        // Each step's candidate word (and its parent) is loaded during the
+        // step before, so the load overlaps that step's probes. The position
+        // is clamped to the batch's last candidate, so the load is always in
+        // bounds and unconditional (no loop-carried value merged at a
+        // branch); T only shrinks, so a prefetched position below the
+        // current T is that step's candidate.
+        uint32_t pf_pi = 0, pf_x = 0;
+        auto prefetch = [&](uint32_t from) {
+            const uint32_t tt = min(from + lane, T - 1);
+            uint32_t lo = 0, hi = nb - 1;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (s.cum[mid] > tt) hi = mid;
+                else lo = mid + 1;
+            }
+            pf_pi = lo;
+            const uint32_t excl = lo ? s.cum[lo - 1] : 0;
+            pf_x = g.col[s.dstart[lo] + (tt - excl)];
+        };
+        if (T) prefetch(0);
         for (uint32_t base = 0, step = 0; base < T; base += WAVE, step++) {
             // Work splitting: when waves are idle, donate half of what is
             // left: the upper half of the remaining parents when at least
@@ -1202,17 +1222,9 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
             const uint32_t tt = base + lane;
             uint32_t nprobe = 0;
             bool ok = false;
-            uint32_t x = 0, pi = 0;
+            const uint32_t x = pf_x, pi = pf_pi;
+            prefetch(base + WAVE);
             if (tt < T) {
-                uint32_t lo = 0, hi = nb - 1;
-                while (lo < hi) {
-                    uint32_t mid = (lo + hi) >> 1;
-                    if (s.cum[mid] > tt) hi = mid;
-                    else lo = mid + 1;
-                }
-                pi = lo;
-                uint32_t excl = pi ? s.cum[pi - 1] : 0;
-                x = g.col[s.dstart[pi] + (tt - excl)];
                 ok = true;
                 const uint32_t *prow = &par[pi * W];
                 for (uint32_t a = 0; a < W; a++) ok &= prow[a] != x;
