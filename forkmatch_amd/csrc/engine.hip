@@ -542,9 +542,6 @@ constexpr uint32_t ALL_CANDIDATES = 0xFFFFFFFFu;
 #ifndef FM_IDLE_BACKOFF_LOG2
 #define FM_IDLE_BACKOFF_LOG2 5
 #endif
-#ifndef FM_IDLE_BACKOFF_LOG2_UPPER
-#define FM_IDLE_BACKOFF_LOG2_UPPER 10
-#endif
 constexpr uint32_t QSLOT_MASK = 0xFFFFFFu;
 __device__ __forceinline__ uint32_t qslot(uint32_t q) { return q & QSLOT_MASK; }
 struct TaskDesc {
@@ -1525,13 +1522,9 @@ __global__ __launch_bounds__(WAVE) void fm_match_kernel(KParams P) {
                 atom_add(&P.ctl->hungry[blockIdx.x % HUNGRY_SHARDS].n, 1);
                 idle_since = wall_clock64();
             }
-            // Exponential back-off keeps idle waves off the queue counters.
-            // The upper half of the grid backs off further: it joins work
-            // that persists and otherwise stays off the counters that the
-            // busy waves update.
-            const uint32_t cap = blockIdx.x * 2 >= gridDim.x ? (uint32_t)FM_IDLE_BACKOFF_LOG2_UPPER
-                                                             : (uint32_t)FM_IDLE_BACKOFF_LOG2;
-            for (uint32_t i = 0, lim = 1u << min(idle, cap); i < lim; i++) __builtin_amdgcn_s_sleep(8);
+            // exponential back-off keeps idle waves off the queue counters
+            for (uint32_t i = 0, lim = 1u << min(idle, (uint32_t)FM_IDLE_BACKOFF_LOG2); i < lim; i++)
+                __builtin_amdgcn_s_sleep(8);
             idle++;
             continue;
         }
