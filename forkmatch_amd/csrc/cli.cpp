@@ -38,7 +38,10 @@ int main(int argc, char **argv) {
     printf("Query Graph - %s Data Graph - %s\n", query_path, data_path);
     // same order as the reference's match(): data graph, then query
     fm_graph *g = nullptr;
-    int rc = fm_graph_parse_g(data_path, &g);
+    // The data graph is parsed on the device, as in the reference. Without a
+    // visible device the host parser checks the file, so the file errors
+    // keep their exit codes (matching then stops at engine creation).
+    int rc = fm_device_count() > 0 ? fm_graph_load_g(data_path, &g) : fm_graph_parse_g(data_path, &g);
     if (rc) return die(rc);
     fm_query *q = nullptr;
     rc = fm_query_load_g(query_path, &q);

@@ -1899,6 +1899,118 @@ __global__ __launch_bounds__(IB) void fm_isect_kernel(const uint32_t *__restrict
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// .g data file parser on the device (the reference parses on the GPU too:
+// parseCSR, trie.cuh:800-951). The host checks the frame and header and
+// uploads the body, lines of "v id label\n" / "e src dst label\n":
+//  - fm_g_count_kernel: newlines per 4 KiB chunk (a 256-thread block, 16
+//    bytes per thread); the host scans the chunk counts;
+//  - fm_g_index_kernel: the start of every line;
+//  - fm_g_parse_kernel: one thread per line, the host parser's checks; the
+//    first malformed line (smallest index) is kept with an atomic minimum;
+//  - fm_g_edges_kernel: the edge lines' values compacted in file order.
+constexpr uint32_t GCHUNK = 4096;
+
+// Bytes equal to '\n' in a 32-bit word (exact: no borrow between bytes).
+__device__ __forceinline__ uint32_t newlines_in(uint32_t w) {
+    const uint32_t x = w ^ 0x0A0A0A0Au;
+    const uint32_t t = ~(((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x | 0x7F7F7F7Fu);
+    return __popc(t);
+}
+
+__global__ __launch_bounds__(IB) void fm_g_count_kernel(const uint4 *body, uint64_t nchunks, uint32_t *counts) {
+    __shared__ uint32_t s_wave[IB / WAVE];
+    for (uint64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+        const uint4 v = body[c * IB + threadIdx.x];
+        const uint32_t n = newlines_in(v.x) + newlines_in(v.y) + newlines_in(v.z) + newlines_in(v.w);
+        uint32_t tot;
+        block_excl_scan(n, s_wave, tot);
+        if (threadIdx.x == 0) counts[c] = tot;
+    }
+}
+
+__global__ __launch_bounds__(IB) void fm_g_index_kernel(const uint4 *body, uint64_t nchunks, const uint64_t *chunk_off,
+                                                        uint64_t *starts) {
+    __shared__ uint32_t s_wave[IB / WAVE];
+    for (uint64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+        const uint4 v = body[c * IB + threadIdx.x];
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+        uint32_t n = 0;
+#pragma unroll
+        for (int i = 0; i < 4; i++) n += newlines_in(w[i]);
+        uint32_t tot;
+        uint64_t k = chunk_off[c] + block_excl_scan(n, s_wave, tot);
+        const uint64_t pos0 = (c * IB + threadIdx.x) * 16;
+#pragma unroll
+        for (int i = 0; i < 16; i++)
+            if (((w[i >> 2] >> (8 * (i & 3))) & 0xFFu) == '\n') starts[1 + k++] = pos0 + i + 1;
+    }
+}
+
+__device__ __forceinline__ bool g_parse_uint(const char *body, uint64_t &p, uint64_t end, uint32_t &v) {
+    if (p >= end || body[p] < '0' || body[p] > '9') return false;
+    unsigned long long x = 0;
+    while (p < end && body[p] >= '0' && body[p] <= '9') {
+        x = x * 10 + (unsigned long long)(body[p] - '0');
+        if (x > 0xFFFFFFFFull) return false;
+        p++;
+    }
+    v = (uint32_t)x;
+    return true;
+}
+
+// Line i is body[starts[i], starts[i + 1] - 1) followed by its newline.
+__global__ __launch_bounds__(IB) void fm_g_parse_kernel(const char *body, const uint64_t *starts, uint64_t nlines,
+                                                        uint32_t V, uint8_t *is_edge, uint32_t *vals,
+                                                        unsigned long long *first_bad) {
+    for (uint64_t i = (uint64_t)blockIdx.x * IB + threadIdx.x; i < nlines; i += (uint64_t)gridDim.x * IB) {
+        uint64_t p = starts[i];
+        const uint64_t end = starts[i + 1] - 1;  // the newline
+        const char kind = p < end ? body[p] : '\n';
+        bool ok = (kind == 'v' || kind == 'e') && p + 1 < end && body[p + 1] == ' ';
+        uint32_t a = 0, b = 0, c = 0;
+        if (ok) {
+            p += 2;
+            ok = g_parse_uint(body, p, end, a) && p < end && body[p++] == ' ' && g_parse_uint(body, p, end, b);
+            if (ok && kind == 'e') ok = p < end && body[p++] == ' ' && g_parse_uint(body, p, end, c);
+            ok = ok && p == end;
+            if (ok && kind == 'v') ok = a < V;
+        }
+        if (!ok) {
+            atomicMin(first_bad, (unsigned long long)i);
+            is_edge[i] = 0;
+            continue;
+        }
+        is_edge[i] = kind == 'e' ? 1 : 0;
+        vals[3 * i] = a;
+        vals[3 * i + 1] = b;
+        vals[3 * i + 2] = c;
+    }
+}
+
+// Edge lines per block of IB lines (counts), then their values scattered to
+// src/dst/lab at the block's offset (edge_off) in line order.
+__global__ __launch_bounds__(IB) void fm_g_edges_kernel(const uint8_t *is_edge, const uint32_t *vals, uint64_t nlines,
+                                                        uint64_t nblocks, const uint64_t *edge_off, uint32_t *counts,
+                                                        uint32_t *src, uint32_t *dst, uint32_t *lab) {
+    __shared__ uint32_t s_wave[IB / WAVE];
+    for (uint64_t blk = blockIdx.x; blk < nblocks; blk += gridDim.x) {
+        const uint64_t i = blk * IB + threadIdx.x;
+        const uint32_t e = i < nlines ? is_edge[i] : 0u;
+        uint32_t tot;
+        const uint32_t pre = block_excl_scan(e, s_wave, tot);
+        if (!edge_off) {
+            if (threadIdx.x == 0) counts[blk] = tot;
+        } else if (e) {
+            const uint64_t k = edge_off[blk] + pre;
+            src[k] = vals[3 * i];
+            dst[k] = vals[3 * i + 1];
+            lab[k] = vals[3 * i + 2];
+        }
+    }
+}
+
 }  // namespace fm
 
 // ===========================================================================
@@ -2070,15 +2182,147 @@ int fm_graph_upload(fm_graph *g) {
     return FM_OK;
 }
 
-int fm_graph_load_g(const char *path, fm_graph **out) {
-    int rc = fm_graph_parse_g(path, out);
-    if (rc) return rc;
-    rc = fm_graph_upload(*out);
-    if (rc) {
-        fm_graph_destroy(*out);
-        *out = nullptr;
+}  // extern "C"
+
+// Parses the body of a .g data file on the current device into edge lists
+// (in file order). Errors carry the host parser's codes and messages: the
+// first malformed line found on the device is parsed again on the host for
+// the message.
+static int gpu_parse_data(const std::string &buf, const DataFrame &f, std::vector<uint32_t> &src,
+                          std::vector<uint32_t> &dst, std::vector<uint32_t> &lab) {
+    const uint64_t nbody = f.body_end - f.body_begin;
+    if (nbody == 0) return FM_OK;
+    const uint64_t nchunks = (nbody + GCHUNK - 1) / GCHUNK;
+    std::vector<void *> owned;
+    struct Free {
+        std::vector<void *> &v;
+        ~Free() {
+            for (void *p : v) (void)hipFree(p);
+        }
+    } guard{owned};
+    auto dalloc = [&](void **p, uint64_t bytes) -> int {
+        HIP_TRY(hipMalloc(p, std::max<uint64_t>(bytes, 16)));
+        owned.push_back(*p);
+        return FM_OK;
+    };
+    int rc;
+    char *d_body = nullptr;
+    if ((rc = dalloc((void **)&d_body, nchunks * GCHUNK))) return rc;
+    HIP_TRY(hipMemset(d_body + nbody, 0, nchunks * GCHUNK - nbody));
+    HIP_TRY(hipMemcpy(d_body, buf.data() + f.body_begin, nbody, hipMemcpyHostToDevice));
+    const unsigned grid = (unsigned)std::min<uint64_t>(nchunks, 65536);
+    // newlines per chunk, their offsets (host scan), the line starts
+    uint32_t *d_counts = nullptr;
+    if ((rc = dalloc((void **)&d_counts, nchunks * 4))) return rc;
+    hipLaunchKernelGGL(fm_g_count_kernel, dim3(grid), dim3(IB), 0, 0, (const uint4 *)d_body, nchunks, d_counts);
+    HIP_TRY(hipGetLastError());
+    std::vector<uint32_t> counts(nchunks);
+    HIP_TRY(hipMemcpy(counts.data(), d_counts, nchunks * 4, hipMemcpyDeviceToHost));
+    std::vector<uint64_t> off(nchunks);
+    uint64_t nl = 0;
+    for (uint64_t c = 0; c < nchunks; c++) {
+        off[c] = nl;
+        nl += counts[c];
     }
-    return rc;
+    // bytes after the last newline form an unterminated (malformed) line
+    const bool tail = buf[f.body_end - 1] != '\n';
+    uint64_t *d_off = nullptr, *d_starts = nullptr;
+    if ((rc = dalloc((void **)&d_off, nchunks * 8))) return rc;
+    if ((rc = dalloc((void **)&d_starts, (nl + 1) * 8))) return rc;
+    HIP_TRY(hipMemcpy(d_off, off.data(), nchunks * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(d_starts, 0, 8));
+    hipLaunchKernelGGL(fm_g_index_kernel, dim3(grid), dim3(IB), 0, 0, (const uint4 *)d_body, nchunks,
+                       (const uint64_t *)d_off, d_starts);
+    HIP_TRY(hipGetLastError());
+    // parse every complete line
+    uint8_t *d_is_edge = nullptr;
+    uint32_t *d_vals = nullptr;
+    unsigned long long *d_bad = nullptr;
+    if ((rc = dalloc((void **)&d_is_edge, nl))) return rc;
+    if ((rc = dalloc((void **)&d_vals, nl * 12))) return rc;
+    if ((rc = dalloc((void **)&d_bad, 8))) return rc;
+    HIP_TRY(hipMemset(d_bad, 0xFF, 8));
+    const uint64_t nblocks = (nl + IB - 1) / IB;
+    const unsigned lgrid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(nblocks, 65536));
+    if (nl) {
+        hipLaunchKernelGGL(fm_g_parse_kernel, dim3(lgrid), dim3(IB), 0, 0, (const char *)d_body,
+                           (const uint64_t *)d_starts, nl, f.V, d_is_edge, d_vals, d_bad);
+        HIP_TRY(hipGetLastError());
+    }
+    unsigned long long bad = ~0ull;
+    HIP_TRY(hipMemcpy(&bad, d_bad, 8, hipMemcpyDeviceToHost));
+    if (bad != ~0ull || tail) {
+        // the first bad line: its start is the end of the line before
+        uint64_t i = bad != ~0ull ? bad : nl, start = 0;
+        if (i > 0) HIP_TRY(hipMemcpy(&start, d_starts + i, 8, hipMemcpyDeviceToHost));
+        const char *p = buf.data() + f.body_begin + start, *stop = buf.data() + f.body_end;
+        char kind;
+        uint32_t vals[3];
+        rc = parse_data_line(p, stop, f.V, i + 3, kind, vals);
+        return rc ? rc : fail(FM_ERR_INTERNAL, "device parser rejected line %llu", (unsigned long long)(i + 3));
+    }
+    // edge lines, compacted in file order
+    uint32_t *d_ecount = nullptr;
+    uint64_t *d_eoff = nullptr;
+    if ((rc = dalloc((void **)&d_ecount, nblocks * 4))) return rc;
+    if ((rc = dalloc((void **)&d_eoff, nblocks * 8))) return rc;
+    if (nl) {
+        hipLaunchKernelGGL(fm_g_edges_kernel, dim3(lgrid), dim3(IB), 0, 0, (const uint8_t *)d_is_edge,
+                           (const uint32_t *)d_vals, nl, nblocks, (const uint64_t *)nullptr, d_ecount, nullptr,
+                           nullptr, nullptr);
+        HIP_TRY(hipGetLastError());
+    }
+    std::vector<uint32_t> ecount(nblocks);
+    if (nblocks) HIP_TRY(hipMemcpy(ecount.data(), d_ecount, nblocks * 4, hipMemcpyDeviceToHost));
+    std::vector<uint64_t> eoff(nblocks);
+    uint64_t ne = 0;
+    for (uint64_t b = 0; b < nblocks; b++) {
+        eoff[b] = ne;
+        ne += ecount[b];
+    }
+    uint32_t *d_e = nullptr;
+    if ((rc = dalloc((void **)&d_e, ne * 12))) return rc;
+    if (nblocks) HIP_TRY(hipMemcpy(d_eoff, eoff.data(), nblocks * 8, hipMemcpyHostToDevice));
+    if (ne) {
+        hipLaunchKernelGGL(fm_g_edges_kernel, dim3(lgrid), dim3(IB), 0, 0, (const uint8_t *)d_is_edge,
+                           (const uint32_t *)d_vals, nl, nblocks, (const uint64_t *)d_eoff, d_ecount, d_e,
+                           d_e + ne, d_e + 2 * ne);
+        HIP_TRY(hipGetLastError());
+    }
+    src.resize(ne);
+    dst.resize(ne);
+    lab.resize(ne);
+    if (ne) {
+        HIP_TRY(hipMemcpy(src.data(), d_e, ne * 4, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(dst.data(), d_e + ne, ne * 4, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(lab.data(), d_e + 2 * ne, ne * 4, hipMemcpyDeviceToHost));
+    }
+    return FM_OK;
+}
+
+extern "C" {
+
+int fm_graph_load_g(const char *path, fm_graph **out) {
+    if (!path || !out) return fail(FM_ERR_INVALID_ARG, "null argument");
+    std::string buf;
+    if (!read_file(path, buf)) return fail(FM_ERR_DATA_MISSING_FILE, "missing file %s", path);
+    DataFrame f;
+    int rc = parse_data_frame(buf, f);
+    if (rc) return rc;
+    std::vector<uint32_t> src, dst, lab;
+    if ((rc = gpu_parse_data(buf, f, src, dst, lab))) return rc;
+    buf.clear();
+    buf.shrink_to_fit();
+    auto *g = new fm_graph();
+    g->h.header_edges = f.header_edges;
+    rc = build_graph(f.V, src.size(), src.data(), dst.data(), lab.data(), g->h);
+    if (!rc) rc = fm_graph_upload(g);
+    if (rc) {
+        fm_graph_destroy(g);
+        return rc;
+    }
+    *out = g;
+    return FM_OK;
 }
 
 int fm_graph_copy_device_csr(const fm_graph *g, int dir, uint32_t *ptr, uint32_t *col) {

@@ -52,6 +52,19 @@ struct HostGraph {
 // Builds a HostGraph from directed (src, dst, label) lines. Duplicate lines
 // are collapsed. Returns FM_OK or a status. A caller that wrote every line
 // in both directions (the generators) skips the symmetry check.
+// The .g data file's frame (trailing "t # -1", header): the body is
+// buf[body_begin, body_end), lines of "v id label\n" / "e src dst label\n".
+struct DataFrame {
+    uint32_t V = 0, header_edges = 0;
+    size_t body_begin = 0, body_end = 0;
+};
+int parse_data_frame(const std::string &buf, DataFrame &f);
+// One body line starting at p (advanced past its newline); `line` is its
+// 1-based line number in the file, for the error message.
+int parse_data_line(const char *&p, const char *stop, uint32_t V, uint64_t line, char &kind, uint32_t vals[3]);
+
+int parse_data_g(const std::string &buf, HostGraph &g);
+
 int build_graph(uint32_t V, uint64_t n, const uint32_t *src, const uint32_t *dst,
                 const uint32_t *label, HostGraph &g,
                 bool symmetric_by_construction = false);

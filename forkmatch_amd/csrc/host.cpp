@@ -285,7 +285,7 @@ static bool parse_uint(const char *&p, const char *end, uint32_t &v) {
     return true;
 }
 
-int parse_data_g(const std::string &buf, HostGraph &g) {
+int parse_data_frame(const std::string &buf, DataFrame &f) {
     size_t size = buf.size();
     if (size < 6 || std::memcmp(buf.data(), "t # 0\n", 6) != 0)
         return fail(FM_ERR_DATA_BAD_START, "data graph does not start with \"t # 0\\n\"");
@@ -320,35 +320,52 @@ int parse_data_g(const std::string &buf, HostGraph &g) {
         }
         if (!any) return fail(FM_ERR_DATA_BAD_HEADER, "empty data graph header");
     }
-    uint32_t V = header[0];
-    g.header_edges = header[1];
+    f.V = header[0];
+    f.header_edges = header[1];
+    f.body_begin = (size_t)(p - buf.data());
+    f.body_end = (size_t)(stop - buf.data());
+    return FM_OK;
+}
+
+int parse_data_line(const char *&p, const char *stop, uint32_t V, uint64_t line, char &kind, uint32_t vals[3]) {
+    kind = *p;
+    if ((kind != 'v' && kind != 'e') || p + 1 >= stop || p[1] != ' ')
+        return fail(FM_ERR_DATA_BAD_BODY, "data graph line %llu is not a v/e line", (unsigned long long)line);
+    p += 2;
+    uint32_t a, b, c = 0;
+    bool ok = parse_uint(p, stop, a) && p < stop && *p++ == ' ' && parse_uint(p, stop, b);
+    if (ok && kind == 'e') ok = p < stop && *p++ == ' ' && parse_uint(p, stop, c);
+    if (ok) ok = p < stop && *p++ == '\n';
+    if (!ok) return fail(FM_ERR_DATA_BAD_BODY, "malformed data graph line %llu", (unsigned long long)line);
+    if (kind == 'v' && a >= V)
+        return fail(FM_ERR_DATA_BAD_BODY, "vertex id %u >= %u on line %llu", a, V, (unsigned long long)line);
+    vals[0] = a;
+    vals[1] = b;
+    vals[2] = c;
+    return FM_OK;
+}
+
+int parse_data_g(const std::string &buf, HostGraph &g) {
+    DataFrame f;
+    int rc = parse_data_frame(buf, f);
+    if (rc) return rc;
+    const uint32_t V = f.V;
+    g.header_edges = f.header_edges;
+    const char *p = buf.data() + f.body_begin, *stop = buf.data() + f.body_end;
     std::vector<uint32_t> src, dst, lab;
-    src.reserve(header[1]);
-    dst.reserve(header[1]);
-    lab.reserve(header[1]);
+    src.reserve(f.header_edges);
+    dst.reserve(f.header_edges);
+    lab.reserve(f.header_edges);
     uint64_t line = 2;
     while (p < stop) {
         line++;
-        char kind = *p;
-        if ((kind != 'v' && kind != 'e') || p + 1 >= stop || p[1] != ' ')
-            return fail(FM_ERR_DATA_BAD_BODY, "data graph line %llu is not a v/e line",
-                        (unsigned long long)line);
-        p += 2;
-        uint32_t a, b, c = 0;
-        bool ok = parse_uint(p, stop, a) && p < stop && *p++ == ' ' && parse_uint(p, stop, b);
-        if (ok && kind == 'e') ok = p < stop && *p++ == ' ' && parse_uint(p, stop, c);
-        if (ok) ok = p < stop && *p++ == '\n';
-        if (!ok)
-            return fail(FM_ERR_DATA_BAD_BODY, "malformed data graph line %llu",
-                        (unsigned long long)line);
-        if (kind == 'v') {
-            if (a >= V)
-                return fail(FM_ERR_DATA_BAD_BODY, "vertex id %u >= %u on line %llu", a, V,
-                            (unsigned long long)line);
-        } else {
-            src.push_back(a);
-            dst.push_back(b);
-            lab.push_back(c);
+        char kind;
+        uint32_t vals[3];
+        if ((rc = parse_data_line(p, stop, V, line, kind, vals))) return rc;
+        if (kind == 'e') {
+            src.push_back(vals[0]);
+            dst.push_back(vals[1]);
+            lab.push_back(vals[2]);
         }
     }
     return build_graph(V, src.size(), src.data(), dst.data(), lab.data(), g);

@@ -109,7 +109,12 @@ int fm_graph_from_edges(uint32_t vertices, uint64_t nedges,
                         const uint32_t *label, fm_graph **out);
 /* Uploads the CSR to the current HIP device (idempotent). */
 int fm_graph_upload(fm_graph *g);
-/* Parse + upload in one call (the reference's parseGraph). */
+/* Load a .g data file onto the current HIP device (replaces parseGraph,
+ * trie.cuh:1051-1222, and parseCSR, trie.cuh:800-951). The host checks the
+ * frame and header; the body's lines are found and parsed on the device
+ * (one thread per line), with the same checks, error codes and messages as
+ * fm_graph_parse_g and 64-bit offsets throughout (no 2 GiB limit); the CSR
+ * is built on the host and uploaded. */
 int fm_graph_load_g(const char *path, fm_graph **out);
 int fm_graph_info_get(const fm_graph *g, fm_graph_info *out);
 /* Copies the host CSR. ptr has labels*vertices+1 entries (row (l,v) is

@@ -441,3 +441,61 @@ def test_symmetry_breaking_gives_the_full_table(engine, table):
         if table:
             assert np.array_equal(r.table(0), ref.table), name
         r.close()
+
+
+_GOOD_G = ("t # 0\n4 8 1 2\nv 0 1\nv 1 1\nv 2 0\nv 3 0\ne 0 1 1\ne 1 0 1\ne 1 2 2\ne 2 1 2\ne 2 3 1\ne 3 2 1\n"
+           "e 3 0 0\ne 0 3 0\nt # -1")
+
+
+def _same_graph(a, b):
+    ia, ib = a.info(), b.info()
+    assert (ia.vertices, ia.edges, ia.labels, ia.symmetric, ia.header_edges) == \
+        (ib.vertices, ib.edges, ib.labels, ib.symmetric, ib.header_edges)
+    for d in (0, 1):
+        for x, y in zip(a.csr(d), b.csr(d)):
+            assert np.array_equal(x, y)
+
+
+def test_device_g_loader_equals_host_parser(tmp_path):
+    """The .g body parsed on the device (line starts found per 4 KiB chunk,
+    one thread per line) gives the host parser's graph: a symmetric
+    multi-label graph, a directed one with vertex lines after edge lines,
+    and a file of ~120k lines crossing thousands of chunks."""
+    g1 = fm.Graph.erdos_renyi(1000, 4000, 8, seed=11)
+    p1 = tmp_path / "er.g"
+    g1.write_g(str(p1))
+    _same_graph(fm.Graph.load_g(str(p1)), fm.Graph.parse_g(str(p1)))
+    rng = np.random.default_rng(3)
+    V, E = 3000, 60000
+    s, d, lab = rng.integers(0, V, E), rng.integers(0, V, E), rng.integers(0, 40, E)
+    lines = [f"e {a} {b} {c}" for a, b, c in zip(s, d, lab)] + [f"v {v} 0" for v in range(V)]
+    p2 = tmp_path / "directed.g"
+    p2.write_bytes(("t # 0\n" + f"{V} {E} 1 40\n" + "\n".join(lines) + "\nt # -1").encode())
+    _same_graph(fm.Graph.load_g(str(p2)), fm.Graph.parse_g(str(p2)))
+    g3 = fm.Graph.rmat(14, 60000, 16, zipf_s=1.2, seed=5)
+    p3 = tmp_path / "rmat.g"
+    g3.write_g(str(p3))
+    _same_graph(fm.Graph.load_g(str(p3)), fm.Graph.parse_g(str(p3)))
+
+
+@pytest.mark.parametrize("text", [
+    _GOOD_G.replace("e 1 2 2", "e 1 2"),             # missing label
+    _GOOD_G.replace("e 1 2 2", "x 1 2 2"),           # not a v/e line
+    _GOOD_G.replace("e 1 2 2", "e 1 9 2"),           # dst out of range (found by the CSR build)
+    _GOOD_G.replace("v 2 0", "v 7 0"),               # vertex id >= V
+    _GOOD_G.replace("v 0 1", "v 0 99999999999"),     # number overflows u32, first body line
+    _GOOD_G.replace("e 3 2 1\n", "e 3 2 1\n\n"),     # empty line
+    _GOOD_G.replace("e 0 1 1", "e 0  1 1"),          # two spaces
+    _GOOD_G.replace("e 2 1 2", "e 2 1 2 ") ,         # trailing space
+    _GOOD_G.replace("e 0 3 0\nt # -1", "e 0 3 0t # -1"),  # last line unterminated
+])
+def test_device_g_loader_errors_equal_host_parser(tmp_path, text):
+    """Malformed bodies: the device loader reports the host parser's code
+    and message (the first bad line, with its line number)."""
+    p = tmp_path / "bad.g"
+    p.write_bytes(text.encode())
+    with pytest.raises(fm.ForkMatchError) as host:
+        fm.Graph.parse_g(str(p))
+    with pytest.raises(fm.ForkMatchError) as dev:
+        fm.Graph.load_g(str(p))
+    assert (dev.value.code, dev.value.message) == (host.value.code, host.value.message)
