@@ -116,6 +116,51 @@ static void parallel_for(uint64_t n, F f) {
     for (auto &x : th) x.join();
 }
 
+// f(i) for every item i < n, the items spread over the threads (for few,
+// large items such as labels).
+template <class F>
+static void parallel_for_items(uint64_t n, F f) {
+    const unsigned T = std::max<unsigned>(1, std::min<uint64_t>(hw_threads(), n));
+    std::atomic<uint64_t> next{0};
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < T; t++)
+        th.emplace_back([&] {
+            for (uint64_t i; (i = next.fetch_add(1)) < n;) f(i);
+        });
+    for (auto &x : th) x.join();
+}
+
+// out[r] = in[0] + ... + in[r - 1] for r <= n (out has n + 1 entries; out and
+// in may alias): per-range sums on all threads, a scan of the range sums,
+// then each range written from its offset. Returns the total.
+static uint64_t parallel_exclusive_scan(const uint32_t *in, uint32_t *out, uint64_t n) {
+    const unsigned T = hw_threads();
+    const uint64_t step = (n + T - 1) / std::max(1u, T);
+    std::vector<uint64_t> part(T + 1, 0);
+    parallel_for(T, [&](uint64_t b, uint64_t e) {
+        for (uint64_t t = b; t < e; t++) {
+            uint64_t acc = 0;
+            for (uint64_t r = t * step; r < std::min(n, (t + 1) * step); r++) acc += in[r];
+            part[t + 1] = acc;
+        }
+    });
+    for (unsigned t = 0; t < T; t++) part[t + 1] += part[t];
+    // T ranges on T threads (parallel_for would run a short range list serially)
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < T; t++)
+        th.emplace_back([&, t] {
+            uint64_t acc = part[t];
+            for (uint64_t r = t * step; r < std::min(n, (t + 1) * step); r++) {
+                const uint32_t v = in[r];
+                out[r] = (uint32_t)acc;
+                acc += v;
+            }
+        });
+    for (auto &x : th) x.join();
+    out[n] = (uint32_t)part[T];
+    return part[T];
+}
+
 // Counting sort of (row key -> value) pairs into a CSR with sorted,
 // de-duplicated rows. key(i) = lid[i] * V + from[i].
 static int build_dir(uint32_t V, uint32_t L, uint64_t n, const uint32_t *lid,
@@ -128,16 +173,13 @@ static int build_dir(uint32_t V, uint32_t L, uint64_t n, const uint32_t *lid,
         for (uint64_t i = b; i < e; i++) __atomic_fetch_add(&cnt[(uint64_t)lid[i] * V + from[i]], 1u, __ATOMIC_RELAXED);
     });
     c.ptr.assign(rows + 1, 0);
-    uint64_t acc = 0;
-    for (uint64_t r = 0; r < rows; r++) {
-        c.ptr[r] = (uint32_t)acc;
-        acc += cnt[r];
-    }
-    c.ptr[rows] = (uint32_t)acc;
+    parallel_exclusive_scan(cnt.data(), c.ptr.data(), rows);
     std::vector<uint32_t> tmp(n);
     {
         // reuse cnt as the running write cursor
-        for (uint64_t r = 0; r < rows; r++) cnt[r] = c.ptr[r];
+        parallel_for(rows, [&](uint64_t b, uint64_t e) {
+            std::memcpy(cnt.data() + b, c.ptr.data() + b, (size_t)(e - b) * 4);
+        });
         parallel_for(n, [&](uint64_t b, uint64_t e) {
             for (uint64_t i = b; i < e; i++)
                 tmp[__atomic_fetch_add(&cnt[(uint64_t)lid[i] * V + from[i]], 1u, __ATOMIC_RELAXED)] = to[i];
@@ -157,44 +199,46 @@ static int build_dir(uint32_t V, uint32_t L, uint64_t n, const uint32_t *lid,
         }
     });
     // compacted offsets (keep[] becomes the new row starts), then the copy
-    uint64_t w = 0;
-    for (uint64_t r = 0; r < rows; r++) {
-        const uint32_t k = keep[r];
-        keep[r] = (uint32_t)w;
-        w += k;
-    }
-    keep[rows] = (uint32_t)w;
+    const uint64_t w = parallel_exclusive_scan(keep.data(), keep.data(), rows);
     c.col.resize(w);
     parallel_for(rows, [&](uint64_t b, uint64_t e) {
         for (uint64_t r = b; r < e; r++)
             std::memcpy(c.col.data() + keep[r], tmp.data() + c.ptr[r], (size_t)(keep[r + 1] - keep[r]) * 4);
     });
     c.ptr.swap(keep);
-    // per-label root lists
+    // per-label root lists: counts per label, their offsets, then the lists
+    // (labels on all threads)
+    std::vector<uint32_t> nroot(L + 1, 0);
+    parallel_for_items(L, [&](uint64_t l) {
+        const uint32_t *p = c.ptr.data() + l * V;
+        uint32_t k = 0;
+        for (uint32_t v = 0; v < V; v++) k += p[v + 1] > p[v];
+        nroot[l] = k;
+    });
     c.root_ptr.assign(L + 1, 0);
-    c.roots.clear();
-    for (uint32_t l = 0; l < L; l++) {
-        c.root_ptr[l] = (uint32_t)c.roots.size();
-        const uint32_t *p = c.ptr.data() + (uint64_t)l * V;
+    for (uint32_t l = 0; l < L; l++) c.root_ptr[l + 1] = c.root_ptr[l] + nroot[l];
+    c.roots.assign(c.root_ptr[L], 0);
+    parallel_for_items(L, [&](uint64_t l) {
+        const uint32_t *p = c.ptr.data() + l * V;
+        uint32_t *dst = c.roots.data() + c.root_ptr[l];
         for (uint32_t v = 0; v < V; v++)
-            if (p[v + 1] > p[v]) c.roots.push_back(v);
-    }
-    c.root_ptr[L] = (uint32_t)c.roots.size();
+            if (p[v + 1] > p[v]) *dst++ = v;
+    });
     csr_stats(c, V, L);
     return FM_OK;
 }
 
 void csr_stats(Csr &c, uint32_t V, uint32_t L) {
     c.row_sq.assign(L, 0.0);
-    for (uint32_t l = 0; l < L; l++) {
-        const uint32_t *p = c.ptr.data() + (uint64_t)l * V;
+    parallel_for_items(L, [&](uint64_t l) {
+        const uint32_t *p = c.ptr.data() + l * V;
         double sq = 0;
         for (uint32_t v = 0; v < V; v++) {
             double k = (double)(p[v + 1] - p[v]);
             sq += k * k;
         }
         c.row_sq[l] = sq;
-    }
+    });
 }
 
 static bool row_has(const Csr &c, uint32_t V, uint32_t l, uint32_t v, uint32_t x) {
@@ -208,10 +252,22 @@ int build_graph(uint32_t V, uint64_t n, const uint32_t *src, const uint32_t *dst
     if (n >= (1ull << 32) - 1)
         return fail(FM_ERR_NOT_SUPPORTED, "graph has %llu directed edges; the CSR holds < 2^32",
                     (unsigned long long)n);
-    for (uint64_t i = 0; i < n; i++)
-        if (src[i] >= V || dst[i] >= V)
-            return fail(FM_ERR_DATA_BAD_BODY, "edge %llu (%u -> %u) names a vertex >= %u",
-                        (unsigned long long)i, src[i], dst[i], V);
+    // the first edge naming a vertex >= V, if any (ranges on all threads)
+    std::atomic<uint64_t> bad{n};
+    parallel_for(n, [&](uint64_t b, uint64_t e) {
+        for (uint64_t i = b; i < e && i < bad.load(std::memory_order_relaxed); i++)
+            if (src[i] >= V || dst[i] >= V) {
+                uint64_t cur = bad.load();
+                while (i < cur && !bad.compare_exchange_weak(cur, i)) {
+                }
+                break;
+            }
+    });
+    if (bad.load() < n) {
+        const uint64_t i = bad.load();
+        return fail(FM_ERR_DATA_BAD_BODY, "edge %llu (%u -> %u) names a vertex >= %u", (unsigned long long)i,
+                    src[i], dst[i], V);
+    }
     g.V = V;
     // dense label ids
     std::atomic<uint32_t> maxa{0};
@@ -227,7 +283,9 @@ int build_graph(uint32_t V, uint64_t n, const uint32_t *src, const uint32_t *dst
     g.label_values.clear();
     if (maxl < (1u << 24)) {
         std::vector<uint32_t> map((size_t)maxl + 1, UINT32_MAX);
-        for (uint64_t i = 0; i < n; i++) map[label[i]] = 0;
+        parallel_for(n, [&](uint64_t b, uint64_t e) {
+            for (uint64_t i = b; i < e; i++) __atomic_store_n(&map[label[i]], 0u, __ATOMIC_RELAXED);
+        });
         for (uint32_t v = 0; v <= maxl && n; v++)
             if (map[v] == 0) {
                 map[v] = (uint32_t)g.label_values.size();
