@@ -443,6 +443,46 @@ def test_symmetry_breaking_gives_the_full_table(engine, table):
         r.close()
 
 
+def test_serving_with_symmetry_breaking():
+    """Serving with exact symmetry breaking: every embedding found is
+    expanded by the query's automorphism group into the dense table (one
+    claimed range per automorphism); each served query's count and sorted
+    table equal the oracle's full enumeration, with all queries in flight
+    together."""
+    import time
+    from test_symmetry import QUERIES
+    V = 300
+    g = fm.Graph.erdos_renyi(V, 1500, 1, seed=31)
+    s, d, l = g.edge_list()
+    og = orc.OracleGraph(V, s, d, l)
+    g.upload()
+    qs = [fm.Query.from_edges(n, e) for name, (n, e) in QUERIES.items()
+          if not (name.startswith("directed") or "loop" in name or "two labels" in name)]
+    refs = [orc.match(og, q.vertices, q.edges(), threads=4) for q in qs]
+    with fm.Engine(soln_prealloc=1 << 29, output_prealloc=1 << 29, bank_size=1 << 13, output_page_bytes=1 << 16) as e:
+        # 2,048 pages of 2,730 six-vertex rows hold the largest table (924k rows)
+        with fm.Server(e, g, slots=32, slot_pages=2048, max_launch_ms=60000, symmetry=True) as srv:
+            t2q = {}
+            for i, q in enumerate(qs):
+                t = srv.submit(q)
+                assert t is not None
+                t2q[t] = i
+            done = {}
+            t_end = time.time() + 60
+            while t2q and time.time() < t_end:
+                for c in srv.poll():
+                    i = t2q.pop(c.ticket)
+                    assert not (c.flags & fm.FM_RESULT_TRUNCATED), i
+                    done[i] = (c.status, c.count, srv.table(c.ticket, c.rows, c.vertices))
+                    srv.release(c.ticket)
+                time.sleep(0.001)
+    assert len(done) == len(qs)
+    for i, ref in enumerate(refs):
+        st, cnt, tab = done[i]
+        assert st == 0 and cnt == ref.count, i
+        assert np.array_equal(tab, ref.table), i
+
+
 _GOOD_G = ("t # 0\n4 8 1 2\nv 0 1\nv 1 1\nv 2 0\nv 3 0\ne 0 1 1\ne 1 0 1\ne 1 2 2\ne 2 1 2\ne 2 3 1\ne 3 2 1\n"
            "e 3 0 0\ne 0 3 0\nt # -1")
 
