@@ -2102,7 +2102,7 @@ static int full_reset(fm_engine *e) {
 static std::string timeout_site(const Ctl &c) {
     static const char *names[] = {"?", "main loop", "task", "host abort", "?", "?", "?", "?", "?", "?", "?",
                                   "bank from free ring", "bank release", "task push", "task pop",
-                                  "page from free ring", "page give-back", "page recycle"};
+                                  "page from free ring", "?", "page recycle", "serving table page"};
     const uint32_t site = c.diag[0];
     char buf[160];
     snprintf(buf, sizeof buf, "at %s, values %u %u %u", site < sizeof(names) / sizeof(names[0]) ? names[site] : "?",
