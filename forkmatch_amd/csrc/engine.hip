@@ -87,7 +87,7 @@ struct DQuery {
     unsigned long long max_partials;
     unsigned long long stored;  // intermediate partial embeddings written
     unsigned long long spare;  // partly filled page handed between waves: (id + 1) << 32 | fill
-    long long qpending;        // this query's unfinished tasks (serving completion)
+    long long qpending;        // serving: this query's unfinished tasks (completion)
     // serving tables are dense: row r sits in page r / page_rows of the slot
     unsigned long long cursor;    // rows claimed
     unsigned long long trunc_at;  // rows from here on are not stored (budget or arena exhausted)
@@ -146,6 +146,11 @@ static_assert(ST_N <= STAT_STRIDE, "counters fit a shard");
 
 struct Ctl {
     alignas(128) long long pending;  // tasks pushed or root chunks not yet finished
+    // Batch launches: set by the wave whose task end takes pending to zero
+    // (pending never rises again from zero there), or by the host for a
+    // launch without root chunks. Waiting waves poll this quiet word rather
+    // than pending, which every task end updates.
+    alignas(128) uint32_t done;
     alignas(128) unsigned long long root_cursor;
     HungryShard hungry[HUNGRY_SHARDS];  // waves that found no work, by workgroup id % HUNGRY_SHARDS
     alignas(128) uint32_t abort;
@@ -172,7 +177,7 @@ struct Ctl {
     // completed, bit 63 set once the launch is closing (no more claims; all
     // waves leave). The host clears bit 63 before it relaunches.
     alignas(128) unsigned long long srv_state;
-    alignas(128) int32_t live_banks;     // banks out of the pool
+    alignas(128) int32_t live_banks;     // serving: banks out of the pool
     alignas(128) uint32_t srv_page_bump;
     alignas(128) uint32_t pf_head;       // recycled output pages
     alignas(128) uint32_t pf_tail;
@@ -272,9 +277,13 @@ struct KParams {
     uint32_t pfcap;
     uint32_t max_slots;
     uint32_t max_slot_pages;
-    uint32_t pad2;
+    // Waves with workgroup id >= active_waves (when nonzero) start dormant
+    // and join the launch once it has run for join_ticks, unless it has
+    // ended by then.
+    uint32_t active_waves;
     unsigned long long idle_ticks;
     unsigned long long slice_ticks;  // serving: the launch retires after this long (0 = never)
+    unsigned long long join_ticks;
 };
 
 // ---------------------------------------------------------------------------
@@ -497,7 +506,7 @@ __device__ uint32_t bank_alloc(const KParams &P, unsigned long long *stats) {
             uint32_t id = s->id;
             st_rel(&s->seq, pos + P.fcap);
             __hip_atomic_store(&P.bank_ref[id], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            atom_add(&c->live_banks, 1);
+            if (P.serving) atom_add(&c->live_banks, 1);
             stats[ST_BANKS_REUSED]++;
             return id;
         }
@@ -507,7 +516,7 @@ __device__ uint32_t bank_alloc(const KParams &P, unsigned long long *stats) {
         uint32_t id = atom_add(&c->bank_bump, 1u);
         if (id < P.nbanks) {
             __hip_atomic_store(&P.bank_ref[id], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            atom_add(&c->live_banks, 1);
+            if (P.serving) atom_add(&c->live_banks, 1);
             stats[ST_BANKS_FRESH]++;
             return id;
         }
@@ -524,7 +533,7 @@ __device__ void bank_release(const KParams &P, uint32_t id) {
         s->id = id;
         st_rel(&s->seq, pos + 1);
         atom_add(&c->f_avail, 1);
-        atom_add(&c->live_banks, -1);
+        if (P.serving) atom_add(&c->live_banks, -1);
     }
 }
 
@@ -558,7 +567,7 @@ __device__ void push_tasks(const KParams &P, uint32_t lvl, uint32_t m, const Tas
     uint32_t pos0 = 0, full = 0;
     if (lane == 0) {
         atom_add(&c->pending, (long long)m);
-        atom_add(&P.qs[qslot(t.q)].qpending, (long long)m);
+        if (P.serving) atom_add(&P.qs[qslot(t.q)].qpending, (long long)m);
         if (bank != ROOT_BANK) atom_add(&P.bank_ref[bank], (int32_t)m);
         pos0 = atom_add(&c->lq[lvl].tail, m);
         uint32_t head = ld_rlx(&c->lq[lvl].head);
@@ -1432,10 +1441,13 @@ __global__ __launch_bounds__(WAVE) void fm_match_kernel(KParams P) {
     __shared__ unsigned long long stats[ST_N];
     if (lane == 0)
         for (int i = 0; i < ST_N; i++) stats[i] = 0;
-    const unsigned long long t_wave = wall_clock64();
+    unsigned long long t_wave = wall_clock64();
     unsigned long long starve_since = 0;
     unsigned long long deadline = 0, retire_at = ~0ull;  // retire_at: lane 0 only
-    bool first_claim = true;                              // lane 0 only
+    // a dormant wave has no first root chunk: the host starts root_cursor at
+    // the number of waves that start at once
+    const bool dormant = P.active_waves && blockIdx.x >= P.active_waves;
+    bool first_claim = !dormant;  // lane 0 only
     if (lane == 0) {
         // Each wave times its deadline and slice from its own start (the
         // waves of a launch start within microseconds); only workgroup 0
@@ -1448,6 +1460,22 @@ __global__ __launch_bounds__(WAVE) void fm_match_kernel(KParams P) {
         if (P.serving && P.slice_ticks) retire_at = now + P.slice_ticks;
     }
     deadline = ((unsigned long long)bcast((uint32_t)(deadline >> 32), 0) << 32) | bcast((uint32_t)deadline, 0);
+    if (dormant) {
+        // Wait, polling every ~14 us, until the launch is join_ticks old
+        // (then take part like any idle wave), has no work left, aborts or
+        // passes its deadline (the loop below then leaves at once).
+        if (lane == 0) {
+            while (true) {
+                for (int i = 0; i < 4; i++) __builtin_amdgcn_s_sleep(127);
+                const unsigned long long now = wall_clock64();
+                if (now - t_wave >= P.join_ticks || now > deadline || ld_rlx(&P.ctl->done) ||
+                    ld_rlx(&P.ctl->abort))
+                    break;
+            }
+        }
+        __syncthreads();
+        t_wave = wall_clock64();  // resident time counts from the join
+    }
     while (true) {
         // 0: work, 1: idle, 2: exit
         if (lane == 0) {
@@ -1499,7 +1527,7 @@ __global__ __launch_bounds__(WAVE) void fm_match_kernel(KParams P) {
                             flag = 2;
                         }
                     }
-                } else if (ld_rlx(&P.ctl->pending) <= 0) {
+                } else if (ld_rlx(&P.ctl->done)) {
                     flag = 2;
                 }
             }
@@ -1554,7 +1582,7 @@ __global__ __launch_bounds__(WAVE) void fm_match_kernel(KParams P) {
         }
         if (lane == 0) {
             if (t.bank != ROOT_BANK && t.bank < P.nbanks) bank_release(P, t.bank);
-            atom_add(&P.ctl->pending, -1ll);
+            if (atom_add(&P.ctl->pending, -1ll) == 1 && !P.serving) st_rlx(&P.ctl->done, 1u);
             if (P.serving && !stale) {
                 // rows of this task reach memory before the pending decrement
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
@@ -2560,16 +2588,20 @@ int fm_match_batch(fm_engine *e, const fm_graph *g, const fm_query *const *qs, u
     std::vector<DPlan> plans(nq);
     std::vector<DQuery> qh(nq);
     uint64_t chunks = 0;
-    // Persistent waves. One query: 4 per CU. Measured on R-MAT 2^20 with 16
-    // queries of 6 vertices, one launch each: 79 ms of kernel time per step
-    // at 4 waves per CU, 97 ms at 8 (every wave's first bank and page claims
-    // and its idle polls contend on shared counters, and one query seldom
-    // keeps 2,048 waves busy). Several queries: every resident wave (12 per
-    // CU on gfx950); 1,024 queries of 4-12 vertices on the same graph take
-    // 340 ms at 12 per CU against 445 ms at 8.
+    // Persistent waves. One query: 8 per CU, of which 4 start at once and the
+    // rest join once the launch has run FM_JOIN_MS. Measured on R-MAT 2^20
+    // with 16 queries of 6 vertices, one launch each: 77 ms of kernel time
+    // per step with 4 waves per CU, 93 ms with 8 from the start (every
+    // wave's first bank and page claims and its idle polls contend on shared
+    // counters, and a short launch seldom keeps 2,048 waves busy), while the
+    // one 30 ms query of the set runs in 21 ms with 8. Several queries:
+    // every resident wave (12 per CU on gfx950), all from the start; 1,024
+    // queries of 4-12 vertices on the same graph take 340 ms at 12 per CU
+    // against 445 ms at 8. An explicit grid_waves starts every wave at once.
     const uint32_t resident = std::max<uint32_t>(1, (uint32_t)e->waves_per_cu * e->cu_count);
     const uint32_t grid = o.grid_waves ? std::min<uint32_t>(o.grid_waves, resident)
-                                       : (nq == 1 ? std::max<uint32_t>(1, e->grid_waves / 2) : resident);
+                                       : (nq == 1 ? std::max<uint32_t>(1, e->grid_waves) : resident);
+    const uint32_t active = (!o.grid_waves && nq == 1 && grid >= 2) ? grid / 2 : grid;
     uint32_t maxn = 1;
     for (uint32_t i = 0; i < nq; i++) {
         if (!qs[i]) return fail(FM_ERR_INVALID_ARG, "null query %u", i);
@@ -2608,7 +2640,8 @@ int fm_match_batch(fm_engine *e, const fm_graph *g, const fm_query *const *qs, u
     HIP_TRY(hipMemcpyAsync(e->d_qs, qh.data(), nq * sizeof(DQuery), hipMemcpyHostToDevice, e->stream));
     Ctl reset{};
     reset.pending = (long long)chunks;
-    reset.root_cursor = grid;  // chunks below the grid size are each wave's first claim
+    reset.root_cursor = active;  // chunks below this are the first claims of the waves that start at once
+    reset.done = chunks == 0 ? 1u : 0u;
     reset.timeout_ticks = (unsigned long long)((double)o.timeout_ms * (e->wall_hz / 1000.0));
     HIP_TRY(hipMemcpyAsync(e->ctl, &reset, CTL_RESET_BYTES, hipMemcpyHostToDevice, e->stream));
     // ---- launch ----
@@ -2645,6 +2678,8 @@ int fm_match_batch(fm_engine *e, const fm_graph *g, const fm_query *const *qs, u
     P.nlevels = std::max<uint32_t>(1, maxn - 1);
     P.starve_ticks = (unsigned long long)(e->wall_hz * 2.0);  // 2 s without a bank
     P.timeout_ticks = (unsigned long long)((double)o.timeout_ms * (e->wall_hz / 1000.0));
+    P.active_waves = active < grid ? active : 0;
+    P.join_ticks = (unsigned long long)((double)FM_JOIN_MS * (e->wall_hz / 1000.0));
     fm_launch_stats st{};
     st.grid_waves = grid;
     HIP_TRY(hipEventRecord(e->ev0, e->stream));

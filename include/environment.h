@@ -54,6 +54,17 @@
  * tasks 380 ms; 8192 and 16384 within 2% of 4096. */
 #define FM_BATCH_TASK_ROWS 4096
 
+/* A launch of one query with the default grid starts half of its waves
+ * (4 per CU) and lets the other half join once it has run this many
+ * milliseconds: short launches lose more to per-wave start costs and shared
+ * counters than extra waves give back, long ones gain from them. Measured
+ * on R-MAT 2^20 with 16 queries of 6 vertices: the one 30 ms query runs in
+ * 22-24 ms with a join after 5-9 ms, while queries of 6-7 ms run 10-15%
+ * slower when the waves join after 5 ms. */
+#ifndef FM_JOIN_MS
+#define FM_JOIN_MS 8
+#endif
+
 /* Number of root vertices claimed by one wave at a time (the reference gives
  * each block a fixed root range, src/smsm.cu:803-811). */
 #define TooLargeThreshold 1024

@@ -255,8 +255,10 @@ typedef struct fm_match_opts {
     uint64_t max_rows;        /* table capacity per query (0 = arena) */
     uint64_t max_count;       /* abort a query past this count (0 = none) */
     uint32_t timeout_ms;      /* launch deadline (0 = 120000) */
-    uint32_t grid_waves;      /* persistent waves (0: 4 per CU for one query,
-                                 every resident wave for several) */
+    uint32_t grid_waves;      /* persistent waves (0: 8 per CU for one query,
+                                 of which 4 start at once and 4 join after
+                                 FM_JOIN_MS; every resident wave for several;
+                                 an explicit value starts all at once) */
     uint32_t task_rows;       /* parents per queued task (0: SubMaxMulSpill for one
                                  query, FM_BATCH_TASK_ROWS for several) */
     uint32_t root_chunk;      /* roots per claim (TooLargeThreshold) */
