@@ -892,7 +892,7 @@ static_assert(POOL_WORDS / (MAXN + 2 * MAXR) >= 1, "pool too small for the wides
 struct Lds {
     uint32_t pool[POOL_WORDS];          // parents | rows start | rows length
     uint32_t dstart[WAVE];              // driver row start
-    uint32_t cum[WAVE];                 // inclusive scan of driver lengths
+    alignas(16) uint32_t cum[WAVE];     // inclusive scan of driver lengths
     uint32_t didx[WAVE];                // driver requirement index
     // instrumented launches: per-task sums, flushed once per task (lane 0)
     unsigned long long inst[4 + MAXN];  // row words, lookups, candidates, probes, partials
@@ -1101,17 +1101,29 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
         // bounds and unconditional (no loop-carried value merged at a
         // branch); T only shrinks, so a prefetched position below the
         // current T is that step's candidate.
+        // The parent of candidate tt is the number of parents p with
+        // cum[p] <= tt: cum is nondecreasing and every one of its 64 entries
+        // is written per batch, those past the batch's parents holding its
+        // original total, which exceeds tt. Three rounds of 4 independent
+        // LDS reads (16-parent groups, 4-parent groups, parents) find it,
+        // instead of a chain of 6 reads in a binary search.
         uint32_t pf_pi = 0, pf_x = 0;
         auto prefetch = [&](uint32_t from) {
             const uint32_t tt = min(from + lane, T - 1);
-            uint32_t lo = 0, hi = nb - 1;
-            while (lo < hi) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (s.cum[mid] > tt) hi = mid;
-                else lo = mid + 1;
-            }
+            uint32_t lo = 0;
+#pragma unroll
+            for (uint32_t j = 0; j < 4; j++) lo += s.cum[16 * j + 15] <= tt ? 16u : 0u;
+            uint32_t lo4 = lo;
+#pragma unroll
+            for (uint32_t j = 0; j < 4; j++) lo4 += s.cum[lo + 4 * j + 3] <= tt ? 4u : 0u;
+            const uint4 c4 = *reinterpret_cast<const uint4 *>(&s.cum[lo4]);
+            uint32_t excl = lo4 ? s.cum[lo4 - 1] : 0u;
+            lo = lo4;
+            if (c4.x <= tt) { lo++; excl = c4.x; }
+            if (c4.y <= tt) { lo++; excl = c4.y; }
+            if (c4.z <= tt) { lo++; excl = c4.z; }
+            if (c4.w <= tt) { lo++; excl = c4.w; }
             pf_pi = lo;
-            const uint32_t excl = lo ? s.cum[lo - 1] : 0;
             pf_x = g.col[s.dstart[lo] + (tt - excl)];
         };
         if (T) prefetch(0);
