@@ -425,6 +425,78 @@ __device__ __forceinline__ bool row_contains(const uint32_t *col, const uint32_t
     return hit;
 }
 
+// Membership of xa in row a and of xb in row b by the steps of
+// row_contains, with both rows' loads in flight together: the candidate loop
+// probes two candidates per lane, so each step waits once per probe level
+// instead of twice. A row of length 0 holds nothing; its loads read the
+// first words of the arrays, which always exist.
+__device__ __forceinline__ void row_contains2(const uint32_t *col, const uint32_t *skip, uint32_t sta, uint32_t lena,
+                                              uint32_t xa, uint32_t stb, uint32_t lenb, uint32_t xb, bool &ha,
+                                              bool &hb) {
+    uint32_t loa = sta, enda = sta + lena, lob = stb, endb = stb + lenb;
+    const bool sa = lena > SKIP_BLOCK, sb = lenb > SKIP_BLOCK;
+    if (sa || sb) {
+        uint32_t bloa = 0, bna = 0, blob = 0, bnb = 0;
+        if (sa) {
+            bloa = (sta >> SKIP_SHIFT) + 1;
+            bna = ((enda - 1) >> SKIP_SHIFT) - (sta >> SKIP_SHIFT);
+        }
+        if (sb) {
+            blob = (stb >> SKIP_SHIFT) + 1;
+            bnb = ((endb - 1) >> SKIP_SHIFT) - (stb >> SKIP_SHIFT);
+        }
+        while (bna > 16 || bnb > 16) {
+            const uint32_t hfa = bna >> 1, hfb = bnb >> 1;
+            const uint32_t va = skip[bna > 16 ? bloa + hfa : 0u];
+            const uint32_t vb = skip[bnb > 16 ? blob + hfb : 0u];
+            if (bna > 16) {
+                if (va <= xa) {
+                    bloa += hfa + 1;
+                    bna -= hfa + 1;
+                } else {
+                    bna = hfa;
+                }
+            }
+            if (bnb > 16) {
+                if (vb <= xb) {
+                    blob += hfb + 1;
+                    bnb -= hfb + 1;
+                } else {
+                    bnb = hfb;
+                }
+            }
+        }
+        const Window20 wa = load_window(skip, bloa), wb = load_window(skip, blob);
+        const uint32_t sa0 = bloa & ~3u, sb0 = blob & ~3u;
+        uint32_t ca = 0, cb = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < 20; j++) {
+            ca += (sa0 + j - bloa < bna && window_word(wa, j) <= xa) ? 1u : 0u;
+            cb += (sb0 + j - blob < bnb && window_word(wb, j) <= xb) ? 1u : 0u;
+        }
+        if (sa) {
+            const uint32_t bucket = bloa - 1 + ca;
+            loa = max(sta, bucket << SKIP_SHIFT);
+            enda = min(enda, (bucket + 1) << SKIP_SHIFT);
+        }
+        if (sb) {
+            const uint32_t bucket = blob - 1 + cb;
+            lob = max(stb, bucket << SKIP_SHIFT);
+            endb = min(endb, (bucket + 1) << SKIP_SHIFT);
+        }
+    }
+    const Window20 wa = load_window(col, loa), wb = load_window(col, lob);
+    const uint32_t a0 = loa & ~3u, b0 = lob & ~3u;
+    bool fa = false, fb = false;
+#pragma unroll
+    for (uint32_t j = 0; j < 20; j++) {
+        fa |= (a0 + j - loa < enda - loa) && window_word(wa, j) == xa;
+        fb |= (b0 + j - lob < endb - lob) && window_word(wb, j) == xb;
+    }
+    ha = fa;
+    hb = fb;
+}
+
 __device__ __forceinline__ void row_of(const DGraph &g, uint32_t req, uint32_t v, uint32_t &st, uint32_t &len) {
     uint32_t lab = req_label(req);
     if (lab == LABEL_ABSENT) {
@@ -930,7 +1002,9 @@ __device__ void flush_run(const KParams &P, WaveOut &o, uint32_t q, uint32_t D, 
 
 
 // Extends one task by one depth. Returns after pushing children tasks; the
-// caller completes the task (bank reference and pending count).
+// caller completes the task (bank reference and pending count). PAIRED takes
+// two candidates per lane per step (more loads in flight, more registers).
+template <bool PAIRED>
 __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOut &o, WaveTab &wt,
                              unsigned long long *stats, unsigned long long &starve_since,
                              unsigned long long deadline) {
@@ -1107,8 +1181,7 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
         // original total, which exceeds tt. Three rounds of 4 independent
         // LDS reads (16-parent groups, 4-parent groups, parents) find it,
         // instead of a chain of 6 reads in a binary search.
-        uint32_t pf_pi = 0, pf_x = 0;
-        auto prefetch = [&](uint32_t from) {
+        auto locate = [&](uint32_t from, uint32_t &pi_out, uint32_t &x_out) {
             const uint32_t tt = min(from + lane, T - 1);
             uint32_t lo = 0;
 #pragma unroll
@@ -1123,17 +1196,25 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
             if (c4.y <= tt) { lo++; excl = c4.y; }
             if (c4.z <= tt) { lo++; excl = c4.z; }
             if (c4.w <= tt) { lo++; excl = c4.w; }
-            pf_pi = lo;
-            pf_x = g.col[s.dstart[lo] + (tt - excl)];
+            pi_out = lo;
+            x_out = g.col[s.dstart[lo] + (tt - excl)];
         };
-        if (T) prefetch(0);
-        for (uint32_t base = 0, step = 0; base < T; base += WAVE, step++) {
+        // Each step takes STEP candidates: one per lane, or with PAIRED two
+        // (base + lane and base + 64 + lane) whose probes are in flight
+        // together.
+        constexpr uint32_t STEP = PAIRED ? 2 * WAVE : WAVE;
+        uint32_t pf_pi = 0, pf_x = 0, pf_pi2 = 0, pf_x2 = 0;
+        if (T) {
+            locate(0, pf_pi, pf_x);
+            if constexpr (PAIRED) locate(WAVE, pf_pi2, pf_x2);
+        }
+        for (uint32_t base = 0, step = 0; base < T; base += STEP, step++) {
             // Work splitting: when waves are idle, donate half of what is
             // left: the upper half of the remaining parents when at least
             // two batches remain, else the upper half of this batch's
             // remaining candidates together with the parents after them (a
             // hub parent's long driver row is split across waves).
-            if ((step & 3) == 0) {
+            if ((step & (PAIRED ? 1u : 3u)) == 0) {  // every 256 candidates
                 uint32_t want = 0;
                 // donate only while idle waves outnumber the waiting work
                 // (queued tasks and unclaimed root chunks): with thousands
@@ -1185,7 +1266,7 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
                 }
             }
             if (!final_level) {
-                const uint32_t need = WAVE * (D + 1);
+                const uint32_t need = STEP * (D + 1);
                 bool yield = false;
                 if (o.bank != NO_BANK && o.used + need > P.bank_words) {
                     // The output bank is full: hand its rows over as tasks and
@@ -1237,70 +1318,116 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
                     return;
                 }
             }
-            const uint32_t tt = base + lane;
+            const uint32_t ta = base + lane, tb = ta + WAVE;
             uint32_t nprobe = 0;
-            bool ok = false;
-            const uint32_t x = pf_x, pi = pf_pi;
-            prefetch(base + WAVE);
-            if (tt < T) {
-                ok = true;
-                const uint32_t *prow = &par[pi * W];
+            const uint32_t xa = pf_x, pia = pf_pi, xb = pf_x2, pib = pf_pi2;
+            locate(base + STEP, pf_pi, pf_x);
+            if constexpr (PAIRED) locate(base + STEP + WAVE, pf_pi2, pf_x2);
+            const uint32_t *const prowa = &par[pia * W];
+            const uint32_t *const prowb = &par[pib * W];
+            // checks without global reads: not a vertex of the parent, and
+            // the symmetry-breaking order constraints
+            auto local_ok = [&](const uint32_t *prow, uint32_t x) {
+                bool ok = true;
                 for (uint32_t a = 0; a < W; a++) ok &= prow[a] != x;
                 for (uint32_t i = pl.sc_begin[D]; i < pl.sc_begin[D + 1] && ok; i++) {
                     const uint32_t c = pl.sc[i], v = prow[c & 63u];
                     ok = (c & 128u) ? x > v : x < v;
                 }
-                const uint32_t dx = s.didx[pi];
-                for (uint32_t i = 0; i < k && ok; i++)
-                    if (i != dx) {
-                        ok = row_contains(g.col, g.skip, rows_st[pi * k + i], rows_len[pi * k + i], x);
+                return ok;
+            };
+            bool oka = ta < T && local_ok(prowa, xa);
+            bool okb = PAIRED && tb < T && local_ok(prowb, xb);
+            // anchored requirements other than the driver
+            const uint32_t dxa = s.didx[pia];
+            if constexpr (PAIRED) {
+                const uint32_t dxb = s.didx[pib];
+                for (uint32_t i = 0; i < k && (oka || okb); i++) {
+                    const bool pa = oka && i != dxa, pb = okb && i != dxb;
+                    if (!(pa || pb)) continue;
+                    const uint32_t sta = pa ? rows_st[pia * k + i] : 0u, lena = pa ? rows_len[pia * k + i] : 0u;
+                    const uint32_t stb = pb ? rows_st[pib * k + i] : 0u, lenb = pb ? rows_len[pib * k + i] : 0u;
+                    bool ha, hb;
+                    row_contains2(g.col, g.skip, sta, lena, xa, stb, lenb, xb, ha, hb);
+                    if (pa) {
+                        oka = ha;
                         nprobe++;
                     }
-                for (uint32_t i = 0; i < nself && ok; i++) {
-                    uint32_t st, len;
-                    row_of(g, pl.req[rb + k + i], x, st, len);
-                    ok = row_contains(g.col, g.skip, st, len, x);
+                    if (pb) {
+                        okb = hb;
+                        nprobe++;
+                    }
                 }
+            } else {
+                for (uint32_t i = 0; i < k && oka; i++)
+                    if (i != dxa) {
+                        oka = row_contains(g.col, g.skip, rows_st[pia * k + i], rows_len[pia * k + i], xa);
+                        nprobe++;
+                    }
+            }
+            for (uint32_t i = 0; i < nself && oka; i++) {
+                uint32_t st, len;
+                row_of(g, pl.req[rb + k + i], xa, st, len);
+                oka = row_contains(g.col, g.skip, st, len, xa);
+            }
+            for (uint32_t i = 0; i < nself && okb; i++) {
+                uint32_t st, len;
+                row_of(g, pl.req[rb + k + i], xb, st, len);
+                okb = row_contains(g.col, g.skip, st, len, xb);
             }
             if (P.instrument) {
                 unsigned long long np = wave_sum_u64(nprobe);
                 if (lane == 0) s.inst[3] += np;
             }
-            const unsigned long long mask = __ballot(ok);
-            const uint32_t cnt = __popcll(mask);
-            if (cnt == 0) continue;
-            const uint32_t rank = __popcll(mask & ((1ull << lane) - 1ull));
-            if (P.instrument && lane == 0) s.inst[4 + D] += cnt;
+            const unsigned long long ma = __ballot(oka), mb = PAIRED ? __ballot(okb) : 0ull;
+            const uint32_t cna = __popcll(ma), cnb = __popcll(mb);
+            if (cna + cnb == 0) continue;
+            const unsigned long long below = (1ull << lane) - 1ull;
+            const uint32_t ranka = __popcll(ma & below), rankb = __popcll(mb & below);
+            if (P.instrument && lane == 0) s.inst[4 + D] += cna + cnb;
             if (final_level) {
                 const uint32_t naut = pl.naut > 1 ? pl.naut : 1u;
-                cnt_acc += (unsigned long long)cnt * naut;
+                cnt_acc += (unsigned long long)(cna + cnb) * naut;
                 if (P.table) {
-                    const uint32_t *prow = &par[pi * W];
-                    if (naut == 1) {
-                        emit_rows(P, wt, qi, n, ok, rank, cnt, stats, [&](uint32_t *dst) {
-                            for (uint32_t j = 0; j < W; j++) dst[pl.order[j]] = prow[j];
-                            dst[pl.order[D]] = x;
-                        });
-                    } else {
-                        // one row per automorphism p: column u holds m(p(u))
-                        for (uint32_t k = 0; k < naut; k++)
+                    // all lanes call (wave-uniform conditions)
+                    auto emit = [&](bool ok, uint32_t rank, uint32_t cnt, const uint32_t *prow, uint32_t x) {
+                        if (naut == 1) {
                             emit_rows(P, wt, qi, n, ok, rank, cnt, stats, [&](uint32_t *dst) {
-                                for (uint32_t u = 0; u < n; u++) {
-                                    const uint32_t dd = pl.autcol[k][u];
-                                    dst[u] = dd == D ? x : prow[dd];
-                                }
+                                for (uint32_t j = 0; j < W; j++) dst[pl.order[j]] = prow[j];
+                                dst[pl.order[D]] = x;
                             });
+                        } else {
+                            // one row per automorphism p: column u holds m(p(u))
+                            for (uint32_t ai = 0; ai < naut; ai++)
+                                emit_rows(P, wt, qi, n, ok, rank, cnt, stats, [&](uint32_t *dst) {
+                                    for (uint32_t u = 0; u < n; u++) {
+                                        const uint32_t dd = pl.autcol[ai][u];
+                                        dst[u] = dd == D ? x : prow[dd];
+                                    }
+                                });
+                        }
+                    };
+                    // one call site for both halves keeps emit_rows inlined once
+#pragma nounroll
+                    for (uint32_t h = 0; h < (PAIRED ? 2u : 1u); h++) {
+                        const uint32_t cnt = h ? cnb : cna;
+                        if (cnt) emit(h ? okb : oka, h ? rankb : ranka, cnt, h ? prowb : prowa, h ? xb : xa);
                     }
                 }
             } else {
                 const uint32_t CW = D + 1;
-                if (ok) {
-                    uint32_t *dst = P.banks + (size_t)o.bank * P.bank_words + o.used + rank * CW;
-                    const uint32_t *prow = &par[pi * W];
-                    for (uint32_t j = 0; j < W; j++) dst[j] = prow[j];
-                    dst[W] = x;
+                uint32_t *const out = P.banks + (size_t)o.bank * P.bank_words + o.used;
+                if (oka) {
+                    uint32_t *dst = out + ranka * CW;
+                    for (uint32_t j = 0; j < W; j++) dst[j] = prowa[j];
+                    dst[W] = xa;
                 }
-                o.used += cnt * CW;
+                if (okb) {
+                    uint32_t *dst = out + (cna + rankb) * CW;
+                    for (uint32_t j = 0; j < W; j++) dst[j] = prowb[j];
+                    dst[W] = xb;
+                }
+                o.used += (cna + cnb) * CW;
                 if ((o.used - o.run_begin) / CW >= run_rows) flush_run(P, o, t.q, D, lvl_children, run_rows, stats);
             }
         }
@@ -1439,6 +1566,7 @@ __device__ void admit(const KParams &P, uint32_t slot, unsigned long long *stats
     }
 }
 
+template <bool PAIRED>
 __global__ __launch_bounds__(WAVE) void fm_match_kernel(KParams P) {
     __shared__ Lds s;
     __shared__ TaskDesc s_task;
@@ -1580,7 +1708,7 @@ __global__ __launch_bounds__(WAVE) void fm_match_kernel(KParams P) {
         uint32_t stale = 0;
         if (P.serving && lane == 0) stale = (t.q >> 24) != (ld_rlx(&P.qs[qslot(t.q)].pad) & 0xFFu);
         stale = bcast(stale, 0);
-        if (!stale) process_task(P, t, s, o, wt, stats, starve_since, deadline);
+        if (!stale) process_task<PAIRED>(P, t, s, o, wt, stats, starve_since, deadline);
         else if (lane == 0) stats[ST_STALE]++;
         __syncthreads();
         if (lane == 0) stats[ST_BUSY] += wall_clock64() - tb;
@@ -2070,7 +2198,8 @@ struct fm_engine {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int cu_count = 0;
-    int waves_per_cu = 0;
+    int waves_per_cu = 0;         // resident waves per CU: fm_match_kernel<false>
+    int waves_per_cu_paired = 0;  // the same for fm_match_kernel<true>
     uint32_t grid_waves = 0;
     double wall_hz = 100e6;
     char name[64] = {0};
@@ -2421,13 +2550,18 @@ int fm_engine_create(const fm_engine_config *cfg, fm_engine **out) {
     int wall_khz = 0;
     if (hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, e->device) == hipSuccess && wall_khz > 0)
         e->wall_hz = wall_khz * 1000.0;
-    int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fm_match_kernel, WAVE, 0) != hipSuccess || blocks <= 0)
+    int blocks = 0, blocks2 = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fm_match_kernel<false>, WAVE, 0) != hipSuccess ||
+        blocks <= 0)
         blocks = 8;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks2, fm_match_kernel<true>, WAVE, 0) != hipSuccess ||
+        blocks2 <= 0)
+        blocks2 = std::min(blocks, 8);
     e->waves_per_cu = blocks;
+    e->waves_per_cu_paired = blocks2;
     // Default grid: 8 waves per CU. The serving launch uses it as is; a
-    // one-query launch uses half of it and a batch every resident wave
-    // (fm_match_batch gives the measurements).
+    // one-query launch starts half of it and a batch uses every resident
+    // wave (fm_match_batch gives the measurements).
     e->grid_waves = (uint32_t)(std::min(blocks, 8) * e->cu_count);
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess)
@@ -2610,9 +2744,15 @@ int fm_match_batch(fm_engine *e, const fm_graph *g, const fm_query *const *qs, u
     // every resident wave (12 per CU on gfx950), all from the start; 1,024
     // queries of 4-12 vertices on the same graph take 340 ms at 12 per CU
     // against 445 ms at 8. An explicit grid_waves starts every wave at once.
-    const uint32_t resident = std::max<uint32_t>(1, (uint32_t)e->waves_per_cu * e->cu_count);
+    // One query runs the kernel that takes two candidates per lane: on
+    // config-2's 16 queries 64 ms of kernel time per step against 68 ms.
+    // Batches keep one per lane: it needs fewer registers, so 12 waves per
+    // CU fit instead of 8, and 1,024 queries take 363 ms against 413 ms.
+    const bool paired = nq == 1;
+    const uint32_t resident =
+        std::max<uint32_t>(1, (uint32_t)(paired ? e->waves_per_cu_paired : e->waves_per_cu) * e->cu_count);
     const uint32_t grid = o.grid_waves ? std::min<uint32_t>(o.grid_waves, resident)
-                                       : (nq == 1 ? std::max<uint32_t>(1, e->grid_waves) : resident);
+                                       : (nq == 1 ? std::max<uint32_t>(1, std::min(e->grid_waves, resident)) : resident);
     const uint32_t active = (!o.grid_waves && nq == 1 && grid >= 2) ? grid / 2 : grid;
     uint32_t maxn = 1;
     for (uint32_t i = 0; i < nq; i++) {
@@ -2695,7 +2835,10 @@ int fm_match_batch(fm_engine *e, const fm_graph *g, const fm_query *const *qs, u
     fm_launch_stats st{};
     st.grid_waves = grid;
     HIP_TRY(hipEventRecord(e->ev0, e->stream));
-    hipLaunchKernelGGL(fm_match_kernel, dim3(grid), dim3(WAVE), 0, e->stream, P);
+    if (paired)
+        hipLaunchKernelGGL(fm_match_kernel<true>, dim3(grid), dim3(WAVE), 0, e->stream, P);
+    else
+        hipLaunchKernelGGL(fm_match_kernel<false>, dim3(grid), dim3(WAVE), 0, e->stream, P);
     hipError_t lerr = hipGetLastError();
     HIP_TRY(hipEventRecord(e->ev1, e->stream));
     std::vector<DQuery> back(nq);
@@ -2938,7 +3081,7 @@ static int server_launch(fm_server *s) {
     P.max_slot_pages = s->cfg.slot_pages;
     P.idle_ticks = (unsigned long long)(s->cfg.idle_ms * (e->wall_hz / 1000.0));
     P.slice_ticks = (unsigned long long)(s->cfg.slice_ms * (e->wall_hz / 1000.0));
-    hipLaunchKernelGGL(fm_match_kernel, dim3(e->grid_waves), dim3(WAVE), 0, e->stream, P);
+    hipLaunchKernelGGL(fm_match_kernel<false>, dim3(e->grid_waves), dim3(WAVE), 0, e->stream, P);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(s->kernel_done, e->stream));
     s->launched = true;

@@ -258,7 +258,9 @@ typedef struct fm_match_opts {
     uint32_t grid_waves;      /* persistent waves (0: 8 per CU for one query,
                                  of which 4 start at once and 4 join after
                                  FM_JOIN_MS; every resident wave for several;
-                                 an explicit value starts all at once) */
+                                 an explicit value starts all at once and is
+                                 capped at the resident waves: 8 per CU for
+                                 one query, 12 for several on gfx950) */
     uint32_t task_rows;       /* parents per queued task (0: SubMaxMulSpill for one
                                  query, FM_BATCH_TASK_ROWS for several) */
     uint32_t root_chunk;      /* roots per claim (TooLargeThreshold) */
