@@ -2179,6 +2179,53 @@ __global__ __launch_bounds__(IB) void fm_g_edges_kernel(const uint8_t *is_edge, 
     }
 }
 
+// Symmetry of a graph's out-rows: every edge (u, v) of label l has the edge
+// (v, u) of label l. A wave takes 64 consecutive rows (row l * V + u), keeps
+// their starts in LDS, steps over their edges 64 at a time (a hub row is
+// spread over the lanes like any other) and probes each edge's reverse row.
+// A missing reverse edge sets *asym; waves stop at their next chunk once it
+// is set.
+__global__ __launch_bounds__(WAVE) void fm_sym_check_kernel(const uint32_t *ptr, const uint32_t *col,
+                                                          const uint32_t *skip, unsigned long long rows, uint32_t V,
+                                                          uint32_t *asym) {
+    __shared__ uint32_t rs[WAVE + 1];           // row starts of the chunk, and its end
+    __shared__ uint32_t rlab[WAVE], rsrc[WAVE];  // label id and source vertex of each row
+    const uint32_t lane = threadIdx.x;
+    const unsigned long long nchunks = (rows + WAVE - 1) / WAVE;
+    for (unsigned long long c = blockIdx.x; c < nchunks; c += gridDim.x) {
+        if (__builtin_amdgcn_readfirstlane(__hip_atomic_load(asym, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) break;
+        const unsigned long long r0 = c * WAVE, r = r0 + lane;
+        // rows past the last one start at the end of the column array
+        rs[lane] = ptr[min(r, rows)];
+        if (lane == 0) rs[WAVE] = ptr[min(r0 + WAVE, rows)];
+        const unsigned long long rc = min(r, rows - 1);
+        rlab[lane] = (uint32_t)(rc / V);
+        rsrc[lane] = (uint32_t)(rc % V);
+        __syncthreads();
+        const uint32_t e1 = rs[WAVE];
+        for (uint32_t base = rs[0]; base < e1; base += WAVE) {
+            const uint32_t e = base + lane;
+            if (e < e1) {
+                // the edge's row: the last j with rs[j] <= e (empty rows
+                // share their start with the next row, so this is the
+                // non-empty row holding e)
+                uint32_t lo = 0, hi = WAVE - 1;
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi + 1) >> 1;
+                    if (rs[mid] <= e) lo = mid;
+                    else hi = mid - 1;
+                }
+                const uint32_t v = col[e];
+                const unsigned long long rr = (unsigned long long)rlab[lo] * V + v;
+                const uint32_t st = ptr[rr], len = ptr[rr + 1] - st;
+                if (!row_contains(col, skip, st, len, rsrc[lo]))
+                    __hip_atomic_store(asym, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        __syncthreads();  // the next chunk rewrites the row starts
+    }
+}
+
 }  // namespace fm
 
 // ===========================================================================
@@ -2471,6 +2518,48 @@ static int gpu_parse_data(const std::string &buf, const DataFrame &f, std::vecto
 
 extern "C" {
 
+}  // extern "C"
+
+// Frees a graph's device arrays (the host CSR stays).
+static void graph_free_device(fm_graph *g) {
+    if (g->device < 0) return;
+    (void)hipFree(g->d_ptr[0]);
+    if (g->d_ptr[1] != g->d_ptr[0]) (void)hipFree(g->d_ptr[1]);
+    (void)hipFree(g->d_col[0]);
+    (void)hipFree(g->d_skip);
+    (void)hipFree(g->d_roots[0]);
+    g->d_ptr[0] = g->d_ptr[1] = g->d_col[0] = g->d_col[1] = g->d_skip = g->d_roots[0] = g->d_roots[1] = nullptr;
+    g->device = -1;
+    g->device_bytes = 0;
+}
+
+// Whether every out-edge (u, v, l) of an uploaded graph has its reverse
+// (v, u, l), checked on the device (fm_sym_check_kernel).
+static int gpu_symmetric(const fm_graph *g, bool &sym) {
+    const HostGraph &h = g->h;
+    sym = true;
+    const uint64_t rows = (uint64_t)h.L() * h.V;
+    if (h.E() == 0 || rows == 0) return FM_OK;
+    uint32_t *d_flag = nullptr;
+    HIP_TRY(hipMalloc((void **)&d_flag, 4));
+    const uint64_t nchunks = (rows + WAVE - 1) / WAVE;
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(nchunks, 8192);
+    hipError_t err = hipMemset(d_flag, 0, 4);
+    if (err == hipSuccess) {
+        hipLaunchKernelGGL(fm_sym_check_kernel, dim3(grid), dim3(WAVE), 0, 0, g->d_ptr[0], g->d_col[0], g->d_skip,
+                           (unsigned long long)rows, h.V, d_flag);
+        err = hipGetLastError();
+    }
+    uint32_t flag = 0;
+    if (err == hipSuccess) err = hipMemcpy(&flag, d_flag, 4, hipMemcpyDeviceToHost);
+    (void)hipFree(d_flag);
+    if (err != hipSuccess) return fail(FM_ERR_DEVICE, "symmetry check: %s", hipGetErrorString(err));
+    sym = flag == 0;
+    return FM_OK;
+}
+
+extern "C" {
+
 int fm_graph_load_g(const char *path, fm_graph **out) {
     if (!path || !out) return fail(FM_ERR_INVALID_ARG, "null argument");
     std::string buf;
@@ -2484,8 +2573,20 @@ int fm_graph_load_g(const char *path, fm_graph **out) {
     buf.shrink_to_fit();
     auto *g = new fm_graph();
     g->h.header_edges = f.header_edges;
-    rc = build_graph(f.V, src.size(), src.data(), dst.data(), lab.data(), g->h);
+    // The out-rows are built as if every line had its reverse, uploaded, and
+    // the symmetry is checked on the device. A directed graph is then built
+    // again on the host with both directions.
+    rc = build_graph(f.V, src.size(), src.data(), dst.data(), lab.data(), g->h, true);
     if (!rc) rc = fm_graph_upload(g);
+    bool sym = true;
+    if (!rc) rc = gpu_symmetric(g, sym);
+    if (!rc && !sym) {
+        graph_free_device(g);
+        g->h = HostGraph();
+        g->h.header_edges = f.header_edges;
+        rc = build_graph(f.V, src.size(), src.data(), dst.data(), lab.data(), g->h);
+        if (!rc) rc = fm_graph_upload(g);
+    }
     if (rc) {
         fm_graph_destroy(g);
         return rc;
@@ -2510,13 +2611,7 @@ int fm_graph_copy_device_csr(const fm_graph *g, int dir, uint32_t *ptr, uint32_t
 
 void fm_graph_destroy(fm_graph *g) {
     if (!g) return;
-    if (g->device >= 0) {
-        (void)hipFree(g->d_ptr[0]);
-        if (g->d_ptr[1] != g->d_ptr[0]) (void)hipFree(g->d_ptr[1]);
-        (void)hipFree(g->d_col[0]);
-        (void)hipFree(g->d_skip);
-        (void)hipFree(g->d_roots[0]);
-    }
+    graph_free_device(g);
     delete g;
 }
 

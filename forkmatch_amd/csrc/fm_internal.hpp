@@ -132,7 +132,7 @@ struct fm_graph {
     // device CSR (dir 1 aliases dir 0 when the graph is symmetric)
     uint32_t *d_ptr[2] = {nullptr, nullptr};
     uint32_t *d_col[2] = {nullptr, nullptr};
-    uint32_t *d_skip = nullptr;  // first word of every 32-word block of the column array
+    uint32_t *d_skip = nullptr;  // first word of every 16-word block of the column array
     uint32_t *d_root_ptr[2] = {nullptr, nullptr};
     uint32_t *d_roots[2] = {nullptr, nullptr};
     uint64_t device_bytes = 0;

@@ -500,7 +500,9 @@ def test_device_g_loader_equals_host_parser(tmp_path):
     """The .g body parsed on the device (line starts found per 4 KiB chunk,
     one thread per line) gives the host parser's graph: a symmetric
     multi-label graph, a directed one with vertex lines after edge lines,
-    and a file of ~120k lines crossing thousands of chunks."""
+    a file of ~120k lines crossing thousands of chunks, and that file with
+    one edge's label changed (directed: the device symmetry check finds the
+    one edge without its reverse)."""
     g1 = fm.Graph.erdos_renyi(1000, 4000, 8, seed=11)
     p1 = tmp_path / "er.g"
     g1.write_g(str(p1))
@@ -516,6 +518,18 @@ def test_device_g_loader_equals_host_parser(tmp_path):
     p3 = tmp_path / "rmat.g"
     g3.write_g(str(p3))
     _same_graph(fm.Graph.load_g(str(p3)), fm.Graph.parse_g(str(p3)))
+    # symmetry is checked on the device: one edge among 120k whose reverse
+    # has another label (one used elsewhere, then a new one) makes the graph
+    # directed, with both directions built
+    text = p3.read_text().split("\n")
+    k = max(i for i, line in enumerate(text) if line.startswith("e "))
+    for new_label in (int(text[k].split()[3]) ^ 1, 999):
+        e = text[k].split()
+        p4 = tmp_path / f"asym{new_label}.g"
+        p4.write_text("\n".join(text[:k] + [f"e {e[1]} {e[2]} {new_label}"] + text[k + 1:]))
+        loaded = fm.Graph.load_g(str(p4))
+        assert not loaded.info().symmetric
+        _same_graph(loaded, fm.Graph.parse_g(str(p4)))
 
 
 @pytest.mark.parametrize("text", [
