@@ -677,8 +677,9 @@ def main():
         best = tune.best
         log(f"autotune: best {best} {tune.best_ms:.2f} ms over {len(tune.trials)} configurations")
         # plain: one launch per query, with the library's single-query
-        # defaults (1,024-parent tasks, 4 waves per CU: the fastest measured
-        # for one query; the batch's tuned knobs are slower there)
+        # defaults (1,024-parent tasks, 8 waves per CU of which 4 start at
+        # once, two candidates per lane: the fastest measured for one query;
+        # the batch's tuned knobs are slower there)
         plain_opts = {}
         for q in queries[:16]:
             engine.match(g, q, timeout_ms=300000, **plain_opts).close()
