@@ -45,12 +45,15 @@ def build_library() -> Path:
     common = ["-O3", "-std=c++17", "-fPIC", f"-I{INCLUDE}", f"-I{CSRC}", "-Wall", "-Wno-unused-function"]
     host_obj = objdir / "host.o"
     engine_obj = objdir / "engine.o"
+    gbuild_obj = objdir / "gbuild.o"
     _run([HIPCC, "-x", "c++", *common, "-c", str(CSRC / "host.cpp"), "-o", str(host_obj)])
     _run([HIPCC, f"--offload-arch={ARCH}", *common, *DEVICE_FLAGS, "-c", str(CSRC / "engine.hip"),
           "-o", str(engine_obj)])
+    _run([HIPCC, f"--offload-arch={ARCH}", *common, *DEVICE_FLAGS, "-c", str(CSRC / "gbuild.hip"),
+          "-o", str(gbuild_obj)])
     tmp = LIB.with_suffix(".so.tmp")
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), str(host_obj),
-          str(engine_obj), "-lpthread"])
+          str(engine_obj), str(gbuild_obj), "-lpthread"])
     os.replace(tmp, LIB)
     tmp_cli = CLI.with_suffix(".tmp")
     _run([HIPCC, "-x", "c++", *common, str(CSRC / "cli.cpp"), "-o", str(tmp_cli), f"-L{PKG}", "-lforkmatch",

@@ -354,13 +354,8 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
     return acc;
 }
 
-// Membership probes read fixed windows past their row: the device column
-// array and its skip index are allocated (and zeroed) with this many words
-// of padding.
-constexpr uint32_t COL_PAD = 64;
-// Skip index: skip[b] = col[b * 16], the first word of every 16-word block of
-// the column array (1/16 of its size).
-constexpr uint32_t SKIP_SHIFT = 4;
+// COL_PAD, SKIP_SHIFT: the device column array's padding and skip index
+// (fm_internal.hpp).
 constexpr uint32_t SKIP_BLOCK = 1u << SKIP_SHIFT;
 
 // The five 16-byte words of a[w0, w0 + 20) for w0 = lo rounded down to 4
@@ -2404,8 +2399,9 @@ int fm_graph_upload(fm_graph *g) {
 // (in file order). Errors carry the host parser's codes and messages: the
 // first malformed line found on the device is parsed again on the host for
 // the message.
-static int gpu_parse_data(const std::string &buf, const DataFrame &f, std::vector<uint32_t> &src,
-                          std::vector<uint32_t> &dst, std::vector<uint32_t> &lab) {
+static int gpu_parse_data(const std::string &buf, const DataFrame &f, uint32_t *&d_edges, uint64_t &n_edges) {
+    d_edges = nullptr;
+    n_edges = 0;
     const uint64_t nbody = f.body_end - f.body_begin;
     if (nbody == 0) return FM_OK;
     const uint64_t nchunks = (nbody + GCHUNK - 1) / GCHUNK;
@@ -2505,13 +2501,23 @@ static int gpu_parse_data(const std::string &buf, const DataFrame &f, std::vecto
                            d_e + ne, d_e + 2 * ne);
         HIP_TRY(hipGetLastError());
     }
-    src.resize(ne);
-    dst.resize(ne);
-    lab.resize(ne);
-    if (ne) {
-        HIP_TRY(hipMemcpy(src.data(), d_e, ne * 4, hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(dst.data(), d_e + ne, ne * 4, hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(lab.data(), d_e + 2 * ne, ne * 4, hipMemcpyDeviceToHost));
+    // the edge arrays stay on the device, owned by the caller
+    owned.erase(std::find(owned.begin(), owned.end(), (void *)d_e));
+    d_edges = d_e;
+    n_edges = ne;
+    return FM_OK;
+}
+
+// Copies device edge arrays src[n] | dst[n] | label[n] to the host.
+static int edges_to_host(const uint32_t *d_e, uint64_t n, std::vector<uint32_t> &src, std::vector<uint32_t> &dst,
+                         std::vector<uint32_t> &lab) {
+    src.resize(n);
+    dst.resize(n);
+    lab.resize(n);
+    if (n) {
+        HIP_TRY(hipMemcpy(src.data(), d_e, n * 4, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(dst.data(), d_e + n, n * 4, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(lab.data(), d_e + 2 * n, n * 4, hipMemcpyDeviceToHost));
     }
     return FM_OK;
 }
@@ -2567,24 +2573,38 @@ int fm_graph_load_g(const char *path, fm_graph **out) {
     DataFrame f;
     int rc = parse_data_frame(buf, f);
     if (rc) return rc;
-    std::vector<uint32_t> src, dst, lab;
-    if ((rc = gpu_parse_data(buf, f, src, dst, lab))) return rc;
+    uint32_t *d_edges = nullptr;
+    uint64_t ne = 0;
+    if ((rc = gpu_parse_data(buf, f, d_edges, ne))) return rc;
+    struct EdgesFree {
+        uint32_t *p;
+        ~EdgesFree() { (void)hipFree(p); }
+    } edges_guard{d_edges};
     buf.clear();
     buf.shrink_to_fit();
     auto *g = new fm_graph();
     g->h.header_edges = f.header_edges;
-    // The out-rows are built as if every line had its reverse, uploaded, and
-    // the symmetry is checked on the device. A directed graph is then built
-    // again on the host with both directions.
-    rc = build_graph(f.V, src.size(), src.data(), dst.data(), lab.data(), g->h, true);
-    if (!rc) rc = fm_graph_upload(g);
+    // The out-rows are built on the device as if every line had its reverse
+    // (on the host when gpu_build_graph declines), and the symmetry is
+    // checked on the device. A directed graph is then built again on the
+    // host with both directions.
+    bool built = false;
+    rc = gpu_build_graph(f.V, ne, d_edges, g, built);
+    if (!rc && !built) {
+        std::vector<uint32_t> src, dst, lab;
+        rc = edges_to_host(d_edges, ne, src, dst, lab);
+        if (!rc) rc = build_graph(f.V, ne, src.data(), dst.data(), lab.data(), g->h, true);
+        if (!rc) rc = fm_graph_upload(g);
+    }
     bool sym = true;
     if (!rc) rc = gpu_symmetric(g, sym);
     if (!rc && !sym) {
         graph_free_device(g);
         g->h = HostGraph();
         g->h.header_edges = f.header_edges;
-        rc = build_graph(f.V, src.size(), src.data(), dst.data(), lab.data(), g->h);
+        std::vector<uint32_t> src, dst, lab;
+        rc = edges_to_host(d_edges, ne, src, dst, lab);
+        if (!rc) rc = build_graph(f.V, ne, src.data(), dst.data(), lab.data(), g->h);
         if (!rc) rc = fm_graph_upload(g);
     }
     if (rc) {

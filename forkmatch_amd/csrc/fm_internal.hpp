@@ -16,6 +16,14 @@ namespace fm {
 int fail(int code, const char *fmt, ...);
 void clear_error();
 
+// Membership probes read fixed windows past their row: the device column
+// array and its skip index are allocated (and zeroed) with this many words
+// of padding.
+constexpr uint32_t COL_PAD = 64;
+// Skip index: skip[b] = col[b * 16], the first word of every 16-word block of
+// the column array (1/16 of its size).
+constexpr uint32_t SKIP_SHIFT = 4;
+
 // One label-partitioned adjacency direction. Row (l, v) is
 // col[ptr[l*V + v] .. ptr[l*V + v + 1]), destinations ascending. roots holds,
 // per label, the ascending list of vertices whose row is non-empty
@@ -141,3 +149,15 @@ struct fm_graph {
 struct fm_query {
     fm::HostQuery h;
 };
+
+namespace fm {
+// Builds g's out-rows on the current device from the device edge arrays
+// d_edges = src[n] | dst[n] | label[n], with build_graph's checks and
+// messages, as if every line had its reverse (the caller checks the
+// symmetry), and keeps them as g's upload (rows, column array and skip
+// index with their padding, root lists) besides the host copy. Leaves
+// built = false, with nothing done, for a graph without edges, a label value
+// of 2^24 or more, or (label, vertex, vertex) keys wider than 64 bits: the
+// host build takes those.
+int gpu_build_graph(uint32_t V, uint64_t n, const uint32_t *d_edges, fm_graph *g, bool &built);
+}  // namespace fm

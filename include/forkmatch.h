@@ -114,8 +114,9 @@ int fm_graph_upload(fm_graph *g);
  * frame and header; the body's lines are found and parsed on the device
  * (one thread per line), with the same checks, error codes and messages as
  * fm_graph_parse_g and 64-bit offsets throughout (no 2 GiB limit); the CSR
- * rows are sorted on the host, uploaded, and checked for symmetry on the
- * device (a directed graph then gets its reverse rows built on the host). */
+ * is built (sorted, de-duplicated, root lists and row statistics) and
+ * checked for symmetry on the device, and copied to the host. A directed
+ * graph, or one with a label value of 2^24 or more, is built on the host. */
 int fm_graph_load_g(const char *path, fm_graph **out);
 int fm_graph_info_get(const fm_graph *g, fm_graph_info *out);
 /* Copies the host CSR. ptr has labels*vertices+1 entries (row (l,v) is

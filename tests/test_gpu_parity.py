@@ -487,13 +487,27 @@ _GOOD_G = ("t # 0\n4 8 1 2\nv 0 1\nv 1 1\nv 2 0\nv 3 0\ne 0 1 1\ne 1 0 1\ne 1 2 
            "e 3 0 0\ne 0 3 0\nt # -1")
 
 
-def _same_graph(a, b):
+def _same_graph(a, b, tmp_path=None):
+    """a (loaded on the device) holds b's graph: header, rows of both
+    directions, the device copy of the rows, and with tmp_path the binary
+    cache files (root lists included) and a cost plan (row statistics)."""
     ia, ib = a.info(), b.info()
     assert (ia.vertices, ia.edges, ia.labels, ia.symmetric, ia.header_edges) == \
         (ib.vertices, ib.edges, ib.labels, ib.symmetric, ib.header_edges)
     for d in (0, 1):
         for x, y in zip(a.csr(d), b.csr(d)):
             assert np.array_equal(x, y)
+        for x, y in zip(a.csr(d, device=True), b.csr(d)):
+            assert np.array_equal(x, y)
+    if tmp_path is not None:
+        a.save_bin(str(tmp_path / "a.bin"))
+        b.save_bin(str(tmp_path / "b.bin"))
+        assert (tmp_path / "a.bin").read_bytes() == (tmp_path / "b.bin").read_bytes()
+        labels = [int(x) for x in b.csr(0)[2][:3]]
+        if len(labels) == 3:
+            q = fm.Query.from_edges(4, [(u, v, labels[min(u, v)]) for u, v in
+                                        ((0, 1), (1, 0), (1, 2), (2, 1), (2, 3), (3, 2))])
+            assert q.plan(a) == q.plan(b)
 
 
 def test_device_g_loader_equals_host_parser(tmp_path):
@@ -506,18 +520,18 @@ def test_device_g_loader_equals_host_parser(tmp_path):
     g1 = fm.Graph.erdos_renyi(1000, 4000, 8, seed=11)
     p1 = tmp_path / "er.g"
     g1.write_g(str(p1))
-    _same_graph(fm.Graph.load_g(str(p1)), fm.Graph.parse_g(str(p1)))
+    _same_graph(fm.Graph.load_g(str(p1)), fm.Graph.parse_g(str(p1)), tmp_path)
     rng = np.random.default_rng(3)
     V, E = 3000, 60000
     s, d, lab = rng.integers(0, V, E), rng.integers(0, V, E), rng.integers(0, 40, E)
     lines = [f"e {a} {b} {c}" for a, b, c in zip(s, d, lab)] + [f"v {v} 0" for v in range(V)]
     p2 = tmp_path / "directed.g"
     p2.write_bytes(("t # 0\n" + f"{V} {E} 1 40\n" + "\n".join(lines) + "\nt # -1").encode())
-    _same_graph(fm.Graph.load_g(str(p2)), fm.Graph.parse_g(str(p2)))
+    _same_graph(fm.Graph.load_g(str(p2)), fm.Graph.parse_g(str(p2)), tmp_path)
     g3 = fm.Graph.rmat(14, 60000, 16, zipf_s=1.2, seed=5)
     p3 = tmp_path / "rmat.g"
     g3.write_g(str(p3))
-    _same_graph(fm.Graph.load_g(str(p3)), fm.Graph.parse_g(str(p3)))
+    _same_graph(fm.Graph.load_g(str(p3)), fm.Graph.parse_g(str(p3)), tmp_path)
     # symmetry is checked on the device: one edge among 120k whose reverse
     # has another label (one used elsewhere, then a new one) makes the graph
     # directed, with both directions built
@@ -529,7 +543,7 @@ def test_device_g_loader_equals_host_parser(tmp_path):
         p4.write_text("\n".join(text[:k] + [f"e {e[1]} {e[2]} {new_label}"] + text[k + 1:]))
         loaded = fm.Graph.load_g(str(p4))
         assert not loaded.info().symmetric
-        _same_graph(loaded, fm.Graph.parse_g(str(p4)))
+        _same_graph(loaded, fm.Graph.parse_g(str(p4)), tmp_path)
 
 
 @pytest.mark.parametrize("text", [
