@@ -1561,8 +1561,10 @@ __device__ void admit(const KParams &P, uint32_t slot, unsigned long long *stats
     }
 }
 
+// The persistent matcher: one wave per workgroup, looping over root chunks,
+// queued tasks and (serving) admissions until the launch's work is done.
 template <bool PAIRED>
-__global__ __launch_bounds__(WAVE) void fm_match_kernel(KParams P) {
+__device__ __forceinline__ void match_waves(const KParams &P) {
     __shared__ Lds s;
     __shared__ TaskDesc s_task;
     __shared__ uint32_t s_flag;
@@ -1735,6 +1737,21 @@ __global__ __launch_bounds__(WAVE) void fm_match_kernel(KParams P) {
     __syncthreads();
     // one lane per counter, into this wave's shard
     if (lane < ST_N && stats[lane]) atom_add(&P.ctl->stats[blockIdx.x % STAT_SHARDS][lane], stats[lane]);
+}
+
+// One query (PAIRED: two candidates per lane), and serving (one per lane).
+template <bool PAIRED>
+__global__ __launch_bounds__(WAVE) void fm_match_kernel(KParams P) {
+    match_waves<PAIRED>(P);
+}
+
+// Batches: two candidates per lane, compiled for 3 waves per SIMD (12 per
+// CU), which spills a few registers; on 1,024 queries of 4-12 vertices
+// 351-357 ms per launch against 362-366 ms with one candidate per lane at
+// the same occupancy. One query runs the unspilled fm_match_kernel<true>
+// instead (67 ms per config-2 step with this one against 64 ms).
+__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3, 3))) void fm_match_kernel_batch(KParams P) {
+    match_waves<true>(P);
 }
 
 __global__ void fm_init_slots_kernel(Slot *ring, uint32_t qcap, uint32_t levels, FSlot *fring, uint32_t fcap) {
@@ -2242,6 +2259,7 @@ struct fm_engine {
     int cu_count = 0;
     int waves_per_cu = 0;         // resident waves per CU: fm_match_kernel<false>
     int waves_per_cu_paired = 0;  // the same for fm_match_kernel<true>
+    int waves_per_cu_batch = 0;   // and for fm_match_kernel_batch
     uint32_t grid_waves = 0;
     double wall_hz = 100e6;
     char name[64] = {0};
@@ -2665,15 +2683,19 @@ int fm_engine_create(const fm_engine_config *cfg, fm_engine **out) {
     int wall_khz = 0;
     if (hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, e->device) == hipSuccess && wall_khz > 0)
         e->wall_hz = wall_khz * 1000.0;
-    int blocks = 0, blocks2 = 0;
+    int blocks = 0, blocks2 = 0, blocks3 = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fm_match_kernel<false>, WAVE, 0) != hipSuccess ||
         blocks <= 0)
         blocks = 8;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks2, fm_match_kernel<true>, WAVE, 0) != hipSuccess ||
         blocks2 <= 0)
         blocks2 = std::min(blocks, 8);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks3, fm_match_kernel_batch, WAVE, 0) != hipSuccess ||
+        blocks3 <= 0)
+        blocks3 = std::min(blocks, 8);
     e->waves_per_cu = blocks;
     e->waves_per_cu_paired = blocks2;
+    e->waves_per_cu_batch = blocks3;
     // Default grid: 8 waves per CU. The serving launch uses it as is; a
     // one-query launch starts half of it and a batch uses every resident
     // wave (fm_match_batch gives the measurements).
@@ -2861,11 +2883,11 @@ int fm_match_batch(fm_engine *e, const fm_graph *g, const fm_query *const *qs, u
     // against 445 ms at 8. An explicit grid_waves starts every wave at once.
     // One query runs the kernel that takes two candidates per lane: on
     // config-2's 16 queries 64 ms of kernel time per step against 68 ms.
-    // Batches keep one per lane: it needs fewer registers, so 12 waves per
-    // CU fit instead of 8, and 1,024 queries take 363 ms against 413 ms.
+    // Batches run the same step compiled for 12 waves per CU
+    // (fm_match_kernel_batch): at 8 per CU, 1,024 queries took 413 ms.
     const bool paired = nq == 1;
     const uint32_t resident =
-        std::max<uint32_t>(1, (uint32_t)(paired ? e->waves_per_cu_paired : e->waves_per_cu) * e->cu_count);
+        std::max<uint32_t>(1, (uint32_t)(paired ? e->waves_per_cu_paired : e->waves_per_cu_batch) * e->cu_count);
     const uint32_t grid = o.grid_waves ? std::min<uint32_t>(o.grid_waves, resident)
                                        : (nq == 1 ? std::max<uint32_t>(1, std::min(e->grid_waves, resident)) : resident);
     const uint32_t active = (!o.grid_waves && nq == 1 && grid >= 2) ? grid / 2 : grid;
@@ -2953,7 +2975,7 @@ int fm_match_batch(fm_engine *e, const fm_graph *g, const fm_query *const *qs, u
     if (paired)
         hipLaunchKernelGGL(fm_match_kernel<true>, dim3(grid), dim3(WAVE), 0, e->stream, P);
     else
-        hipLaunchKernelGGL(fm_match_kernel<false>, dim3(grid), dim3(WAVE), 0, e->stream, P);
+        hipLaunchKernelGGL(fm_match_kernel_batch, dim3(grid), dim3(WAVE), 0, e->stream, P);
     hipError_t lerr = hipGetLastError();
     HIP_TRY(hipEventRecord(e->ev1, e->stream));
     std::vector<DQuery> back(nq);
