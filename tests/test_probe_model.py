@@ -9,7 +9,9 @@ window. Checked against plain membership on random rows and edge cases
 (empty rows, rows of 1..300 and thousands of words, x below, above, on and
 between the row's values, rows starting at any alignment), and every word
 the probe reads must lie inside the column array or the skip index plus
-their 64-word padding."""
+their 64-word padding. The paired probe (row_contains2, two candidates per
+lane) is modelled step for step and must give the single probe's answers,
+with its dummy reads (index 0 of either array) in bounds too."""
 import numpy as np
 import pytest
 
@@ -104,3 +106,79 @@ def test_probe_planted_mistake_is_caught():
 
     assert any(not shifted(int(x)) for x in row)
     assert all(probe(col, skip, 0, 200, int(x), []) for x in row)
+
+
+def probe2(col, skip, sta, na, xa, stb, nb, xb, reads):
+    """engine.hip row_contains2: the two probes of a lane in lockstep, both
+    rows' windows requested together. A side without a skip step reads the
+    skip index at 0 while the other steps; a row of length 0 (a lane with no
+    probe on that side: st = 0) reads the column array at 0 and holds
+    nothing."""
+    loa, enda, lob, endb = sta, sta + na, stb, stb + nb
+    sa, sb = na > SKIP_BLOCK, nb > SKIP_BLOCK
+    if sa or sb:
+        bloa = bna = blob = bnb = 0
+        if sa:
+            bloa, bna = (sta >> SKIP_SHIFT) + 1, ((enda - 1) >> SKIP_SHIFT) - (sta >> SKIP_SHIFT)
+        if sb:
+            blob, bnb = (stb >> SKIP_SHIFT) + 1, ((endb - 1) >> SKIP_SHIFT) - (stb >> SKIP_SHIFT)
+        while bna > 16 or bnb > 16:
+            hfa, hfb = bna >> 1, bnb >> 1
+            ia, ib = (bloa + hfa if bna > 16 else 0), (blob + hfb if bnb > 16 else 0)
+            reads += [("skip", ia), ("skip", ib)]
+            va, vb = skip[ia], skip[ib]
+            if bna > 16:
+                if va <= xa:
+                    bloa, bna = bloa + hfa + 1, bna - hfa - 1
+                else:
+                    bna = hfa
+            if bnb > 16:
+                if vb <= xb:
+                    blob, bnb = blob + hfb + 1, bnb - hfb - 1
+                else:
+                    bnb = hfb
+        sa0, wa = window(skip, bloa, reads, "skip")
+        sb0, wb = window(skip, blob, reads, "skip")
+        ca = sum(1 for j in range(20) if 0 <= sa0 + j - bloa < bna and wa[j] <= xa)
+        cb = sum(1 for j in range(20) if 0 <= sb0 + j - blob < bnb and wb[j] <= xb)
+        if sa:
+            bucket = bloa - 1 + ca
+            loa, enda = max(sta, bucket << SKIP_SHIFT), min(enda, (bucket + 1) << SKIP_SHIFT)
+        if sb:
+            bucket = blob - 1 + cb
+            lob, endb = max(stb, bucket << SKIP_SHIFT), min(endb, (bucket + 1) << SKIP_SHIFT)
+    a0, wa = window(col, loa, reads, "col")
+    b0, wb = window(col, lob, reads, "col")
+    ha = any(0 <= a0 + j - loa < enda - loa and wa[j] == xa for j in range(20))
+    hb = any(0 <= b0 + j - lob < endb - lob and wb[j] == xb for j in range(20))
+    return ha, hb
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_paired_probe_matches_membership_and_stays_in_bounds(seed):
+    rng = np.random.default_rng(100 + seed)
+    lens = [0, 1, 16, 17, 33, 272, 273, 300] + [int(v) for v in rng.integers(0, 3000, 10)]
+    col, skip, starts, ncol, nskip = _arrays(rng, lens)
+    rows = list(zip(starts, lens)) + [(0, 0)]  # (0, 0): a side with no probe
+
+    def xs_of(st, n):
+        row = col[st:st + n]
+        xs = [0, 10 ** 9]
+        if n:
+            xs += [int(row[0]) - 1, int(row[-1]) + 1, int(row[-1])]
+            xs += [int(v) for v in row[:: max(1, n // 6)]] + [int(v) + 1 for v in row[:: max(1, n // 4)]]
+        return xs
+
+    for _ in range(400):
+        (sta, na), (stb, nb) = rows[rng.integers(len(rows))], rows[rng.integers(len(rows))]
+        xa = int(rng.choice(xs_of(sta, na)))
+        xb = int(rng.choice(xs_of(stb, nb)))
+        reads = []
+        ha, hb = probe2(col, skip, sta, na, xa, stb, nb, xb, reads)
+        assert ha == (xa in set(int(v) for v in col[sta:sta + na])), (sta, na, xa)
+        assert hb == (xb in set(int(v) for v in col[stb:stb + nb])), (stb, nb, xb)
+        # the same answers as two single probes
+        assert (ha, hb) == (probe(col, skip, sta, na, xa, []), probe(col, skip, stb, nb, xb, []))
+        for name, i in reads:
+            limit = ncol + COL_PAD if name == "col" else nskip + COL_PAD
+            assert 0 <= i < limit, (name, i, limit)
