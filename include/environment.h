@@ -66,7 +66,9 @@
 #endif
 
 /* Number of root vertices claimed by one wave at a time (the reference gives
- * each block a fixed root range, src/smsm.cu:803-811). */
+ * each block a fixed root range, src/smsm.cu:803-811). A one-query launch
+ * with the default takes about 8 chunks per starting wave instead, between
+ * 64 roots and this many. */
 #define TooLargeThreshold 1024
 
 /* Bytes of one output-table page. A wave appends to its own page; a

@@ -265,7 +265,9 @@ typedef struct fm_match_opts {
                                  one query, 12 for several on gfx950) */
     uint32_t task_rows;       /* parents per queued task (0: SubMaxMulSpill for one
                                  query, FM_BATCH_TASK_ROWS for several) */
-    uint32_t root_chunk;      /* roots per claim (TooLargeThreshold) */
+    uint32_t root_chunk;      /* roots per claim (0: TooLargeThreshold; one
+                                 query: ~8 chunks per starting wave, 64 to
+                                 TooLargeThreshold roots) */
     uint32_t plan;            /* FM_PLAN_COST (0) or FM_PLAN_REFERENCE */
     /* abort a query once it has stored more than this many intermediate
      * partial embeddings (0 = none); its status becomes COUNT_LIMIT */
