@@ -567,3 +567,31 @@ def test_device_g_loader_errors_equal_host_parser(tmp_path, text):
     with pytest.raises(fm.ForkMatchError) as dev:
         fm.Graph.load_g(str(p))
     assert (dev.value.code, dev.value.message) == (host.value.code, host.value.message)
+
+
+@pytest.mark.parametrize("name", ["_3_triangle", "_4_square", "_5_road5"])
+def test_cli_end_to_end_equals_oracle(tmp_path, name):
+    """The ForkMatch CLI as the reference is run (`ForkMatch <query.g>
+    <data.g>`: device .g load, one-query launch) prints the oracle's count
+    and, with FM_TABLE_OUT, writes the oracle's table."""
+    import os
+    import re
+    import subprocess
+    from pathlib import Path
+    g = fm.Graph.erdos_renyi(300, 1500, 1, seed=7)
+    s, d, _ = g.edge_list()
+    ones = np.ones_like(s)
+    data = tmp_path / "d.g"
+    fm.Graph.from_edges(300, s, d, ones).write_g(str(data))
+    q = fm.Query.load_g(str(fixture_path(name)))
+    ref = orc.match(orc.OracleGraph(300, s, d, ones), q.vertices, q.edges())
+    cli = Path(fm.__file__).resolve().parent / "ForkMatch"
+    table = tmp_path / "table.txt"
+    r = subprocess.run([str(cli), str(fixture_path(name)), str(data)], capture_output=True, text=True,
+                       timeout=120, cwd=tmp_path, env=dict(os.environ, FM_TABLE_OUT=str(table)))
+    assert r.returncode == 0, r.stdout + r.stderr
+    m = re.search(r"^Soln Size: (\d+)$", r.stdout, re.M)
+    assert m and int(m.group(1)) == ref.count, r.stdout
+    rows = np.loadtxt(table, dtype=np.uint32, ndmin=2) if ref.count else np.zeros((0, q.vertices), np.uint32)
+    rows = rows[np.lexsort(rows.T[::-1])] if len(rows) else rows
+    assert np.array_equal(rows, ref.table.reshape(-1, q.vertices))
