@@ -415,8 +415,8 @@ def main():
         args.budget = {3: 1 << 22, 4: 1 << 18, 5: 1 << 27}.get(args.config, 1 << 26)
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py runs on one MI355X only: this build has no multi-GPU path "
-              "(the data graph would be replicated and root ranges sharded; see DESIGN.md)", file=sys.stderr)
+        print("bench.py runs on one MI355X only: there is no multi-GPU path "
+              "(the data graph would be replicated and root ranges sharded)", file=sys.stderr)
         sys.exit(2)
     if fm.device_count() == 0:
         raise SystemExit("no HIP device")

@@ -1749,7 +1749,8 @@ __global__ __launch_bounds__(WAVE) void fm_match_kernel(KParams P) {
 // CU), which spills a few registers; on 1,024 queries of 4-12 vertices
 // 351-357 ms per launch against 362-366 ms with one candidate per lane at
 // the same occupancy. One query runs the unspilled fm_match_kernel<true>
-// instead (67 ms per config-2 step with this one against 64 ms).
+// instead (on R-MAT 2^20 with 16 queries of 6 vertices, one launch each:
+// 67 ms of kernel time per step with this one against 64 ms).
 __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3, 3))) void fm_match_kernel_batch(KParams P) {
     match_waves<true>(P);
 }
@@ -2883,7 +2884,8 @@ int fm_match_batch(fm_engine *e, const fm_graph *g, const fm_query *const *qs, u
     // queries of 4-12 vertices on the same graph take 340 ms at 12 per CU
     // against 445 ms at 8. An explicit grid_waves starts every wave at once.
     // One query runs the kernel that takes two candidates per lane: on
-    // config-2's 16 queries 64 ms of kernel time per step against 68 ms.
+    // R-MAT 2^20 with 16 queries of 6 vertices, one launch each, 64 ms of
+    // kernel time per step against 68 ms.
     // Batches run the same step compiled for 12 waves per CU
     // (fm_match_kernel_batch): at 8 per CU, 1,024 queries took 413 ms.
     const bool paired = nq == 1;
@@ -2901,9 +2903,9 @@ int fm_match_batch(fm_engine *e, const fm_graph *g, const fm_query *const *qs, u
             // One query: about 8 root chunks per starting wave, between 64
             // roots (one batch) and TooLargeThreshold, so that waves that
             // draw hub roots do not hold the root depth back. Measured on
-            // config 2 (every query starts from all 2^20 vertices, 1,024
-            // waves): 62.1 ms of kernel time per step with 128-root chunks
-            // against 63.6-64.2 ms with 1,024.
+            // R-MAT 2^20 with 16 queries of 6 vertices (every query starts
+            // from all 2^20 vertices, 1,024 waves): 62.1 ms of kernel time
+            // per step with 128-root chunks against 63.6-64.2 ms with 1,024.
             const uint64_t per = ((uint64_t)plans[i].nroots + 8ull * active - 1) / (8ull * active);
             o.root_chunk = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(per, 64), TooLargeThreshold);
         }

@@ -35,7 +35,8 @@ def engine():
 
 @pytest.fixture(scope="session")
 def er_graph():
-    """Config-1 shape: G(1000, 4000) undirected, edge labels uniform in 0..7."""
+    """G(1000, 4000) undirected, edge labels uniform in 0..7 (the reference's
+    CPU-runnable example size)."""
     import forkmatch_amd as fm
     return fm.Graph.erdos_renyi(1000, 4000, 8, seed=0xF0F1)
 

@@ -81,7 +81,8 @@ def test_random_walk_queries(engine, er):
 
 
 def test_rmat_zipf_queries(engine):
-    # config-2 shape at scale 12: R-MAT, Zipf(1.2) labels over 0..86
+    # R-MAT at scale 12 with Zipf(1.2) labels over 0..86 (the shape of the
+    # 2^20-vertex benchmark graph)
     g = fm.Graph.rmat(12, 16 * 4096, 87, zipf_s=1.2, seed=0xF0F2).upload()
     og = _oracle_graph(g)
     for seed in range(8):
