@@ -1,23 +1,27 @@
 """ForkMatch MI355X benchmark.
 
-Default workload (N=1): plain matching at scale. An R-MAT data graph
-(a, b, c, d = .57, .19, .19, .05), 2^20 vertices, 2^24 undirected edges
-written as 2^25 directed edges, edge labels Zipf(1.2) over 0..86, resident in
-HBM; 32 seeded queries of 6-8 vertices (16 trees, 16 with >= 2 independent
-cycles) whose embedding count is at most the table budget, each matched in
-its own launch with the full embedding table written to HBM. One step = all
-32 launches. value = embeddings/s over the timed steps (whole job, inputs
-resident before the timed region).
+Default workload (N=1): plain matching at scale, with the batched form
+beside it. An R-MAT data graph (a, b, c, d = .57, .19, .19, .05), 2^20
+vertices, 2^24 undirected edges written as 2^25 directed edges, edge labels
+Zipf(1.2) over 0..86, resident in HBM.
+ - Plain (the headline): 32 seeded random-walk queries of 6-8 vertices, each
+   matched in its own launch. 16 are trees, enumerated in count mode (the
+   reference's WriteSolnFlag = false, src/environment.cuh:65) and screened to
+   at most 2^31 embeddings; 16 have >= 2 independent cycles and are matched
+   with their full embedding table written to HBM (at most 2^26 embeddings
+   each). One step = all 32 launches. value = embeddings/s over the timed
+   steps (whole job, inputs resident before the timed region).
+ - Batched (the `batched` object): 1,024 queries of 4..12 vertices (shape
+   caps 40% trees, 40% cycle+chord, 20% near-cliques, each with a table of at
+   most 2^22 rows) in ONE persistent launch, against the same queries one
+   launch each: queries/s of both and their ratio.
 
---config 3: the same graph, 1,024 queries of 4..12 vertices (40% trees,
-40% cycle+chord, 20% near-cliques) matched in ONE persistent launch
-(batched), with the plain one-launch-per-query rate measured beside it;
-value = queries/s of the batched form.
+--config 3: the batched leg alone (value = batched queries/s).
 
 --config 5: stress. R-MAT 2^24 vertices, 2^28 undirected edges, edge labels
-uniform over 0..3; 64 dense 10-vertex queries (15 edges each) with at most
-2^27 embeddings each and 2.5e9 in all (~100 GB of tables), matched in one
-batched launch with every table resident; value = embeddings/s.
+uniform over 0..3; 64 dense 10-vertex queries (15 edges each), each capped at
+2.5e9 / 64 rows (~100 GB of tables in all), matched in one batched launch
+with every table resident; value = rows stored per second.
 
 --config 4: serving. R-MAT 2^22 vertices, 2^26 undirected edges, edge
 labels Zipf(1.2) over 0..127, 4 vertex labels; 8 MiB banks (CurBankSize =
@@ -81,12 +85,11 @@ def make_candidates(g, cfg: int, seed: int, count: int):
             out.append((3, 10, q))
             continue
         if cfg == 2:
-            # tree-like: a walk tree plus exactly one induced chord (every pure
-            # 6-8 vertex tree on this graph has > 4e8 embeddings); the other
-            # half has >= 2 independent cycles
-            shape = 3 if len(out) % 2 == 0 else 1
+            # half trees (matched count-only), half with >= 2 independent
+            # cycles (matched with tables)
+            shape = 0 if len(out) % 2 == 0 else 1
             n = int(rng.integers(6, 9))
-            cyc = 1 if shape == 3 else 2
+            cyc = 2 if shape == 1 else 0
         else:
             u = rng.random()
             shape = 0 if u < 0.4 else (1 if u < 0.8 else 2)
@@ -187,23 +190,76 @@ def validate_sample(g, q, table: np.ndarray, rng, k=256) -> bool:
     return True
 
 
-def cpu_baseline(g, queries, target_s: float, threads: int):
-    """The oracle (C backtracking) on a strided root sample of the same queries:
-    roots stride/2, 3 stride/2, ... (the midpoint of each stride window, so
-    the sample is not dominated by vertex 0, R-MAT's largest hub)."""
+def host_threads() -> tuple[int, int, int]:
+    """(threads for the all-cores leg, CPUs this process may run on,
+    os.cpu_count()). The pool is the process's CPU affinity, capped at
+    OMP_NUM_THREADS when that is set: a GPU box shares its host among its
+    GPUs and sets it to this GPU's share of the cores."""
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    cap = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return (min(avail, cap) if cap > 0 else avail), avail, (os.cpu_count() or avail)
+
+
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baselines(g, chosen, target_s: float):
+    """The oracle on all host threads of this GPU's share and on one thread,
+    each on its own bounded root sample of the same queries (count only,
+    the reference's span: enumeration without graph load or planning)."""
     import oracle as orc
     t0 = time.perf_counter()
     s, d, l = g.edge_list()
     og = orc.OracleGraph(g.vertices, s, d, l)
     log(f"cpu baseline: oracle graph built in {time.perf_counter() - t0:.1f} s")
-    V = g.vertices
+    threads, avail, ncpu = host_threads()
+    out = {}
+    for leg, nt in (("all_cores", threads), ("one_thread", 1)):
+        cnt, secs, stride = cpu_baseline(og, g.vertices, chosen, target_s, nt)
+        out[leg] = {"value": cnt / secs if secs > 0 else None, "cores": nt, "embeddings": cnt,
+                    "seconds": secs, "root_stride": stride}
+        log(f"cpu baseline ({leg}, {nt} threads): {out[leg]['value']:.3g} embeddings/s")
+    main = out["all_cores"]
+    return {"value": main["value"], "unit": "embeddings/s", "cores": threads, "kind": "port",
+            "sample": f"oracle (C backtracking, count only) on the {len(chosen)} queries, roots "
+                      f"{main['root_stride'] // 2}, {main['root_stride'] + main['root_stride'] // 2}, ... "
+                      f"({main['embeddings']} embeddings in {main['seconds']:.1f} s)",
+            "one_thread": out["one_thread"], "all_cores": out["all_cores"],
+            "host": {"cpu_model": cpu_model(), "os_cpu_count": ncpu, "affinity_cpus": avail,
+                     "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}}
+
+
+def cpu_baseline(og, V, queries, target_s: float, threads: int):
+    """The oracle (C backtracking) on a strided root sample of the same queries:
+    roots stride/2, 3 stride/2, ... (the midpoint of each stride window, so
+    the sample is not dominated by vertex 0, R-MAT's largest hub). Each
+    query's enumeration also stops after its share of twice the target, so
+    one root with a huge subtree cannot stretch the run; the count of a
+    stopped enumeration is what it enumerated in its time."""
+    import oracle as orc
+    cap = max(0.25, 2.0 * target_s / max(1, len(queries)))
+
+    def sample(stride):
+        cnt, secs = 0, 0.0
+        for _, _, q in queries:
+            c, t, _ = orc.timed_count(og, q.vertices, q.edges(), threads, cap, roots=(stride // 2, V), stride=stride)
+            cnt += c
+            secs += t
+        return cnt, secs
+
     # calibrate the stride on ever denser samples (1, 16, 256, ... roots per
     # query) until one takes a twentieth of the target, so the probe of a
     # heavy query set stays short; then scale the stride to the target
     stride = max(1, V)
     while True:
-        probe = sum(orc.match(og, q.vertices, q.edges(), threads=threads, table=False,
-                              roots=(stride // 2, V), stride=stride).seconds for _, _, q in queries)
+        probe = sample(stride)[1]
         log(f"cpu baseline: probe at stride {stride} took {probe:.2f} s")
         if probe >= target_s / 20 or stride == 1:
             break
@@ -211,12 +267,7 @@ def cpu_baseline(g, queries, target_s: float, threads: int):
     if probe > 0:
         stride = int(max(1, min(V, stride * probe / target_s)))
     for _ in range(4):
-        total_cnt, total_s = 0, 0.0
-        for _, _, q in queries:
-            r = orc.match(og, q.vertices, q.edges(), threads=threads, table=False, roots=(stride // 2, V),
-                          stride=stride)
-            total_cnt += r.count
-            total_s += r.seconds
+        total_cnt, total_s = sample(stride)
         log(f"cpu baseline: stride {stride}: {total_cnt} embeddings in {total_s:.2f} s")
         # thread start-up dominates a tiny probe: refine until the sample is
         # at least a third of the target
@@ -260,7 +311,7 @@ def intersect_bench(engine, pairs: int = 2048, mean_len: int = 32768, reps: int 
         ms.append(r["kernel_ms"])
     nbytes = 4 * (int(la.sum()) + int(lb.sum())) + 4 * inter
     gbs = nbytes / (np.mean(ms) / 1e3) / 1e9
-    out = {"kernel": "fm_isect_cut_kernel + fm_isect_kernel", "pairs": pairs, "elements": int(lens.sum()),
+    out = {"kernel": "fm_isect_plan_kernel + fm_isect_kernel", "pairs": pairs, "elements": int(lens.sum()),
            "bytes": nbytes, "kernel_ms": float(np.mean(ms)), "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
            "frac": gbs / HBM_PEAK_GBS, "checked": bool(ok), "traffic": None}
     # HBM bytes per launch of both kernels, measured with rocprofv3 FETCH_SIZE /
@@ -268,7 +319,7 @@ def intersect_bench(engine, pairs: int = 2048, mean_len: int = 32768, reps: int 
     tpath = ROOT / "profiles" / "fm_isect_kernel_intersect_2048pairs_32k_traffic.json"
     if tpath.exists() and pairs == 2048 and mean_len == 32768 and seed == 7:
         tr = json.loads(tpath.read_text())
-        if tr.get("algorithmic_bytes_per_launch") == nbytes:
+        if tr.get("algorithmic_bytes_per_launch") == nbytes and tr.get("kernel") == "fm_isect_kernel":
             out["traffic"] = tr["hbm_bytes_per_launch"]
             out["traffic_source"] = str(tpath.relative_to(ROOT))
     return out
@@ -389,8 +440,163 @@ def serve_mix_check(srv, pool, alone, rng, extra: int = 56) -> bool:
 
 def b_alg(o, n: int, table: bool) -> int:
     """Algorithmic bytes: neighbour words of every requirement row of every
-    partial embedding, an 8-byte record per partial, the final table."""
-    return o.alg_row_bytes + 8 * sum(o.partials) + (4 * n * o.count if table else 0)
+    partial embedding, an 8-byte record per stored partial, the final table.
+    A count-only query stores no final level: no record or row for it."""
+    records = sum(o.partials[:n]) if table else sum(o.partials[:max(n - 1, 0)])
+    return o.alg_row_bytes + 8 * records + (4 * n * o.count if table else 0)
+
+
+# A device-memory budget the benchmarks' explicitly sized arenas fit in
+# (MI355X: 288 GB of HBM3E).
+BENCH_PREALLOC = 270 * 10 ** 9
+
+
+def select_plain(engine, g, seed: int, nq: int, budget: int, tree_budget: int):
+    """Half trees (count mode, at most tree_budget embeddings), half with
+    >= 2 independent cycles (tables, at most budget), from seeded rounds of
+    candidates. Returns (chosen, counts, tables)."""
+    half = nq - nq // 2
+    trees, tcounts, cyc, ccounts = [], [], [], []
+    # about 1 walk tree of 6-8 vertices in 100 has at most 2^31 embeddings on
+    # the 2^20-vertex graph, 1 cyclic query in 30 at most 2^26
+    for rnd in range(48):
+        if len(trees) >= half and len(cyc) >= nq - half:
+            break
+        cands = make_candidates(g, 2, seed + 7919 * rnd, 4 * nq)
+        if len(trees) < half:
+            k, c = screen(engine, g, [x for x in cands if x[0] == 0], tree_budget, half - len(trees),
+                          max_partials=1 << 28)
+            trees += k
+            tcounts += c
+        if len(cyc) < nq - half:
+            k, c = screen(engine, g, [x for x in cands if x[0] == 1], budget, nq - half - len(cyc))
+            cyc += k
+            ccounts += c
+    return trees + cyc, tcounts + ccounts, [False] * len(trees) + [True] * len(cyc)
+
+
+def run_plain(engine, g, queries, tables, counts, steps: int, warmup: int, validate: bool):
+    """One launch per query; a step is every query once."""
+    for _ in range(warmup):
+        for q, t in zip(queries, tables):
+            engine.match(g, q, table=t, timeout_ms=300000).close()
+    step_times, kernel_ms, lat, busy = [], np.zeros(len(queries)), [], []
+    splits = requeues = tasks = 0
+    last = []
+    for _ in range(steps):
+        ts = time.perf_counter()
+        last = []
+        for i, (q, t) in enumerate(zip(queries, tables)):
+            tq = time.perf_counter()
+            r = engine.match(g, q, table=t, timeout_ms=300000)
+            lat.append((time.perf_counter() - tq) * 1e3)
+            st = r.stats()
+            kernel_ms[i] += st.kernel_ms
+            busy.append(st.busy_frac * st.kernel_ms)
+            splits += st.tasks_split
+            requeues += st.requeues
+            tasks += st.tasks_pushed
+            last.append(r.query(0))
+            r.close()
+        step_times.append(time.perf_counter() - ts)
+    kernel_ms /= steps
+    ok = all(o.status == 0 and o.count == counts[i] and o.rows == (o.count if tables[i] else 0)
+             for i, o in enumerate(last))
+    # table validity on a sample (untimed)
+    rng = np.random.default_rng(1)
+    for i in range(0, len(queries) if validate else 0, 4):
+        if tables[i]:
+            r = engine.match(g, queries[i], timeout_ms=300000)
+            ok = ok and validate_sample(g, queries[i], r.table(0, sorted=False, max_rows=1 << 16), rng)
+            r.close()
+    ms_step = 1e3 * float(np.mean(step_times))
+
+    def half(sel):
+        idx = [i for i in range(len(queries)) if sel(i)]
+        emb = sum(last[i].count for i in idx)
+        kms = float(sum(kernel_ms[i] for i in idx))
+        return {"queries": len(idx), "embeddings_per_step": emb, "kernel_ms_per_step": kms,
+                "kernel_embeddings_per_s": emb / (kms / 1e3) if kms > 0 else None}
+
+    return {"ms_per_step": ms_step, "kernel_ms": kernel_ms, "outcomes": last,
+            "embeddings_per_step": sum(o.count for o in last),
+            "latency_ms_p50": float(np.percentile(lat, 50)), "latency_ms_p99": float(np.percentile(lat, 99)),
+            "wave_busy_frac": float(sum(busy) / (float(np.sum(kernel_ms)) * steps)),
+            "per_step": {"tasks": tasks / steps, "tasks_split": splits / steps, "requeues": requeues / steps},
+            "count_only_trees": half(lambda i: not tables[i]), "cyclic_tables": half(lambda i: tables[i]),
+            "parity_checks_passed": bool(ok)}
+
+
+def run_batched(g, seed: int, nq: int, budget: int, steps: int, warmup: int):
+    """The batched form: nq queries of 4..12 vertices in one persistent
+    launch (knobs from the autotuner) against the same queries one launch
+    each. Returns the leg's results and its queries."""
+    from forkmatch_amd.autotune import autotune
+    # 1,024 tables resident at once: a larger table arena
+    engine = fm.Engine(output_prealloc=96 << 30, prealloc=BENCH_PREALLOC)
+    t1 = time.time()
+    chosen, counts = select_mixed(engine, g, 3, seed, nq, budget)
+    queries = [c[2] for c in chosen]
+    log(f"batched leg: {len(queries)} queries selected in {time.time() - t1:.1f} s")
+    ri = engine.match_batch(g, queries, table=False, instrument=True, timeout_ms=300000)
+    balg = [b_alg(ri.query(i), chosen[i][1], table=True) for i in range(len(queries))]
+    ri.close()
+    space = {"root_chunk": [1024, 4096], "task_rows": [1024, 4096], "grid_frac": [1.0, 1.5]}
+    tune = autotune(engine, g, queries, space=space, table=False, log=log)
+    best = tune.best
+    log(f"autotune: best {best} {tune.best_ms:.2f} ms over {len(tune.trials)} configurations")
+    # plain: one launch per query, with the library's single-query defaults
+    # (the fastest measured for one query; the batch's tuned knobs are slower
+    # there)
+    for q in queries[:16]:
+        engine.match(g, q, timeout_ms=300000).close()
+    tp = time.perf_counter()
+    plain_counts = []
+    for q in queries:
+        r = engine.match(g, q, timeout_ms=300000)
+        plain_counts.append(r.count(0))
+        r.close()
+    plain_s = time.perf_counter() - tp
+    for _ in range(warmup):
+        engine.match_batch(g, queries, timeout_ms=300000, **best).close()
+    step_times, kms = [], []
+    rb = None
+    for _ in range(steps):
+        if rb is not None:
+            rb.close()
+        ts = time.perf_counter()
+        rb = engine.match_batch(g, queries, timeout_ms=300000, **best)
+        step_times.append(time.perf_counter() - ts)
+        kms.append(rb.stats().kernel_ms)
+    st = rb.stats()
+    ok = all(rb.query(i).count == counts[i] == plain_counts[i] and rb.query(i).status == 0 and
+             rb.query(i).rows == counts[i] for i in range(len(queries)))
+    embeddings = sum(rb.query(i).count for i in range(len(queries)))
+    rb.close()
+    ms_step = 1e3 * float(np.mean(step_times))
+    value = len(queries) / (ms_step / 1e3)
+    dom_ms = float(np.mean(kms))
+    achieved = sum(balg) / (dom_ms / 1e3) / 1e9
+    mix = {name: sum(1 for c in chosen if c[0] == k) for k, name in ((0, "tree"), (1, "cycle+chord"),
+                                                                     (2, "near-clique"))}
+    out = {
+        "queries": len(queries), "batched_queries_per_s": value, "batched_ms_per_launch": ms_step,
+        "plain_queries_per_s": len(queries) / plain_s, "batched_over_plain": value / (len(queries) / plain_s),
+        "autotune": {"best": best, "best_kernel_ms": tune.best_ms, "trials": len(tune.trials)},
+        "shape_mix": mix,
+        "shape_mix_note": "caps 40/40/20; trees with at most the table budget of embeddings are rare on this "
+                          "graph, and their share is filled by the other shapes",
+        "table_budget": budget, "embeddings_per_launch": embeddings,
+        "embeddings_per_s": embeddings / (ms_step / 1e3), "wave_busy_frac": st.busy_frac,
+        "launch": {"tasks": int(st.tasks_pushed), "tasks_split": int(st.tasks_split), "requeues": int(st.requeues)},
+        "parity_checks_passed": bool(ok),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "alg_bytes_per_launch": sum(balg), "kernel": "fm_match_kernel_batch",
+                     "kernel_ms_per_launch": dom_ms},
+    }
+    engine.close()
+    return out, chosen
 
 
 def main():
@@ -400,9 +606,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--seed", type=int, default=0xF0F2)
-    ap.add_argument("--budget", type=int, default=0, help="max embeddings per query (default 2^26, 2^22 for config 3)")
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--budget", type=int, default=0,
+                    help="max embeddings per query with a table (default 2^26, 2^22 for config 3)")
+    ap.add_argument("--tree-budget", type=int, default=1 << 31,
+                    help="config 2: max embeddings of a count-only tree query")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline: seconds per leg")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="config 2: leave out the batched leg")
     ap.add_argument("--queries", type=int, default=0)
     ap.add_argument("--save-queries", default="", help="write the selected queries + manifest here")
     ap.add_argument("--load-queries", default="", help="reuse a saved query set (no screening launches)")
@@ -438,39 +648,51 @@ def main():
     g.upload()
     gi = g.info()
     log(f"graph: V={gi.vertices} E={gi.edges} labels={gi.labels} built in {time.time() - t0:.1f}s")
+    graph_desc = f"rmat scale {gi.vertices.bit_length() - 1}, {gi.edges} directed edges, {gi.labels} labels"
+
     if args.config == 3:
-        # 1,024 tables resident at once: a larger table arena
-        engine = fm.Engine(output_prealloc=96 << 30)
-    elif args.config == 4:
+        res, chosen = run_batched(g, args.seed, args.queries or 1024, args.budget, args.steps, args.warmup)
+        with fm.Engine() as engine:
+            res["intersect_roofline"] = intersect_bench(engine)
+        cpu = None
+        if not args.no_cpu:
+            cpu = cpu_baselines(g, chosen[:32], args.cpu_seconds)
+        line = {"metric": "queries/s (batched, one persistent launch)", "value": res.pop("batched_queries_per_s"),
+                "unit": "queries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                "ms_per_step": res.pop("batched_ms_per_launch"), "higher_is_better": True, "scaling": "weak",
+                "vs_baseline": None, "dtype": "u32", "data": "synthetic (seeded R-MAT graph and random-walk queries)",
+                "config": {"workload": f"rmat20_batched_{len(chosen)}q", "graph": graph_desc,
+                           "queries": len(chosen), "budget": args.budget, "seed": args.seed},
+                "roofline": res.pop("roofline"), "cpu_baseline": cpu}
+        line.update(res)
+        print(json.dumps(line), flush=True)
+        return
+
+    if args.config == 4:
         # CurBankSize = 1 << 20 (8 MiB banks); 64 GiB of banks, 64 GiB of pages
-        engine = fm.Engine(soln_prealloc=64 << 30, output_prealloc=64 << 30, bank_size=1 << 20)
+        engine = fm.Engine(soln_prealloc=64 << 30, output_prealloc=64 << 30, bank_size=1 << 20,
+                           prealloc=BENCH_PREALLOC)
     elif args.config == 5:
         # tables up to ~100 GB resident at once
-        engine = fm.Engine(soln_prealloc=48 << 30, output_prealloc=160 << 30)
+        engine = fm.Engine(soln_prealloc=48 << 30, output_prealloc=160 << 30, prealloc=BENCH_PREALLOC)
     else:
         engine = fm.Engine()
     ei = engine.info()
     log(f"engine: {ei.device_name.decode()} CUs={ei.compute_units} waves/CU={ei.waves_per_cu} "
         f"grid={ei.grid_waves} banks={ei.banks}x{ei.bank_bytes}B")
 
-    nq = args.queries or {2: 32, 3: 1024, 4: 512, 5: 64}[args.config]
+    nq = args.queries or {2: 32, 4: 512, 5: 64}[args.config]
     t1 = time.time()
     manifest = None
+    tables = None
     if args.load_queries:
         manifest = json.loads((Path(args.load_queries) / "manifest.json").read_text())
         chosen = [(m["shape"], m["n"], fm.Query.load_g(str(Path(args.load_queries) / m["file"])))
                   for m in manifest["queries"]]
         counts = [m["count"] for m in manifest["queries"]]
+        tables = [m.get("table", True) for m in manifest["queries"]]
     elif args.config == 2:
-        cands = make_candidates(g, 2, args.seed, 4 * nq)
-        chosen, counts = screen(engine, g, cands, args.budget, nq, per_shape={3: nq // 2, 1: nq - nq // 2})
-        if len(chosen) < nq:
-            more = make_candidates(g, 2, args.seed + 1, 16 * nq)
-            c2, n2 = screen(engine, g, more, args.budget, nq - len(chosen),
-                            per_shape={3: nq // 2 - sum(1 for c in chosen if c[0] == 3),
-                                       1: nq - nq // 2 - sum(1 for c in chosen if c[0] == 1)})
-            chosen += c2
-            counts += n2
+        chosen, counts, tables = select_plain(engine, g, args.seed, nq, args.budget, args.tree_budget)
     elif args.config == 5:
         # Dense 10-vertex patterns on this graph have far more than 10^9
         # embeddings each (the hub core of the R-MAT graph holds them), so
@@ -479,6 +701,8 @@ def main():
         counts = [None] * len(chosen)
     else:
         chosen, counts = select_mixed(engine, g, args.config, args.seed, nq, args.budget)
+    if tables is None:
+        tables = [True] * len(chosen)
     log(f"selected {len(chosen)} queries in {time.time() - t1:.1f}s; "
         f"total embeddings {sum(counts) if None not in counts else 'capped'}")
     queries = [c[2] for c in chosen]
@@ -493,7 +717,7 @@ def main():
         rows_bytes = sum(m["row_bytes"] for m in manifest["queries"])
     else:
         ri = engine.match_batch(g, queries, table=False, instrument=True, timeout_ms=300000)
-        balg = [b_alg(ri.query(i), ns[i], table=True) for i in range(len(queries))]
+        balg = [b_alg(ri.query(i), ns[i], table=tables[i]) for i in range(len(queries))]
         row_b = [ri.query(i).alg_row_bytes for i in range(len(queries))]
         rows_bytes = sum(row_b)
         ri.close()
@@ -504,62 +728,44 @@ def main():
             for i, (shape, n, q) in enumerate(chosen):
                 q.write_g(str(d / f"q{i:04d}.g"))
                 items.append({"file": f"q{i:04d}.g", "shape": shape, "n": n, "count": counts[i], "balg": balg[i],
-                              "row_bytes": row_b[i]})
+                              "row_bytes": row_b[i], "table": tables[i]})
             (d / "manifest.json").write_text(json.dumps({"config": args.config, "seed": args.seed,
                                                          "budget": args.budget, "queries": items}))
 
     result = {}
     if args.config == 2:
-        for _ in range(args.warmup):
-            for q in queries:
-                engine.match(g, q, timeout_ms=300000)
-        step_times, kernel_ms, lat, busy = [], np.zeros(len(queries)), [], []
-        last = []
-        for _ in range(args.steps):
-            ts = time.perf_counter()
-            last = []
-            for i, q in enumerate(queries):
-                tq = time.perf_counter()
-                r = engine.match(g, q, timeout_ms=300000)
-                lat.append((time.perf_counter() - tq) * 1e3)
-                st = r.stats()
-                kernel_ms[i] += st.kernel_ms
-                busy.append(st.busy_frac * st.kernel_ms)
-                last.append(r.query(0))
-                r.close()
-            step_times.append(time.perf_counter() - ts)
-        kernel_ms /= args.steps
-        embeddings = sum(o.count for o in last)
-        ok = all(o.status == 0 and o.count == counts[i] and o.rows == o.count for i, o in enumerate(last))
-        # table validity on a sample (untimed; skipped for a loaded query set
-        # so that a profiled run holds only the timed launches)
-        rng = np.random.default_rng(1)
-        for i in range(0, len(queries) if manifest is None else 0, 4):
-            r = engine.match(g, queries[i], timeout_ms=300000)
-            ok = ok and validate_sample(g, queries[i], r.table(0, sorted=False, max_rows=1 << 16), rng)
-            r.close()
-        ms_step = 1e3 * float(np.mean(step_times))
-        value = embeddings / (ms_step / 1e3)
-        dom_ms = float(np.sum(kernel_ms))
+        pr = run_plain(engine, g, queries, tables, counts, args.steps, args.warmup, validate=manifest is None)
+        ntree = sum(1 for t in tables if not t)
+        workload = f"rmat20_plain_{len(queries)}q_{ntree}trees_count_{len(queries) - ntree}cyclic_table"
+        dom_ms = float(np.sum(pr["kernel_ms"]))
+        nl = len(queries)
         achieved = sum(balg) / (dom_ms / 1e3) / 1e9
+        embeddings = pr["embeddings_per_step"]
         result = {
             "metric": "embeddings/s (plain, one launch per query)",
-            "value": value,
+            "value": embeddings / (pr["ms_per_step"] / 1e3),
             "unit": "embeddings/s",
-            "ms_per_step": ms_step,
-            "queries_per_s": len(queries) / (ms_step / 1e3),
+            "ms_per_step": pr["ms_per_step"],
+            "queries_per_s": nl / (pr["ms_per_step"] / 1e3),
             "kernel_embeddings_per_s": embeddings / (dom_ms / 1e3),
-            "latency_ms_p50": float(np.percentile(lat, 50)),
-            "latency_ms_p99": float(np.percentile(lat, 99)),
-            "wave_busy_frac": float(sum(busy) / (dom_ms * args.steps)),
-            "parity_checks_passed": bool(ok),
+            "latency_ms_p50": pr["latency_ms_p50"],
+            "latency_ms_p99": pr["latency_ms_p99"],
+            "wave_busy_frac": pr["wave_busy_frac"],
+            "parity_checks_passed": pr["parity_checks_passed"],
             "embeddings_per_step": embeddings,
+            "count_only_trees": pr["count_only_trees"],
+            "cyclic_tables": pr["cyclic_tables"],
+            "launch_per_step": pr["per_step"],
+            # per launch: algorithmic bytes (mean over the step's launches)
+            # over the kernel's mean launch time; traffic (rocprofv3) per
+            # launch too
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                         "alg_bytes_per_step": sum(balg), "row_bytes_per_step": rows_bytes,
-                         "kernel": "fm_match_kernel", "kernel_ms_per_step": dom_ms},
+                         "alg_bytes_per_launch": sum(balg) / nl, "kernel_ms_per_launch": dom_ms / nl,
+                         "launches_per_step": nl, "alg_bytes_per_step": sum(balg),
+                         "row_bytes_per_step": rows_bytes, "kernel": "fm_match_kernel<true>",
+                         "kernel_ms_per_step": dom_ms},
         }
-        workload = "rmat20_plain_32q"
         # HBM bytes per launch, measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
         # on this workload and calibrated against a copy of known size
         tpath = ROOT / "profiles" / "fm_match_kernel_rmat20_plain_traffic.json"
@@ -567,6 +773,7 @@ def main():
             tr = json.loads(tpath.read_text())
             if tr.get("workload") == workload and tr.get("embeddings_per_step") == embeddings:
                 result["roofline"]["traffic"] = tr["hbm_bytes_per_launch"]
+                result["roofline"]["traffic_per_step"] = tr["hbm_bytes_per_launch"] * nl
                 result["roofline"]["traffic_source"] = str(tpath.relative_to(ROOT))
     elif args.config == 4:
         rng = np.random.default_rng(args.seed)
@@ -670,81 +877,21 @@ def main():
                          "kernel": "fm_match_kernel", "kernel_ms_per_step": dom_ms},
         }
         workload = f"rmat24_stress_{len(queries)}q_dense10"
-    else:
-        from forkmatch_amd.autotune import autotune
-        space = {"root_chunk": [1024, 4096], "task_rows": [1024, 4096], "grid_frac": [1.0, 1.5]}
-        tune = autotune(engine, g, queries, space=space, table=False, log=log)
-        best = tune.best
-        log(f"autotune: best {best} {tune.best_ms:.2f} ms over {len(tune.trials)} configurations")
-        # plain: one launch per query, with the library's single-query
-        # defaults (1,024-parent tasks, 8 waves per CU of which 4 start at
-        # once, two candidates per lane: the fastest measured for one query;
-        # the batch's tuned knobs are slower there)
-        plain_opts = {}
-        for q in queries[:16]:
-            engine.match(g, q, timeout_ms=300000, **plain_opts).close()
-        tp = time.perf_counter()
-        plain_counts = []
-        for q in queries:
-            r = engine.match(g, q, timeout_ms=300000, **plain_opts)
-            plain_counts.append(r.count(0))
-            r.close()
-        plain_s = time.perf_counter() - tp
-        # batched: all queries in one persistent launch
-        for _ in range(args.warmup):
-            engine.match_batch(g, queries, timeout_ms=300000, **best).close()
-        step_times, kms = [], []
-        rb = None
-        for _ in range(args.steps):
-            if rb is not None:
-                rb.close()
-            ts = time.perf_counter()
-            rb = engine.match_batch(g, queries, timeout_ms=300000, **best)
-            step_times.append(time.perf_counter() - ts)
-            kms.append(rb.stats().kernel_ms)
-            busy_b = rb.stats().busy_frac
-        ok = all(rb.query(i).count == counts[i] == plain_counts[i] and rb.query(i).status == 0
-                 for i in range(len(queries)))
-        embeddings = sum(rb.query(i).count for i in range(len(queries)))
-        ms_step = 1e3 * float(np.mean(step_times))
-        value = len(queries) / (ms_step / 1e3)
-        dom_ms = float(np.mean(kms))
-        achieved = sum(balg) / (dom_ms / 1e3) / 1e9
-        result = {
-            "metric": "queries/s (batched, one persistent launch)",
-            "value": value,
-            "unit": "queries/s",
-            "ms_per_step": ms_step,
-            "plain_queries_per_s": len(queries) / plain_s,
-            "batched_over_plain": value / (len(queries) / plain_s),
-            "autotune": {"best": best, "best_kernel_ms": tune.best_ms, "trials": len(tune.trials)},
-            "shape_mix": {name: sum(1 for c in chosen if c[0] == k)
-                          for k, name in ((0, "tree"), (1, "cycle+chord"), (2, "near-clique"))},
-            "embeddings_per_step": embeddings,
-            "embeddings_per_s": embeddings / (ms_step / 1e3),
-            "wave_busy_frac": busy_b,
-            "parity_checks_passed": bool(ok),
-            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": achieved / HBM_PEAK_GBS, "traffic": None, "alg_bytes_per_step": sum(balg),
-                         "kernel": "fm_match_kernel", "kernel_ms_per_step": dom_ms},
-        }
-        workload = "rmat20_batched_1024q"
 
     if not args.load_queries:
         result["intersect_roofline"] = intersect_bench(engine)
+    engine.close()
+
+    if args.config == 2 and not args.no_batched and not args.load_queries:
+        # the batched form on the same graph (its own engine: 1,024 tables)
+        res, _ = run_batched(g, args.seed, 1024, 1 << 22, args.steps, args.warmup)
+        result["batched"] = res
 
     cpu = None
     # config 5: the oracle's sorted copy of 2^29 edge lines alone takes minutes
     # of host time, so the stress line carries no CPU baseline
     if not args.no_cpu and args.config != 5:
-        threads = min(16, os.cpu_count() or 1)
-        sample = chosen[: min(len(chosen), 32)]
-        cnt, secs, stride = cpu_baseline(g, sample, args.cpu_seconds, threads)
-        cpu = {"value": cnt / secs if secs > 0 else None, "unit": "embeddings/s", "cores": threads,
-               "kind": "port",
-               "sample": f"oracle on {len(sample)} of the queries, roots {stride // 2}, {stride + stride // 2}, "
-                         f"{2 * stride + stride // 2}, ... "
-                         f"({cnt} embeddings in {secs:.1f}s)"}
+        cpu = cpu_baselines(g, chosen[: min(len(chosen), 32)], args.cpu_seconds)
 
     line = {
         "metric": result.pop("metric"),
@@ -759,15 +906,15 @@ def main():
         "vs_baseline": None,
         "dtype": "u32",
         "data": "synthetic (seeded R-MAT graph and random-walk queries)",
-        "config": {"workload": workload,
-                   "graph": f"rmat scale {gi.vertices.bit_length() - 1}, {gi.edges} directed edges, {gi.labels} labels",
-                   "queries": len(queries), "budget": args.budget, "seed": args.seed},
+        "config": {"workload": workload, "graph": graph_desc, "queries": len(queries), "budget": args.budget,
+                   "seed": args.seed},
         "roofline": result.pop("roofline"),
         "cpu_baseline": cpu,
     }
+    if args.config == 2:
+        line["config"]["tree_budget"] = args.tree_budget
     line.update(result)
     print(json.dumps(line), flush=True)
-    engine.close()
 
 
 if __name__ == "__main__":

@@ -30,6 +30,8 @@ FM_RESULT_TRUNCATED = 1
 FM_RESULT_ABORTED = 2
 PLAN_COST = 0       # order from the data graph's label statistics
 PLAN_REFERENCE = 1  # the reference's query-degree order
+PREALLOC_UNLIMITED = (1 << 64) - 1  # Engine(prealloc=...): no device-memory budget
+NO_DEADLINE = 0xFFFFFFFF  # match(timeout_ms=...): the launch has no deadline
 
 STATUS_NAMES = {
     0: "FM_OK", 1: "FM_ERR_GENERIC", 12: "FM_ERR_DATA_MISSING_FILE", 76: "FM_ERR_DATA_BAD_START",
@@ -349,8 +351,13 @@ class Engine:
     """Device context: bank arena, output arena, work queues, stream."""
 
     def __init__(self, soln_prealloc: int = 0, output_prealloc: int = 0, bank_size: int = 0,
-                 output_page_bytes: int = 0, queue_capacity: int = 0, max_queries: int = 0, device: int = 0):
+                 output_page_bytes: int = 0, queue_capacity: int = 0, max_queries: int = 0, device: int = 0,
+                 prealloc: int = 0):
+        """prealloc: device bytes the arenas and the matched graph may take
+        together (PREALLOC; 0 = the build default, PREALLOC_UNLIMITED = no
+        limit). Beyond it the engine and the launches fail with status 1."""
         cfg = L.EngineConfig()
+        cfg.prealloc = prealloc
         cfg.soln_prealloc = soln_prealloc
         cfg.output_prealloc = output_prealloc
         cfg.bank_size = bank_size
@@ -547,9 +554,9 @@ def intersect_pairs(engine: Engine, data, a_off, a_len, b_off, b_len, out_off=No
     oo = DeviceBuffer.from_array(np.asarray(out_off, np.uint64)) if out_off is not None else None
     out = DeviceBuffer(4 * max(out_words, 1)) if out_off is not None else None
     ms = C.c_double()
-    _check(L.lib().fm_intersect_pairs(engine._h, bufs["data"].ptr, bufs["a_off"].ptr, bufs["a_len"].ptr,
-                                      bufs["b_off"].ptr, bufs["b_len"].ptr, npairs, counts.ptr,
-                                      oo.ptr if oo else None, out.ptr if out else None, C.byref(ms)))
+    fn = L.lib().fm_intersect_pairs
+    _check(fn(engine._h, bufs["data"].ptr, bufs["a_off"].ptr, bufs["a_len"].ptr, bufs["b_off"].ptr, bufs["b_len"].ptr,
+              npairs, counts.ptr, oo.ptr if oo else None, out.ptr if out else None, C.byref(ms)))
     res = {"counts": counts.to_array(np.uint32, npairs), "kernel_ms": ms.value, "buffers": bufs}
     res["out"] = out.to_array(np.uint32, out_words) if out is not None else None
     return res

@@ -30,13 +30,13 @@ class PlanInfo(C.Structure):
 class EngineConfig(C.Structure):
     _fields_ = [("soln_prealloc", u64), ("output_prealloc", u64), ("bank_size", u32),
                 ("output_page_bytes", u32), ("queue_capacity", u32), ("max_queries", u32),
-                ("device", i32), ("reserved", u32 * 7)]
+                ("device", i32), ("reserved0", u32), ("prealloc", u64), ("reserved", u32 * 4)]
 
 
 class EngineInfo(C.Structure):
     _fields_ = [("compute_units", u32), ("waves_per_cu", u32), ("grid_waves", u32), ("banks", u32),
                 ("bank_bytes", u64), ("output_pages", u32), ("output_page_bytes", u32),
-                ("device_name", C.c_char * 64)]
+                ("device_name", C.c_char * 64), ("arena_bytes", u64)]
 
 
 class MatchOpts(C.Structure):
@@ -71,7 +71,7 @@ class ServerCompletion(C.Structure):
 
 class ServerStats(C.Structure):
     _fields_ = [("submitted", u64), ("completed", u64), ("refused", u64), ("launches", u64), ("in_flight", u32),
-                ("live_banks", u32), ("dropped_tasks", u64)]
+                ("live_banks", u32), ("dropped_tasks", u64), ("refused_memory", u64)]
 
 
 # name -> (restype, argtypes); every function declared in include/forkmatch.h

@@ -6,8 +6,12 @@
 // "End to End Time" and "Pop Count".
 //
 // Knobs: PREALLOC-family defaults come from environment.h; each may be
-// overridden by an environment variable of the same name (SOLN_PREALLOC,
-// OUTPUT_PREALLOC, CurBankSize). FM_TABLE_OUT=<path> writes the sorted
+// overridden by an environment variable of the same name (PREALLOC,
+// SOLN_PREALLOC, OUTPUT_PREALLOC, CurBankSize). PREALLOC bounds the arenas
+// plus the graph; beyond it the CLI exits with status 1, as the reference
+// does when its slab overflows (src/trie.cuh:1034-1049). The match runs to
+// completion like the reference's (no deadline); FM_TIMEOUT_MS=<ms> sets
+// one. FM_TABLE_OUT=<path> writes the sorted
 // embedding table as text, one row per line, columns in query vertex order.
 // FM_COUNT_ONLY=1 skips table construction (the reference's WriteSolnFlag,
 // src/environment.cuh:65).
@@ -50,6 +54,7 @@ int main(int argc, char **argv) {
     cfg.soln_prealloc = env_u64("SOLN_PREALLOC", SOLN_PREALLOC);
     cfg.output_prealloc = env_u64("OUTPUT_PREALLOC", OUTPUT_PREALLOC);
     cfg.bank_size = (uint32_t)env_u64("CurBankSize", CurBankSize);
+    cfg.prealloc = env_u64("PREALLOC", PREALLOC);
     fm_engine *e = nullptr;
     rc = fm_engine_create(&cfg, &e);
     if (rc) return die(rc > 255 ? 1 : rc);
@@ -59,6 +64,7 @@ int main(int argc, char **argv) {
     printf("\nParse Time: %fs\n", std::chrono::duration<double>(parsed - start).count());
     fm_match_opts opts{};
     opts.mode = env_u64("FM_COUNT_ONLY", 0) ? FM_MODE_COUNT : FM_MODE_TABLE;
+    opts.timeout_ms = (uint32_t)env_u64("FM_TIMEOUT_MS", FM_NO_DEADLINE);
     fm_result *r = nullptr;
     rc = fm_match(e, g, q, &opts, &r);
     if (rc && !r) return die(rc > 255 ? 1 : rc);

@@ -40,6 +40,9 @@ namespace fm {
 constexpr uint32_t WAVE = 64;
 constexpr uint32_t MAXN = FM_MAX_QUERY_VERTICES;
 constexpr uint32_t MAXR = FM_MAX_REQS_PER_DEPTH;
+// Smallest bank (8-byte units): a fresh bank holds one paired candidate
+// step's children, 2 x WAVE rows of at most MAXN words.
+constexpr uint32_t MIN_BANK_SIZE = WAVE * MAXN;
 constexpr uint32_t NLEVELS = MAXN;
 constexpr uint32_t LABEL_ABSENT = 0xFFFFFFu;
 constexpr uint32_t ROOT_BANK = 0xFFFFFFFFu;
@@ -963,6 +966,12 @@ struct Lds {
     uint32_t didx[WAVE];                // driver requirement index
     // instrumented launches: per-task sums, flushed once per task (lane 0)
     unsigned long long inst[4 + MAXN];  // row words, lookups, candidates, probes, partials
+    // instrumented launches, per parent of the batch: its requirement row
+    // words, and flags (bit 0: alive and its driver range starts in this
+    // task at candidate 0; bit 1: its driver row is empty). A parent is
+    // counted once, by the task whose processed range holds its candidate 0.
+    unsigned long long acct_words[WAVE];
+    uint32_t acct_flags[WAVE];
 };
 
 // Pushes the rows written since run_begin as tasks of at most `per`
@@ -1126,6 +1135,7 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
                 alive = row_contains(g.col, g.skip, st, len, v);
             }
         }
+        uint32_t acct = 0;
         if (lane < nb) {
             uint32_t best = 0xFFFFFFFFu;
             for (uint32_t i = 0; i < k; i++) {
@@ -1143,9 +1153,12 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
             }
             uint32_t endv = best;
             if (b0 + lane == cnt_local - 1) endv = min(endv, last_end);
-            const uint32_t startv = (b0 == 0 && lane == 0) ? min(t.skip, endv) : 0;
+            const bool first_par = b0 == 0 && lane == 0;
+            const uint32_t startv = first_par ? min(t.skip, endv) : 0;
             dst_ += startv;
             dlen = alive ? endv - startv : 0;
+            acct = (alive && !(first_par && t.skip)) ? 1u : 0u;
+            if (best == 0) acct |= 2u;
         }
         s.dstart[lane] = dst_;
         s.didx[lane] = didx;
@@ -1153,16 +1166,31 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
         s.cum[lane] = incl;
         uint32_t T = bcast(incl, WAVE - 1);
         if (P.instrument) {
-            unsigned long long wsum = wave_sum_u64(alive ? words : 0ull);
-            uint32_t nalive = __popcll(__ballot(alive));
-            if (lane == 0) {
-                s.inst[0] += wsum;
-                s.inst[1] += (unsigned long long)nalive * k;
-                if (root) s.inst[4 + 0] += nalive;
-                s.inst[2] += T;
-            }
+            s.acct_words[lane] = words;
+            s.acct_flags[lane] = acct;
         }
         __syncthreads();
+        // Instrumented launches: the batch's algorithmic counters once its
+        // processed candidates are final ([0, Tfin) of the flattened range:
+        // a split or a yield hands the rest to another task). A parent's row
+        // words, lookups and root record are counted by the one task whose
+        // processed range holds its candidate 0 (with an empty driver row:
+        // the task that holds the parent), so the sums do not depend on how
+        // the work was split.
+        auto account = [&](uint32_t Tfin) {
+            if (!P.instrument) return;
+            const uint32_t fl = s.acct_flags[lane];
+            const uint32_t hi = s.cum[lane], lo = lane ? s.cum[lane - 1] : 0u;
+            const bool c = (fl & 1u) && ((fl & 2u) ? hi <= Tfin : (lo < hi && lo < Tfin));
+            const unsigned long long wsum = wave_sum_u64(c ? s.acct_words[lane] : 0ull);
+            const uint32_t nc = __popcll(__ballot(c));
+            if (lane == 0) {
+                s.inst[0] += wsum;
+                s.inst[1] += (unsigned long long)nc * k;
+                if (root) s.inst[4 + 0] += nc;
+                s.inst[2] += Tfin;
+            }
+        };
         // ---- candidates, 64 per step ----
         // Each step's candidate word (and its parent) is loaded during the
         // step before, so the load overlaps that step's probes. The position
@@ -1262,6 +1290,7 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
             }
             if (!final_level) {
                 const uint32_t need = STEP * (D + 1);
+                static_assert(2 * WAVE * MAXN <= 2 * MIN_BANK_SIZE, "a fresh bank holds one step of children");
                 bool yield = false;
                 if (o.bank != NO_BANK && o.used + need > P.bank_words) {
                     // The output bank is full: hand its rows over as tasks and
@@ -1309,6 +1338,7 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
                     r.skip = (base - excl) + ((b0 == 0 && lo == 0) ? t.skip : 0);
                     r.last_end = last_end;
                     if (lane == 0) stats[ST_REQUEUE]++;
+                    account(base);
                     push_tasks(P, lvl_self, 1, r, t.bank, stats);
                     return;
                 }
@@ -1426,6 +1456,7 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
                 if ((o.used - o.run_begin) / CW >= run_rows) flush_run(P, o, t.q, D, lvl_children, run_rows, stats);
             }
         }
+        account(T);
         if (final_level && lane == 0 && cnt_acc >= (1ull << 20)) {
             unsigned long long now_cnt = atom_add(&Q.count, cnt_acc) + cnt_acc;
             cnt_acc = 0;
@@ -1633,7 +1664,12 @@ __device__ __forceinline__ void match_waves(const KParams &P) {
                 TaskDesc t;
                 const bool first = first_claim;
                 first_claim = false;
-                if (pop_task(P, t) || (!P.serving && claim_root(P, t, stats, first))) {
+                // A starting wave's own chunk (workgroup id) is claimed before
+                // anything else: no other wave takes it (root_cursor starts
+                // past the starting waves' chunks), so skipping it would leave
+                // its roots unmatched and the launch waiting for them.
+                if ((first && !P.serving && claim_root(P, t, stats, true)) || pop_task(P, t) ||
+                    (!P.serving && claim_root(P, t, stats, false))) {
                     s_task = t;
                     flag = 0;
                 } else if (P.serving) {
@@ -1773,19 +1809,29 @@ __global__ void fm_init_slots_kernel(Slot *ring, uint32_t qcap, uint32_t levels,
 // Intersection microkernel: |A n B| and the ordered intersection for a batch
 // of pairs of sorted u32 lists (A = the shorter list of a pair).
 //
-// Each pair is cut into tiles, planned on the host from the list lengths:
+// Each pair is cut into tiles by its list lengths:
 //  - merge tiles (lengths within 32x): tile k covers merge-path diagonals
 //    [k*XT, (k+1)*XT) of merge(A, B);
 //  - skew tiles (longer >= 32x shorter): tile k covers A[k*IB, (k+1)*IB) and
 //    the range of B between its first and last value.
-// fm_isect_cut_kernel finds every tile's cut, all tiles in parallel (one
-// global merge-path or range search each). fm_isect_kernel runs persistent
-// 256-thread blocks that claim whole pairs, largest first, and stream their
-// tiles: a merge tile is staged in LDS (plus the next B element, so a match
-// on the tile's last A element is seen), each thread merges a 15-element
-// stretch of the path, a block scan orders the matches, and they leave
-// through LDS in coalesced stores. Every list element is read from HBM once.
-
+// The plan is made on the device, so a call needs no host copy of the
+// lengths and no memory beyond one work array (`work`):
+//  - fm_isect_plan_kernel (one block): every pair's first tile (a scan of
+//    the tile counts), the pairs with tiles ordered largest first, counts
+//    and the claim counter cleared;
+//  - fm_isect_cut_kernel: one block per pair finds the cuts of its tiles
+//    (one merge-path or range search per tile, all in parallel);
+//  - fm_isect_kernel: persistent 256-thread blocks claim whole pairs in that
+//    order and stream their tiles: a merge tile is staged in LDS (plus the
+//    next B element, so a match on the tile's last A element is seen), each
+//    thread merges a 15-element stretch of the path, a block scan orders the
+//    matches, and they leave through LDS in coalesced stores. Every list
+//    element is read from HBM once.
+//
+// work (u32 words): [0] claim counter, [1] pairs with tiles, [2] tiles in
+// all, [3] set when the tiles exceed the array, then from word ISECT_HDR:
+// order[npairs], tile_begin[npairs + 1], cut[cap], cut2[cap].
+constexpr uint32_t ISECT_HDR = 64;
 constexpr uint32_t IB = 256;              // threads per block
 // Merge-path elements per thread. Odd: the threads' stretches start about
 // XVT / 2 words apart in each LDS slice, and an odd stride spreads a wave's
@@ -1867,26 +1913,103 @@ __device__ __forceinline__ bool isect_skewed(uint32_t na, uint32_t nb) {
     return (unsigned long long)nb >= (unsigned long long)ISKEW * na;
 }
 
-// One thread per tile: the tile's cuts. Merge tiles: cut = A elements before
-// the tile's first diagonal. Skew tiles: [cut, cut2) = the range of B that
-// holds the values of the tile's A chunk.
+// The tiles of pair p: none without an element in the shorter list.
+__device__ __forceinline__ uint32_t isect_tiles(uint32_t la, uint32_t lb) {
+    const uint32_t na = min(la, lb), nb = max(la, lb);
+    if (na == 0) return 0;
+    return isect_skewed(na, nb) ? (na + IB - 1) / IB
+                                : (uint32_t)(((unsigned long long)na + nb + XT - 1) / XT);
+}
+
+// One block of 1,024 threads. tile_begin by a scan of the tile counts, the
+// pairs with tiles in order of their bit length of tiles, largest first;
+// counts cleared; work[0] = 0 (claim counter), work[1] = pairs with tiles,
+// work[2] = tiles, work[3] = 1 (and no work) when the tiles exceed cap.
+__global__ __launch_bounds__(1024) void fm_isect_plan_kernel(const uint32_t *a_len, const uint32_t *b_len,
+                                                             uint32_t npairs, uint32_t *counts, uint32_t *work,
+                                                             uint32_t cap) {
+    constexpr uint32_t NB = 33;  // bit lengths of a tile count (< 2^32)
+    __shared__ uint32_t hist[NB], start[NB];
+    __shared__ unsigned long long s_part[1024 / WAVE];
+    __shared__ unsigned long long s_carry;
+    const uint32_t tid = threadIdx.x, lane = lane_id(), wid = tid / WAVE;
+    uint32_t *const order = work + ISECT_HDR;
+    uint32_t *const tile_begin = order + npairs;
+    if (tid < NB) hist[tid] = 0;
+    if (tid == 0) s_carry = 0;
+    __syncthreads();
+    for (uint32_t p0 = 0; p0 < npairs; p0 += 1024) {
+        const uint32_t p = p0 + tid;
+        const uint32_t nt = p < npairs ? isect_tiles(a_len[p], b_len[p]) : 0u;
+        if (p < npairs) counts[p] = 0;
+        if (nt) atomicAdd(&hist[(NB - 1) - (31 - __clz(nt))], 1u);
+        // block scan of the tile counts (64-bit: the total may pass 2^32)
+        unsigned long long v = nt;
+#pragma unroll
+        for (int o = 1; o < (int)WAVE; o <<= 1) {
+            const unsigned long long t = __shfl_up(v, o, WAVE);
+            if (lane >= (uint32_t)o) v += t;
+        }
+        if (lane == WAVE - 1) s_part[wid] = v;
+        __syncthreads();
+        unsigned long long pre = s_carry, tot = s_carry;
+        for (uint32_t w = 0; w < 1024 / WAVE; w++) {
+            if (w < wid) pre += s_part[w];
+            tot += s_part[w];
+        }
+        const unsigned long long begin = pre + v - nt;
+        if (p < npairs) tile_begin[p] = begin <= cap ? (uint32_t)begin : cap;
+        __syncthreads();
+        if (tid == 0) s_carry = tot;
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const unsigned long long total = s_carry;
+        tile_begin[npairs] = total <= cap ? (uint32_t)total : cap;
+        uint32_t acc = 0;
+        for (uint32_t b = 0; b < NB; b++) {
+            start[b] = acc;
+            acc += hist[b];
+        }
+        work[0] = 0;
+        work[1] = total <= cap ? acc : 0u;
+        work[2] = total <= cap ? (uint32_t)total : cap;
+        work[3] = total <= cap ? 0u : 1u;
+    }
+    __syncthreads();
+    for (uint32_t p = tid; p < npairs; p += 1024) {
+        const uint32_t nt = isect_tiles(a_len[p], b_len[p]);
+        if (nt) order[atomicAdd(&start[(NB - 1) - (31 - __clz(nt))], 1u)] = p;
+    }
+}
+
+// Blocks take pairs (p = blockIdx.x, + gridDim.x, ...) and find the cuts of
+// their tiles, one thread per tile. Merge tiles: cut = A elements before the
+// tile's first diagonal. Skew tiles: [cut, cut2) = the range of B that holds
+// the values of the tile's A chunk.
 __global__ __launch_bounds__(IB) void fm_isect_cut_kernel(const uint32_t *__restrict__ data, const uint64_t *a_off,
                                                           const uint32_t *a_len, const uint64_t *b_off,
-                                                          const uint32_t *b_len, const uint32_t *tile_begin,
-                                                          const uint32_t *tile_pair, uint32_t ntiles, uint32_t *cut,
-                                                          uint32_t *cut2) {
-    for (uint32_t t = blockIdx.x * IB + threadIdx.x; t < ntiles; t += gridDim.x * IB) {
-        const uint32_t p = tile_pair[t];
+                                                          const uint32_t *b_len, uint32_t npairs, const uint32_t *work,
+                                                          uint32_t cap) {
+    if (__hip_atomic_load(&work[3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+    const uint32_t *const tile_begin = work + ISECT_HDR + npairs;
+    uint32_t *const cut = const_cast<uint32_t *>(tile_begin) + npairs + 1;
+    uint32_t *const cut2 = cut + cap;
+    for (uint32_t p = blockIdx.x; p < npairs; p += gridDim.x) {
+        const uint32_t tb = tile_begin[p], te = tile_begin[p + 1];
+        if (tb == te) continue;
         const IsectPair q = isect_pair(data, a_off, a_len, b_off, b_len, p);
-        const uint32_t k = t - tile_begin[p];
-        if (isect_skewed(q.na, q.nb)) {
-            const uint32_t first = k * IB, last = min(first + IB, q.na) - 1;
-            const uint32_t lo = lower_bound_u32(q.B, 0, q.nb, q.A[first]);
-            cut[t] = lo;
-            cut2[t] = upper_bound_u32(q.B, lo, q.nb, q.A[last]);
-        } else {
-            cut[t] = merge_split(q.A, q.na, q.B, q.nb, k * XT);
-            cut2[t] = 0;
+        for (uint32_t t = tb + threadIdx.x; t < te; t += IB) {
+            const uint32_t k = t - tb;
+            if (isect_skewed(q.na, q.nb)) {
+                const uint32_t first = k * IB, last = min(first + IB, q.na) - 1;
+                const uint32_t lo = lower_bound_u32(q.B, 0, q.nb, q.A[first]);
+                cut[t] = lo;
+                cut2[t] = upper_bound_u32(q.B, lo, q.nb, q.A[last]);
+            } else {
+                cut[t] = merge_split(q.A, q.na, q.B, q.nb, k * XT);
+                cut2[t] = 0;
+            }
         }
     }
 }
@@ -1901,10 +2024,8 @@ __global__ __launch_bounds__(IB) void fm_isect_cut_kernel(const uint32_t *__rest
 __global__ __launch_bounds__(IB) void fm_isect_kernel(const uint32_t *__restrict__ data, const uint64_t *a_off,
                                                       const uint32_t *a_len, const uint64_t *b_off,
                                                       const uint32_t *b_len, uint32_t *counts,
-                                                      const uint64_t *out_off, uint32_t *out,
-                                                      const uint32_t *tile_begin, const uint32_t *order,
-                                                      uint32_t nwork, const uint32_t *cut, const uint32_t *cut2,
-                                                      uint32_t *ctr) {
+                                                      const uint64_t *out_off, uint32_t *out, uint32_t npairs,
+                                                      uint32_t *work, uint32_t cap) {
     __shared__ uint32_t sAB[XT + 1];  // a tile's A slice, then its B slice (+ the next B element)
     // A match takes two merge steps (its A element, then the equal B
     // element), so a stretch holds at most XM matches. They are parked at
@@ -1919,10 +2040,19 @@ __global__ __launch_bounds__(IB) void fm_isect_kernel(const uint32_t *__restrict
     constexpr uint32_t XCUT = 2 * IB;
     __shared__ uint32_t s_cut[XCUT + 1], s_cut2[XCUT];
     const uint32_t tid = threadIdx.x;
+    uint32_t *const ctr = work;
+    const uint32_t *const order = work + ISECT_HDR;
+    const uint32_t *const tile_begin = order + npairs;
+    const uint32_t *const cut = tile_begin + npairs + 1;
+    const uint32_t *const cut2 = cut + cap;
+    // loop controls in scalar registers (readfirstlane): every barrier below
+    // is taken by the whole block on a scalar branch
+    const uint32_t nwork =
+        __builtin_amdgcn_readfirstlane(__hip_atomic_load(&work[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
     while (true) {
         if (tid == 0) s_w = atomicAdd(ctr, 1u);
         __syncthreads();
-        const uint32_t w = s_w;
+        const uint32_t w = __builtin_amdgcn_readfirstlane(s_w);
         if (w >= nwork) break;
         const uint32_t p = order[w];
         const IsectPair q = isect_pair(data, a_off, a_len, b_off, b_len, p);
@@ -2079,6 +2209,8 @@ __global__ __launch_bounds__(IB) void fm_isect_kernel(const uint32_t *__restrict
         __syncthreads();  // s_cut is refilled for the next pair
     }
 }
+
+
 
 
 // ---------------------------------------------------------------------------
@@ -2287,11 +2419,33 @@ struct fm_engine {
     uint64_t epoch = 0;
     bool dirty = true;  // queues/pool need a full re-initialisation
     fm_server *server = nullptr;  // a server owns the engine while it lives
-    void *isect_buf = nullptr;    // intersection tile plan and look-back state (grow only)
+    void *isect_buf = nullptr;    // intersection work counter and pair order (grow only)
     size_t isect_cap = 0;
     FSlot *pfree_ring = nullptr;
     uint32_t pfcap = 0;
+    uint64_t prealloc = 0;     // PREALLOC: arenas + graph may take this many device bytes
+    uint64_t arena_bytes = 0;  // device bytes of the engine's arenas
 };
+
+// PREALLOC (src/environment.cuh:4): the reference bump-allocates the graph
+// and its arenas from one slab and exits with status 1 when they do not fit
+// (src/trie.cuh:1034-1049). Here the arenas are separate allocations; the
+// same budget is checked before a launch on graph g.
+static int check_prealloc(const fm_engine *e, const fm_graph *g) {
+    const uint64_t need = e->arena_bytes + (g ? g->device_bytes : 0);
+    if (e->prealloc != FM_PREALLOC_UNLIMITED && need > e->prealloc)
+        return fail(FM_ERR_GENERIC, "PREALLOC exceeded: arenas %llu bytes + graph %llu bytes > PREALLOC %llu bytes",
+                    (unsigned long long)e->arena_bytes, (unsigned long long)(g ? g->device_bytes : 0),
+                    (unsigned long long)e->prealloc);
+    return FM_OK;
+}
+
+// Launch deadline in wall-clock ticks; FM_NO_DEADLINE gives one no launch
+// reaches (2^62 ticks), so deadline arithmetic never wraps.
+static unsigned long long deadline_ticks(uint32_t timeout_ms, double wall_hz) {
+    if (timeout_ms == FM_NO_DEADLINE) return 1ull << 62;
+    return (unsigned long long)((double)timeout_ms * (wall_hz / 1000.0));
+}
 
 // A launch counter summed over its shards.
 static unsigned long long stat_sum(const Ctl &c, uint32_t i) {
@@ -2704,9 +2858,13 @@ int fm_engine_create(const fm_engine_config *cfg, fm_engine **out) {
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess)
         return cleanup_fail(fail(FM_ERR_DEVICE, "stream/event creation failed"));
-    if (c.bank_size < WAVE * MAXN / 2)
+    // A fresh bank must hold one candidate step's children: up to 2 x 64
+    // rows (two candidates per lane) of up to MAXN words, i.e. WAVE * MAXN
+    // 8-byte units; a bank that already holds rows is flushed before a step
+    // that would not fit.
+    if (c.bank_size < MIN_BANK_SIZE)
         return cleanup_fail(fail(FM_ERR_INVALID_ARG, "bank_size %u is below the minimum %u (one step of rows)",
-                                 c.bank_size, WAVE * MAXN / 2));
+                                 c.bank_size, MIN_BANK_SIZE));
     e->bank_words = (uint64_t)c.bank_size * 2;  // 8-byte units -> u32 words
     e->nbanks = (uint32_t)std::min<uint64_t>(c.soln_prealloc / (e->bank_words * 4), 1u << 30);
     if (e->nbanks == 0) return cleanup_fail(fail(FM_ERR_INVALID_ARG, "SOLN_PREALLOC smaller than one bank"));
@@ -2717,6 +2875,11 @@ int fm_engine_create(const fm_engine_config *cfg, fm_engine **out) {
     if (e->npages == 0) return cleanup_fail(fail(FM_ERR_INVALID_ARG, "OUTPUT_PREALLOC smaller than one page"));
     if (e->page_words < WAVE * MAXN)
         return cleanup_fail(fail(FM_ERR_INVALID_ARG, "output pages must hold at least %u words", WAVE * MAXN));
+    e->prealloc = c.prealloc ? c.prealloc : (uint64_t)PREALLOC;
+    e->arena_bytes = (uint64_t)e->nbanks * e->bank_words * 4 + (uint64_t)e->nbanks * 4 +
+                     (uint64_t)e->fcap * sizeof(FSlot) + (uint64_t)e->qcap * NLEVELS * sizeof(Slot) +
+                     (uint64_t)e->npages * e->page_words * 4 + (uint64_t)e->npages * 8 + sizeof(Ctl);
+    if (int rc = check_prealloc(e, nullptr)) return cleanup_fail(rc);
     auto alloc = [&](void **p, size_t bytes) { return hipMalloc(p, std::max<size_t>(bytes, 4)) == hipSuccess; };
     if (!alloc((void **)&e->banks, (size_t)e->nbanks * e->bank_words * 4) ||
         !alloc((void **)&e->bank_ref, (size_t)e->nbanks * 4) ||
@@ -2764,6 +2927,7 @@ int fm_engine_info_get(fm_engine *e, fm_engine_info *out) {
     out->output_pages = e->npages;
     out->output_page_bytes = (uint32_t)(e->page_words * 4);
     snprintf(out->device_name, sizeof out->device_name, "%s", e->name);
+    out->arena_bytes = e->arena_bytes;
     return FM_OK;
 }
 
@@ -2861,6 +3025,7 @@ int fm_match_batch(fm_engine *e, const fm_graph *g, const fm_query *const *qs, u
     if (opts) o = *opts;
     if (o.timeout_ms == 0) o.timeout_ms = 120000;
     if (o.task_rows == 0) o.task_rows = nq == 1 ? SubMaxMulSpill : FM_BATCH_TASK_ROWS;
+    if (int rc = check_prealloc(e, g)) return rc;
     const bool default_chunk = o.root_chunk == 0;
     if (o.root_chunk == 0) o.root_chunk = TooLargeThreshold;
     HIP_TRY(hipSetDevice(e->device));
@@ -2944,7 +3109,7 @@ int fm_match_batch(fm_engine *e, const fm_graph *g, const fm_query *const *qs, u
     reset.pending = (long long)chunks;
     reset.root_cursor = active;  // chunks below this are the first claims of the waves that start at once
     reset.done = chunks == 0 ? 1u : 0u;
-    reset.timeout_ticks = (unsigned long long)((double)o.timeout_ms * (e->wall_hz / 1000.0));
+    reset.timeout_ticks = deadline_ticks(o.timeout_ms, e->wall_hz);
     HIP_TRY(hipMemcpyAsync(e->ctl, &reset, CTL_RESET_BYTES, hipMemcpyHostToDevice, e->stream));
     // ---- launch ----
     KParams P{};
@@ -2979,7 +3144,7 @@ int fm_match_batch(fm_engine *e, const fm_graph *g, const fm_query *const *qs, u
     P.npages = e->npages;
     P.nlevels = std::max<uint32_t>(1, maxn - 1);
     P.starve_ticks = (unsigned long long)(e->wall_hz * 2.0);  // 2 s without a bank
-    P.timeout_ticks = (unsigned long long)((double)o.timeout_ms * (e->wall_hz / 1000.0));
+    P.timeout_ticks = reset.timeout_ticks;
     P.active_waves = active < grid ? active : 0;
     P.join_ticks = (unsigned long long)((double)FM_JOIN_MS * (e->wall_hz / 1000.0));
     fm_launch_stats st{};
@@ -3291,6 +3456,7 @@ int fm_server_create(fm_engine *e, const fm_graph *g, const fm_server_config *cf
     if (!e || !g || !out) return fail(FM_ERR_INVALID_ARG, "null argument");
     if (e->server) return fail(FM_ERR_BUSY, "the engine already has a server");
     if (g->device != e->device) return fail(FM_ERR_INVALID_ARG, "graph is not uploaded to the engine's device");
+    if (int rc = check_prealloc(e, g)) return rc;
     auto *s = new fm_server();
     s->e = e;
     s->g = g;
@@ -3375,6 +3541,7 @@ int fm_server_submit(fm_server *s, const fm_query *q, uint64_t max_rows, uint32_
     const bool idle = s->free_slots.size() == s->slot_state.size();
     if (s->free_slots.empty() || (!idle && live >= s->cfg.bank_budget)) {
         s->st.refused++;
+        if (!s->free_slots.empty()) s->st.refused_memory++;
         return fail(FM_ERR_BUSY, "no slot or bank budget available");
     }
     uint32_t slot = s->free_slots.back();
@@ -3536,45 +3703,75 @@ void fm_server_destroy(fm_server *s) {
 
 }  // extern "C"
 
+// ---------------------------------------------------------------------------
+// Intersection host side.
+
+// Words of the work array for npairs pairs and cap tiles.
+static uint64_t isect_work_words(uint32_t npairs, uint64_t cap) {
+    return ISECT_HDR + 2ull * npairs + 1 + 2 * cap;
+}
+
+// The plan, cut and main kernels on the engine's stream with `work` holding
+// isect_work_words(npairs, cap) words; allocates nothing. When the pairs
+// need more than cap tiles the kernels do no work and *over is set.
+static int isect_run(fm_engine *e, uint32_t *work, uint32_t cap, const uint32_t *data, const uint64_t *a_off,
+                     const uint32_t *a_len, const uint64_t *b_off, const uint32_t *b_len, uint32_t npairs,
+                     uint32_t *counts, const uint64_t *out_off, uint32_t *out, double *kernel_ms, bool *over) {
+    if (!data || !a_off || !a_len || !b_off || !b_len || !counts) return fail(FM_ERR_INVALID_ARG, "null argument");
+    if (out_off && !out) return fail(FM_ERR_INVALID_ARG, "out_off without out");
+    if (kernel_ms) *kernel_ms = 0;
+    if (over) *over = false;
+    if (npairs == 0) return FM_OK;
+    HIP_TRY(hipSetDevice(e->device));
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fm_isect_kernel, IB, 0) != hipSuccess || blocks <= 0)
+        blocks = 4;
+    const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(npairs, (uint64_t)blocks * e->cu_count));
+    const uint32_t cut_grid = std::min<uint32_t>(npairs, 4096);
+    HIP_TRY(hipEventRecord(e->ev0, e->stream));
+    hipLaunchKernelGGL(fm_isect_plan_kernel, dim3(1), dim3(1024), 0, e->stream, a_len, b_len, npairs, counts, work,
+                       cap);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(fm_isect_cut_kernel, dim3(cut_grid), dim3(IB), 0, e->stream, data, a_off, a_len, b_off,
+                       b_len, npairs, (const uint32_t *)work, cap);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(fm_isect_kernel, dim3(grid), dim3(IB), 0, e->stream, data, a_off, a_len, b_off, b_len, counts,
+                       out_off, out, npairs, work, cap);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(e->ev1, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+    if (kernel_ms) *kernel_ms = ms;
+    if (over) {
+        uint32_t flag = 0;
+        HIP_TRY(hipMemcpy(&flag, work + 3, 4, hipMemcpyDeviceToHost));
+        *over = flag != 0;
+    }
+    return FM_OK;
+}
+
 extern "C" {
 
 int fm_intersect_pairs(fm_engine *e, const uint32_t *data, const uint64_t *a_off, const uint32_t *a_len,
                        const uint64_t *b_off, const uint32_t *b_len, uint32_t npairs, uint32_t *counts,
                        const uint64_t *out_off, uint32_t *out, double *kernel_ms) {
-    if (!e || !data || !a_off || !a_len || !b_off || !b_len || !counts)
-        return fail(FM_ERR_INVALID_ARG, "null argument");
-    if (out_off && !out) return fail(FM_ERR_INVALID_ARG, "out_off without out");
-    if (kernel_ms) *kernel_ms = 0;
-    if (npairs == 0) return FM_OK;
+    if (!e) return fail(FM_ERR_INVALID_ARG, "null argument");
+    if (npairs == 0 || !a_len || !b_len)
+        return isect_run(e, nullptr, 0, data, a_off, a_len, b_off, b_len, npairs, counts, out_off, out, kernel_ms,
+                         nullptr);
+    // size the work array: the tiles of every pair (lengths read back)
     HIP_TRY(hipSetDevice(e->device));
-    // ---- tile plan (host): tiles of each pair, pair of each tile ----
     std::vector<uint32_t> la(npairs), lb(npairs);
     HIP_TRY(hipMemcpy(la.data(), a_len, npairs * 4ull, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(lb.data(), b_len, npairs * 4ull, hipMemcpyDeviceToHost));
-    std::vector<uint32_t> tile_begin(npairs + 1);
-    uint64_t ntiles = 0;
+    uint64_t tiles = 0;
     for (uint32_t p = 0; p < npairs; p++) {
-        tile_begin[p] = (uint32_t)ntiles;
         const uint64_t na = std::min(la[p], lb[p]), nb = std::max(la[p], lb[p]);
-        if (na == 0) continue;
-        ntiles += nb >= (uint64_t)ISKEW * na ? (na + IB - 1) / IB : (na + nb + XT - 1) / XT;
-        if (ntiles >= (1ull << 31)) return fail(FM_ERR_INVALID_ARG, "too many intersection tiles");
+        if (na) tiles += nb >= (uint64_t)ISKEW * na ? (na + IB - 1) / IB : (na + nb + XT - 1) / XT;
     }
-    tile_begin[npairs] = (uint32_t)ntiles;
-    std::vector<uint32_t> tile_pair(ntiles);
-    for (uint32_t p = 0; p < npairs; p++)
-        for (uint32_t t = tile_begin[p]; t < tile_begin[p + 1]; t++) tile_pair[t] = p;
-    // pairs with tiles, most tiles first (claimed in this order)
-    std::vector<uint32_t> order;
-    order.reserve(npairs);
-    for (uint32_t p = 0; p < npairs; p++)
-        if (tile_begin[p + 1] > tile_begin[p]) order.push_back(p);
-    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
-        return tile_begin[x + 1] - tile_begin[x] > tile_begin[y + 1] - tile_begin[y];
-    });
-    const uint32_t nwork = (uint32_t)order.size();
-    // ---- scratch: tile_begin | order | tile_pair | cut | cut2 | ctr ----
-    const size_t need = ((size_t)npairs + 1 + npairs + 3 * ntiles + 1) * 4;
+    if (tiles >= (1ull << 31)) return fail(FM_ERR_INVALID_ARG, "too many intersection tiles");
+    const size_t need = isect_work_words(npairs, tiles) * 4;
     if (need > e->isect_cap) {
         (void)hipFree(e->isect_buf);
         e->isect_buf = nullptr;
@@ -3582,39 +3779,8 @@ int fm_intersect_pairs(fm_engine *e, const uint32_t *data, const uint64_t *a_off
         HIP_TRY(hipMalloc((void **)&e->isect_buf, need));
         e->isect_cap = need;
     }
-    uint32_t *d_tile_begin = (uint32_t *)e->isect_buf;
-    uint32_t *d_order = d_tile_begin + npairs + 1;
-    uint32_t *d_tile_pair = d_order + npairs;
-    uint32_t *d_cut = d_tile_pair + ntiles;
-    uint32_t *d_cut2 = d_cut + ntiles;
-    uint32_t *d_ctr = d_cut2 + ntiles;
-    HIP_TRY(hipMemcpyAsync(d_tile_begin, tile_begin.data(), (npairs + 1) * 4ull, hipMemcpyHostToDevice, e->stream));
-    if (nwork) HIP_TRY(hipMemcpyAsync(d_order, order.data(), nwork * 4ull, hipMemcpyHostToDevice, e->stream));
-    if (ntiles)
-        HIP_TRY(hipMemcpyAsync(d_tile_pair, tile_pair.data(), ntiles * 4ull, hipMemcpyHostToDevice, e->stream));
-    HIP_TRY(hipMemsetAsync(d_ctr, 0, 4, e->stream));
-    // pairs without tiles (an empty list) keep count 0
-    HIP_TRY(hipMemsetAsync(counts, 0, npairs * 4ull, e->stream));
-    int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fm_isect_kernel, IB, 0) != hipSuccess || blocks <= 0)
-        blocks = 4;
-    const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(nwork, (uint64_t)blocks * e->cu_count));
-    const uint32_t cut_grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((ntiles + IB - 1) / IB, 65535));
-    HIP_TRY(hipEventRecord(e->ev0, e->stream));
-    if (ntiles) {
-        hipLaunchKernelGGL(fm_isect_cut_kernel, dim3(cut_grid), dim3(IB), 0, e->stream, data, a_off, a_len, b_off,
-                           b_len, d_tile_begin, d_tile_pair, (uint32_t)ntiles, d_cut, d_cut2);
-        HIP_TRY(hipGetLastError());
-        hipLaunchKernelGGL(fm_isect_kernel, dim3(grid), dim3(IB), 0, e->stream, data, a_off, a_len, b_off, b_len,
-                           counts, out_off, out, d_tile_begin, d_order, nwork, d_cut, d_cut2, d_ctr);
-        HIP_TRY(hipGetLastError());
-    }
-    HIP_TRY(hipEventRecord(e->ev1, e->stream));
-    HIP_TRY(hipStreamSynchronize(e->stream));
-    float ms = 0;
-    HIP_TRY(hipEventElapsedTime(&ms, e->ev0, e->ev1));
-    if (kernel_ms) *kernel_ms = ms;
-    return FM_OK;
+    return isect_run(e, (uint32_t *)e->isect_buf, (uint32_t)tiles, data, a_off, a_len, b_off, b_len, npairs, counts,
+                     out_off, out, kernel_ms, nullptr);
 }
 
 int fm_device_alloc(uint64_t bytes, void **out) {

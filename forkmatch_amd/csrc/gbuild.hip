@@ -1,11 +1,14 @@
 // The out-rows of a loaded .g graph built on the device, from the edge lines
 // the device parser left in device memory: what build_graph (host.cpp) does
 // on the host, with its checks and messages. The reference builds its CSR
-// on the device too (sort, scan and csrToTrie, src/trie.cuh:800-951); here
-// the steps are a radix sort of (row, destination) keys, a scan over the
-// de-duplicated keys and a scan over the row index.
+// on the device too (sort, scan and csrToTrie, src/trie.cuh:800-951, with
+// CUB's DeviceRadixSort::SortKeys and DeviceScan::ExclusiveScan); here the
+// steps are a radix sort of (row, destination) keys, a scan over the
+// de-duplicated keys and a scan over the row index, with AMD's rocPRIM
+// device primitives (rocprim::radix_sort_keys, rocprim::exclusive_scan).
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
 #include <cstdint>
@@ -155,14 +158,15 @@ struct Bufs {
     void keep(void *p) { std::replace(v.begin(), v.end(), p, (void *)nullptr); }
 };
 
-template <class In, class Out>
-int exclusive_sum(Bufs &b, In in, Out out, uint64_t n) {
+// out[i] = in[0] + ... + in[i - 1] (u32, exact: every sum here is a count
+// of edges or rows below 2^32)
+int exclusive_sum(Bufs &b, const uint32_t *in, uint32_t *out, uint64_t n) {
     size_t bytes = 0;
-    GB_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, (size_t)n));
+    GB_TRY(rocprim::exclusive_scan(nullptr, bytes, in, out, 0u, (size_t)n, rocprim::plus<uint32_t>()));
     char *tmp = nullptr;
-    int rc = b.alloc(&tmp, bytes);
+    int rc = b.alloc(&tmp, std::max<size_t>(bytes, 1));
     if (rc) return rc;
-    GB_TRY(hipcub::DeviceScan::ExclusiveSum(tmp, bytes, in, out, (size_t)n));
+    GB_TRY(rocprim::exclusive_scan(tmp, bytes, in, out, 0u, (size_t)n, rocprim::plus<uint32_t>()));
     return FM_OK;
 }
 
@@ -226,10 +230,10 @@ int gpu_build_graph(uint32_t V, uint64_t n, const uint32_t *d_edges, fm_graph *g
     GB_TRY(hipGetLastError());
     {
         size_t bytes = 0;
-        GB_TRY(hipcub::DeviceRadixSort::SortKeys(nullptr, bytes, keys, sorted, (size_t)n, 0, (int)kbits));
+        GB_TRY(rocprim::radix_sort_keys(nullptr, bytes, keys, sorted, (size_t)n, 0u, kbits));
         char *tmp = nullptr;
-        if ((rc = b.alloc(&tmp, bytes))) return rc;
-        GB_TRY(hipcub::DeviceRadixSort::SortKeys(tmp, bytes, keys, sorted, (size_t)n, 0, (int)kbits));
+        if ((rc = b.alloc(&tmp, std::max<size_t>(bytes, 1)))) return rc;
+        GB_TRY(rocprim::radix_sort_keys(tmp, bytes, keys, sorted, (size_t)n, 0u, kbits));
     }
     // ---- distinct edges: positions, destinations, row lengths ----
     uint32_t *first = nullptr, *pos = nullptr;

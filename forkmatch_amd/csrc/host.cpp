@@ -1186,10 +1186,41 @@ int gen_query_walk(const HostGraph &g, uint32_t k, int shape, uint32_t min_cycle
         return (uint32_t)(lo % V);
     };
     std::vector<std::pair<uint32_t, uint32_t>> nb;
+    if (shape == 4) {
+        if (k < 3) return fail(FM_ERR_INVALID_ARG, "a cycle needs at least 3 vertices, not %u", k);
+        // a closing edge is found with probability about degree / V per walk
+        max_attempts = std::max<uint32_t>(max_attempts, 256u * 1000u);
+    }
     for (uint32_t attempt = 0; attempt < max_attempts; attempt++) {
         std::vector<uint32_t> verts{random_endpoint()};
         std::vector<std::array<uint32_t, 3>> tree;  // (local a, local b, label)
-        if (shape == 0) {
+        if (shape == 4) {
+            // a self-avoiding walk v0 .. v(k-1), kept when an edge joins
+            // v(k-1) back to v0: a k-cycle whose edges keep their labels
+            bool stuck = false;
+            while (verts.size() < k && !stuck) {
+                neighbours(verts.back(), nb);
+                nb.erase(std::remove_if(nb.begin(), nb.end(),
+                                        [&](const std::pair<uint32_t, uint32_t> &p) {
+                                            return std::find(verts.begin(), verts.end(), p.first) != verts.end();
+                                        }),
+                         nb.end());
+                if (nb.empty()) {
+                    stuck = true;
+                    break;
+                }
+                auto pick = nb[rng.below(nb.size())];
+                tree.push_back({(uint32_t)verts.size() - 1, (uint32_t)verts.size(), pick.second});
+                verts.push_back(pick.first);
+            }
+            if (stuck) continue;
+            neighbours(verts.back(), nb);
+            std::vector<uint32_t> closing;
+            for (auto &p : nb)
+                if (p.first == verts[0]) closing.push_back(p.second);
+            if (closing.empty()) continue;
+            tree.push_back({0, k - 1, closing[rng.below(closing.size())]});
+        } else if (shape == 0) {
             uint32_t cur = 0, steps = 0;
             while (verts.size() < k && steps < 64 * k) {
                 steps++;
@@ -1257,7 +1288,7 @@ int gen_query_walk(const HostGraph &g, uint32_t k, int shape, uint32_t min_cycle
         }
         if (verts.size() < k) continue;
         std::vector<std::array<uint32_t, 3>> und = tree;
-        if (shape != 0) {
+        if (shape != 0 && shape != 4) {
             std::vector<std::array<uint32_t, 3>> chords;
             for (uint32_t i = 0; i < k; i++)
                 for (uint32_t l = 0; l < L; l++) {

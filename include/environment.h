@@ -29,7 +29,7 @@
  * (8 MiB banks). Here a wave yields its task whenever its output bank
  * fills, so the bank is also the unit of depth-first progress; the default
  * is 1 << 15 (256 KiB banks, 96k banks in SOLN_PREALLOC). The minimum is
- * 1024 (one 64-row step of 32-vertex rows). */
+ * 2048 (one step of 2 x 64 rows of 32 vertices). */
 #define CurBankSize (1u << 15)
 
 /* Threads per workgroup of the reference's main kernel

@@ -153,7 +153,9 @@ int fm_gen_rmat(uint32_t scale, uint64_t undirected_edges, double a, double b,
  * walk crosses edges of every label). shape 0: tree (the edge that first
  * reached each vertex); 1: a weighted-growth tree plus min_cycles (or one
  * more) induced chords; 2: the whole induced subgraph of the grown vertex
- * set; 3: the growth tree plus exactly min_cycles chords.
+ * set; 3: the growth tree plus exactly min_cycles chords; 4: a cycle
+ * through `vertices` vertices (a self-avoiding walk closed by an edge back
+ * to its start; the query of the 1,000-vertex configuration is a 4-cycle).
  * Edge labels are copied from the data, so the query has at least one
  * embedding. */
 int fm_gen_query_walk(const fm_graph *g, uint32_t vertices, int shape,
@@ -220,8 +222,17 @@ typedef struct fm_engine_config {
     uint32_t queue_capacity;  /* task slots per priority level (pow2) */
     uint32_t max_queries;     /* queries per launch */
     int32_t device;           /* HIP device id (default 0) */
-    uint32_t reserved[7];
+    uint32_t reserved0;
+    /* PREALLOC (src/environment.cuh:4): device bytes that the engine's
+     * arenas and the graph it matches on may take together (0 = PREALLOC of
+     * include/environment.h; FM_PREALLOC_UNLIMITED = no limit). Beyond it,
+     * fm_engine_create, fm_match* and fm_server_create return
+     * FM_ERR_GENERIC (1), the reference's exit status when its bump
+     * allocator overflows the slab (src/trie.cuh:1034-1049). */
+    uint64_t prealloc;
+    uint32_t reserved[4];
 } fm_engine_config;
+#define FM_PREALLOC_UNLIMITED (~0ull)
 int fm_engine_create(const fm_engine_config *cfg, fm_engine **out);
 void fm_engine_destroy(fm_engine *e);
 /* Device and launch-shape facts: compute units, resident waves of the
@@ -235,6 +246,9 @@ typedef struct fm_engine_info {
     uint32_t output_pages;
     uint32_t output_page_bytes;
     char device_name[64];
+    uint64_t arena_bytes;     /* device bytes of the arenas, queues and
+                                 counters (what PREALLOC counts besides the
+                                 graph) */
 } fm_engine_info;
 int fm_engine_info_get(fm_engine *e, fm_engine_info *out);
 
@@ -243,6 +257,7 @@ int fm_engine_info_get(fm_engine *e, fm_engine_info *out);
  * (src/smsm.cu:784-1139,1182-1349): one persistent kernel launch whose
  * waves pull partial-embedding tasks from device work queues. */
 #define FM_MODE_COUNT 0u  /* count only; the last level is never stored */
+#define FM_NO_DEADLINE 0xFFFFFFFFu  /* fm_match_opts.timeout_ms: no deadline */
 #define FM_MODE_TABLE 1u  /* count and store every embedding */
 
 /* Matching order. The embedding set does not depend on it; the work does.
@@ -256,7 +271,8 @@ typedef struct fm_match_opts {
     uint32_t instrument;      /* 1: gather algorithmic-byte counters */
     uint64_t max_rows;        /* table capacity per query (0 = arena) */
     uint64_t max_count;       /* abort a query past this count (0 = none) */
-    uint32_t timeout_ms;      /* launch deadline (0 = 120000) */
+    uint32_t timeout_ms;      /* launch deadline (0 = 120000; FM_NO_DEADLINE:
+                                 none, as the reference's CLI runs) */
     uint32_t grid_waves;      /* persistent waves (0: 8 per CU for one query,
                                  of which 4 start at once and 4 join after
                                  FM_JOIN_MS; every resident wave for several;
@@ -380,6 +396,8 @@ typedef struct fm_server_stats {
     uint64_t submitted, completed, refused, launches;
     uint32_t in_flight, live_banks;
     uint64_t dropped_tasks;  /* tasks of already completed queries (should be 0) */
+    uint64_t refused_memory; /* of `refused`: a slot was free but the live banks
+                                had reached bank_budget (admission by memory) */
 } fm_server_stats;
 int fm_server_create(fm_engine *e, const fm_graph *g,
                      const fm_server_config *cfg, fm_server **out);

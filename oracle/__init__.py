@@ -38,6 +38,11 @@ def lib() -> C.CDLL:
         h.oracle_match.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, u32p, u32p, u32p, u32p, C.c_int, C.c_uint64,
                                    C.c_uint64, C.c_uint64, C.c_int, C.c_uint64, u32p, C.c_int, u64p, u64p, u64p, u64p,
                                    C.POINTER(C.c_double)]
+        h.oracle_match_budget.restype = C.c_int
+        h.oracle_match_budget.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, u32p, u32p, u32p, u32p, C.c_int,
+                                          C.c_uint64, C.c_uint64, C.c_uint64, C.c_int, C.c_uint64, u32p, C.c_int,
+                                          C.c_double, u64p, u64p, u64p, u64p, C.POINTER(C.c_double),
+                                          C.POINTER(C.c_int)]
         _lib = h
     return _lib
 
@@ -135,3 +140,23 @@ def match(graph: OracleGraph, n: int, edges, threads: int = 1, table: bool = Tru
         if len(tab):
             tab = tab[np.lexsort(tab.T[::-1])]
     return OracleResult(int(cnt.value), tab, int(words.value), [int(x) for x in partials[:n]], float(secs.value))
+
+
+def timed_count(graph: OracleGraph, n: int, edges, threads: int, max_seconds: float,
+                roots: tuple[int, int] | None = None, stride: int = 1) -> tuple[int, float, bool]:
+    """A throughput sample for the timed CPU baseline: count-only
+    enumeration that stops after max_seconds. Returns (embeddings counted,
+    seconds, stopped early); a stopped run's count is partial."""
+    e = np.asarray(edges, dtype=np.uint32).reshape(-1, 3)
+    s, d, l = _u32(e[:, 0]), _u32(e[:, 1]), _u32(e[:, 2])
+    rb, re_ = (0, 2 ** 64 - 1) if roots is None else roots
+    cnt, rows, words = C.c_uint64(), C.c_uint64(), C.c_uint64()
+    partials = np.zeros(max(n, 1), np.uint64)
+    secs, stopped = C.c_double(), C.c_int()
+    buf = np.zeros((1, max(n, 1)), np.uint32)
+    rc = lib().oracle_match_budget(graph._h, n, len(e), _p(s), _p(d), _p(l), None, threads, rb, re_, stride, 0, 0,
+                                   _p(buf), 0, max_seconds, C.byref(cnt), C.byref(rows), C.byref(words),
+                                   partials.ctypes.data_as(u64p), C.byref(secs), C.byref(stopped))
+    if rc:
+        raise ValueError(f"oracle error {rc}")
+    return int(cnt.value), float(secs.value), bool(stopped.value)

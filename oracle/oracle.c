@@ -331,10 +331,17 @@ typedef struct {
     uint64_t count;
     uint64_t row_words;
     uint64_t partials[OMAXN];
+    /* time budget (timed baselines only): past `deadline` (monotonic
+     * seconds, 0 = none) the workers stop and the counts are partial */
+    double deadline;
+    int stop;
 } octx;
+
+static double now_s(void);
 
 typedef struct {
     octx *c;
+    uint64_t ticks;
     uint64_t count, row_words, partials[OMAXN];
     uint32_t *buf;
     uint64_t buf_rows, buf_cap;
@@ -380,6 +387,10 @@ static int check_self(const octx *c, uint32_t d, uint32_t x) {
 static void extend(oworker *w, uint32_t d, uint32_t *m) {
     octx *c = w->c;
     uint32_t n = c->p->n;
+    if (c->deadline > 0) {
+        if (__atomic_load_n(&c->stop, __ATOMIC_RELAXED)) return;
+        if ((++w->ticks & 4095) == 0 && now_s() > c->deadline) __atomic_store_n(&c->stop, 1, __ATOMIC_RELAXED);
+    }
     if (d == n) {
         emit(w, m);
         return;
@@ -417,7 +428,7 @@ static void *worker_main(void *arg) {
     const uint64_t chunk = 64;
     /* roots are root_begin + i * root_stride for i in [0, nroots) */
     uint64_t nroots = (c->root_end - c->root_begin + c->root_stride - 1) / c->root_stride;
-    while (1) {
+    while (!__atomic_load_n(&c->stop, __ATOMIC_RELAXED)) {
         uint64_t b = __atomic_fetch_add(&c->root_next, chunk, __ATOMIC_RELAXED);
         if (b >= nroots) break;
         uint64_t e = b + chunk < nroots ? b + chunk : nroots;
@@ -452,11 +463,33 @@ static double now_s(void) {
  * depth (n entries) and the enumeration time in seconds.
  * Returns 0, or the planner's error (1 disconnected, 2 bad query).
  */
+int oracle_match_budget(void *gp, uint32_t n, uint32_t m, const uint32_t *qs, const uint32_t *qd,
+                        const uint32_t *ql, const uint32_t *order_in, int threads, uint64_t root_begin,
+                        uint64_t root_end, uint64_t root_stride, int want_table, uint64_t max_rows,
+                        uint32_t *table, int instrument, double max_seconds, uint64_t *count, uint64_t *rows,
+                        uint64_t *row_words, uint64_t *partials, double *seconds, int *stopped);
+
 int oracle_match(void *gp, uint32_t n, uint32_t m, const uint32_t *qs, const uint32_t *qd,
                  const uint32_t *ql, const uint32_t *order_in, int threads, uint64_t root_begin,
                  uint64_t root_end, uint64_t root_stride, int want_table, uint64_t max_rows, uint32_t *table,
                  int instrument, uint64_t *count, uint64_t *rows, uint64_t *row_words, uint64_t *partials,
                  double *seconds) {
+    return oracle_match_budget(gp, n, m, qs, qd, ql, order_in, threads, root_begin, root_end, root_stride,
+                               want_table, max_rows, table, instrument, 0.0, count, rows, row_words, partials,
+                               seconds, NULL);
+}
+
+/*
+ * oracle_match with a time budget, for the timed CPU baseline: after
+ * max_seconds (0 = none) of enumeration the workers stop; *stopped is then 1
+ * and the outputs count what was enumerated until then (a throughput
+ * sample, not a result).
+ */
+int oracle_match_budget(void *gp, uint32_t n, uint32_t m, const uint32_t *qs, const uint32_t *qd,
+                        const uint32_t *ql, const uint32_t *order_in, int threads, uint64_t root_begin,
+                        uint64_t root_end, uint64_t root_stride, int want_table, uint64_t max_rows,
+                        uint32_t *table, int instrument, double max_seconds, uint64_t *count, uint64_t *rows,
+                        uint64_t *row_words, uint64_t *partials, double *seconds, int *stopped) {
     const ograph *g = (const ograph *)gp;
     octx *c = (octx *)calloc(1, sizeof(octx));
     oplan *p = (oplan *)malloc(sizeof(oplan));
@@ -504,6 +537,7 @@ int oracle_match(void *gp, uint32_t n, uint32_t m, const uint32_t *qs, const uin
     oworker *ws = (oworker *)calloc((size_t)threads, sizeof(oworker));
     pthread_t *th = (pthread_t *)calloc((size_t)threads, sizeof(pthread_t));
     double t0 = now_s();
+    c->deadline = max_seconds > 0 ? t0 + max_seconds : 0.0;
     for (int t = 0; t < threads; t++) {
         ws[t].c = c;
         if (threads == 1) worker_main(&ws[t]);
@@ -525,6 +559,7 @@ int oracle_match(void *gp, uint32_t n, uint32_t m, const uint32_t *qs, const uin
     if (row_words) *row_words = words;
     if (partials) memcpy(partials, part, sizeof(uint64_t) * n);
     if (seconds) *seconds = t1 - t0;
+    if (stopped) *stopped = c->stop;
     pthread_mutex_destroy(&c->lock);
     free(ws);
     free(th);

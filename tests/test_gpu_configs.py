@@ -1,0 +1,270 @@
+"""GPU parity at the benchmark configurations' real sizes.
+
+Each test builds one benchmark graph at full size (R-MAT 2^20, 2^22 and
+2^24 vertices) and checks the HIP engine against the CPU oracle on queries
+the oracle finishes in seconds: the same count and the same sorted table.
+Queries are screened on the device first (count mode, instrumented) so that
+only those with a bounded count and bounded oracle work are compared. The
+stress configuration's counts are far beyond any CPU enumeration, so its
+test checks size-independent properties of the stored rows instead.
+"""
+import time
+
+import numpy as np
+import pytest
+
+import forkmatch_amd as fm
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+THREADS = 16      # host threads of the oracle (the GPU box's CPU share)
+TIMEOUT_MS = 120000
+
+
+def log(msg):
+    print(msg, flush=True)
+
+
+def oracle_graph(g):
+    s, d, l = g.edge_list()
+    return orc.OracleGraph(g.vertices, s, d, l)
+
+
+def screen(engine, g, cands, budget, want, caps=None, max_partials=1 << 24):
+    """Candidates (shape, n, query) whose count is in [1, budget] with at
+    most max_partials stored partial embeddings (a bound on the oracle's
+    work), screened in count-mode batches. caps: per-shape limits. Returns
+    [(shape, n, query, count)]."""
+    keep, taken = [], {}
+    for b0 in range(0, len(cands), 64):
+        chunk = cands[b0:b0 + 64]
+        r = engine.match_batch(g, [c[2] for c in chunk], table=False, max_count=budget, max_partials=max_partials,
+                               timeout_ms=TIMEOUT_MS, allow_errors=True)
+        for j, c in enumerate(chunk):
+            o = r.query(j)
+            if o.status != 0 or not 1 <= o.count <= budget:
+                continue
+            if caps is not None and taken.get(c[0], 0) >= caps.get(c[0], 0):
+                continue
+            taken[c[0]] = taken.get(c[0], 0) + 1
+            keep.append((*c, o.count))
+            if len(keep) == want:
+                r.close()
+                return keep
+        r.close()
+    return keep
+
+
+def walk_candidates(g, seed, count, sizes, shapes):
+    """Seeded random-walk queries: shapes[i] = (shape, min_cycles, weight)."""
+    rng = np.random.default_rng(seed)
+    w = np.array([s[2] for s in shapes], float)
+    out, i = [], 0
+    while len(out) < count:
+        i += 1
+        shape, cyc, _ = shapes[rng.choice(len(shapes), p=w / w.sum())]
+        n = int(rng.integers(sizes[0], sizes[1] + 1))
+        try:
+            out.append((shape, n, fm.Query.random_walk(g, n, shape=shape, min_cycles=cyc, seed=seed * 7919 + i)))
+        except fm.ForkMatchError:
+            continue
+    return out
+
+
+def oracle_ref(og, g, q, table=True, instrument=False):
+    """The oracle in the engine's (cost-based) matching order: the same
+    embeddings as any order, and partials comparable with the engine's."""
+    return orc.match(og, q.vertices, q.edges(), threads=THREADS, table=table, instrument=instrument,
+                     order=q.plan(g).order)
+
+
+def valid_rows(csr, V, edges, rows):
+    """Per row: injective, and every query edge (s, d, l) present in the
+    data graph (a vectorised binary search in the out-row of label l)."""
+    ptr, col, labels = csr
+    lab_index = {int(v): i for i, v in enumerate(labels)}
+    rows = np.asarray(rows, np.int64)
+    srt = np.sort(rows, axis=1)
+    ok = (srt[:, 1:] != srt[:, :-1]).all(axis=1)
+    last = len(col) - 1
+    for s, d, l in edges:
+        li = lab_index.get(int(l))
+        if li is None:
+            return np.zeros(len(rows), bool)
+        r = li * V + rows[:, s]
+        lo, end = ptr[r].astype(np.int64), ptr[r + 1].astype(np.int64)
+        hi = end.copy()
+        x = rows[:, d]
+        for _ in range(34):
+            act = lo < hi
+            if not act.any():
+                break
+            mid = (lo + hi) // 2
+            go = act & (col[np.minimum(mid, last)] < x)
+            lo = np.where(go, mid + 1, lo)
+            hi = np.where(act & ~go, mid, hi)
+        ok &= (lo < end) & (col[np.minimum(lo, last)] == x)
+    return ok
+
+
+def distinct_rows(rows):
+    v = np.ascontiguousarray(rows).view(np.dtype((np.void, rows.dtype.itemsize * rows.shape[1]))).ravel()
+    return len(np.unique(v)) == len(v)
+
+
+@pytest.fixture(scope="module")
+def rmat20():
+    """R-MAT (.57, .19, .19, .05), 2^20 vertices, 2^24 undirected edges,
+    edge labels Zipf(1.2) over 0..86, 2 vertex labels: the plain and batched
+    configurations' graph."""
+    t0 = time.time()
+    g = fm.Graph.rmat(20, 1 << 24, 87, zipf_s=1.2, seed=0xF0F2, vlabels=2).upload()
+    og = oracle_graph(g)
+    log(f"rmat20: graph and oracle graph in {time.time() - t0:.1f} s")
+    return g, og
+
+
+def test_config2_plain_rmat20_equals_oracle(engine, rmat20):
+    """One launch per query (the plain form) on the full 2^20-vertex graph,
+    with 6-8 vertex walk queries as in the plain benchmark: two with >= 2
+    independent cycles, matched with tables (sorted tables equal the
+    oracle's), and two trees of at most 2^30 embeddings matched count-only
+    (counts equal the oracle's). For each, the algorithmic counters (row
+    words and partials per depth, gathered while idle waves split the work)
+    equal the oracle's too."""
+    g, og = rmat20
+    cyc = screen(engine, g, walk_candidates(g, 21, 768, (6, 8), [(1, 2, 1.0)]), 1 << 24, 2)
+    # about 1 in 100 walk trees of 6 vertices has at most 2^31 embeddings
+    trees = screen(engine, g, walk_candidates(g, 22, 1536, (6, 6), [(0, 0, 1.0)]), 1 << 31, 2,
+                   max_partials=1 << 28)
+    log(f"config 2: {[(c[0], c[1], c[3]) for c in cyc + trees]}")
+    assert len(cyc) == 2 and len(trees) == 2
+    splits = 0
+    for shape, n, q, count in cyc + trees:
+        table = shape != 0
+        t0 = time.time()
+        ref = oracle_ref(og, g, q, table=table, instrument=True)
+        log(f"config 2: oracle {n} vertices, {ref.count} embeddings in {time.time() - t0:.1f} s")
+        assert ref.count == count
+        r = engine.match(g, q, table=table, timeout_ms=TIMEOUT_MS)
+        out = r.query(0)
+        assert out.status == 0 and out.count == ref.count, (shape, n)
+        if table:
+            assert out.rows == ref.count and np.array_equal(r.table(0), ref.table), (shape, n)
+        r.close()
+        ri = engine.match(g, q, table=False, instrument=True, timeout_ms=TIMEOUT_MS)
+        oi = ri.query(0)
+        splits += ri.stats().tasks_split
+        assert oi.alg_row_bytes == 4 * ref.row_words, (shape, n)
+        assert oi.partials[:n] == ref.partials, (shape, n)
+        ri.close()
+    log(f"config 2: {splits} splits in the instrumented launches")
+    assert splits > 0
+
+
+def test_config3_batched_rmat20_equals_oracle(rmat20):
+    """One persistent launch of 128 queries of 4..12 vertices (shape caps 40%
+    trees, 40% cycle+chord, 20% near-cliques; shapes that do not fill their
+    share are topped up by the others) on the full 2^20-vertex graph: every
+    count and every sorted table equals the oracle's, with the default batch
+    knobs and with another task size, root chunk and grid."""
+    g, og = rmat20
+    nq = 128
+    caps = {0: 51, 1: 51, 2: 26}
+    # 128 tables at once: pages are claimed per wave and query, so the arena
+    # is sized for many partly filled pages
+    engine = fm.Engine(soln_prealloc=4 << 30, output_prealloc=32 << 30, output_page_bytes=1 << 16)
+    cands = walk_candidates(g, 33, 4096, (4, 12), [(0, 0, 0.4), (1, 1, 0.4), (2, 0, 0.2)])
+    chosen = screen(engine, g, cands, 1 << 20, nq, caps=caps)
+    if len(chosen) < nq:
+        taken = {id(c[2]) for c in chosen}
+        chosen += screen(engine, g, [c for c in cands if id(c[2]) not in taken], 1 << 20, nq - len(chosen))
+    mix = {k: sum(1 for c in chosen if c[0] == k) for k in (0, 1, 2)}
+    log(f"config 3: {len(chosen)} queries, shape mix {mix}")
+    assert len(chosen) == nq
+    queries = [c[2] for c in chosen]
+    t0 = time.time()
+    refs = [oracle_ref(og, g, q) for q in queries]
+    log(f"config 3: oracle in {time.time() - t0:.1f} s")
+    for knobs in ({}, {"task_rows": 1024, "root_chunk": 4096, "grid_waves": 2048}):
+        rb = engine.match_batch(g, queries, timeout_ms=TIMEOUT_MS, **knobs)
+        for i, ref in enumerate(refs):
+            out = rb.query(i)
+            assert out.status == 0 and out.count == ref.count and out.rows == ref.count, (knobs, i)
+            assert np.array_equal(rb.table(i), ref.table), (knobs, i)
+        rb.close()
+    engine.close()
+
+
+def test_config4_serving_rmat22_equals_oracle():
+    """The serving configuration's graph (R-MAT 2^22 vertices, 2^26
+    undirected edges, Zipf(1.2) labels over 0..127, 4 vertex labels) with
+    8 MiB banks (CurBankSize = 1 << 20) and a bank budget of 2: 32 queries of
+    4..16 vertices submitted together. Submissions are refused while the
+    live banks exceed the budget (admission by memory) and admitted as
+    queries complete; every served count and sorted table equals the
+    oracle's."""
+    t0 = time.time()
+    g = fm.Graph.rmat(22, 1 << 26, 128, zipf_s=1.2, seed=0xF0F4, vlabels=4).upload()
+    og = oracle_graph(g)
+    log(f"rmat22: graph and oracle graph in {time.time() - t0:.1f} s")
+    # 4,096 banks of 8 MiB: more than the 2,048 waves that each hold one
+    with fm.Engine(soln_prealloc=32 << 30, output_prealloc=4 << 30, bank_size=1 << 20) as e:
+        cands = walk_candidates(g, 44, 4096, (4, 16), [(1, 1, 0.3), (2, 0, 0.6), (0, 0, 0.1)])
+        chosen = screen(e, g, cands, 1 << 16, 32, max_partials=1 << 22)
+        log(f"config 4: {len(chosen)} queries, sizes {sorted(c[1] for c in chosen)}")
+        assert len(chosen) == 32
+        queries = [c[2] for c in chosen]
+        refs = [oracle_ref(og, g, q) for q in queries]
+        with fm.Server(e, g, slots=64, slot_pages=64, bank_budget=2, max_launch_ms=60000) as srv:
+            pending, done, nxt = {}, {}, 0
+            t_end = time.time() + 120
+            while len(done) < len(queries) and time.time() < t_end:
+                while nxt < len(queries):
+                    t = srv.submit(queries[nxt])
+                    if t is None:
+                        break
+                    pending[t] = nxt
+                    nxt += 1
+                for c in srv.poll():
+                    i = pending.pop(c.ticket)
+                    done[i] = (c.status, c.count, c.flags, srv.table(c.ticket, c.rows, c.vertices))
+                    srv.release(c.ticket)
+                time.sleep(0.0005)
+            st = srv.stats()
+        log(f"config 4: {st.submitted} admitted, {st.refused} refused ({st.refused_memory} by the bank budget), "
+            f"{st.launches} launches")
+    assert len(done) == len(queries)
+    assert st.refused_memory > 0 and st.dropped_tasks == 0
+    for i, ref in enumerate(refs):
+        status, count, flags, tab = done[i]
+        assert status == 0 and count == ref.count and not flags & fm.FM_RESULT_TRUNCATED, i
+        assert np.array_equal(tab, ref.table), i
+
+
+def test_config5_stress_rmat24_rows_are_embeddings():
+    """The stress configuration's graph (R-MAT 2^24 vertices, 2^28
+    undirected edges, labels uniform over 0..3, 8 vertex labels) with two
+    dense 10-vertex queries (a spanning tree of the walk plus 6 chords).
+    Their counts are far beyond a CPU enumeration, so each query stops
+    past a cap of 2^20 embeddings with its table full: status COUNT_LIMIT
+    with rows == cap (or status 0 with rows == count), and every stored row
+    is a distinct embedding (injective, every query edge in the graph)."""
+    t0 = time.time()
+    g = fm.Graph.rmat(24, 1 << 28, 4, zipf_s=0.0, seed=0xF0F5, vlabels=8).upload()
+    log(f"rmat24: graph in {time.time() - t0:.1f} s")
+    qs = [fm.Query.random_walk(g, 10, shape=3, min_cycles=6, seed=s) for s in (1, 2)]
+    cap = 1 << 20
+    with fm.Engine(soln_prealloc=16 << 30, output_prealloc=4 << 30, prealloc=fm.PREALLOC_UNLIMITED) as e:
+        r = e.match_batch(g, qs, max_rows=cap, max_count=cap, timeout_ms=TIMEOUT_MS, allow_errors=True)
+        csr = g.csr(0)
+        for i, q in enumerate(qs):
+            o = r.query(i)
+            log(f"config 5: query {i}: status {o.status}, count {o.count}, rows {o.rows}")
+            assert (o.status == 0 and o.rows == o.count) or (o.status == 306 and o.rows == cap), o
+            rows = r.table(i, sorted=False)
+            assert rows.shape == (o.rows, 10)
+            assert distinct_rows(rows)
+            assert valid_rows(csr, g.vertices, q.edges(), rows).all()
+        r.close()

@@ -50,11 +50,24 @@ def test_device_csr_matches_host(er):
         assert np.array_equal(hc, dc)
 
 
-@pytest.mark.parametrize("name", FIXTURE_NAMES)
+# The fixtures' label 1 closes no triangle and no 4-cycle on the 1,000-vertex
+# graph with labels 0..7, so those two run on the single-label graph below.
+@pytest.mark.parametrize("name", [n for n in FIXTURE_NAMES if n not in ("_3_triangle", "_4_square")])
 def test_reference_fixture_queries(engine, er, name):
     g, og = er
     q = fm.Query.load_g(str(fixture_path(name)))
-    _check(engine, g, og, q, name)
+    assert _check(engine, g, og, q, name) > 0, name
+
+
+def test_config1_walk_cycles(engine, er):
+    """The 1,000-vertex configuration's query: a 4-cycle taken by seeded
+    random walk on the data graph, keeping its edge labels (at least the
+    walk itself matches), against the oracle's count and table."""
+    g, og = er
+    for seed in range(8):
+        q = fm.Query.random_walk(g, 4, shape=4, seed=seed)
+        assert len(q.edges()) == 8
+        assert _check(engine, g, og, q, f"walk 4-cycle seed={seed}") > 0
 
 
 @pytest.mark.parametrize("name", FIXTURE_NAMES)
@@ -231,8 +244,9 @@ def test_count_limit_stops_query(engine, er):
 
 def test_small_bank_pool_recycles_banks(er):
     g, og = er
-    # 64 banks of 8 KiB: the frontier must be recycled through the free list
-    with fm.Engine(soln_prealloc=64 * 8192, output_prealloc=1 << 26, bank_size=1024,
+    # 64 banks of 16 KiB (the smallest bank): the frontier must be recycled
+    # through the free list
+    with fm.Engine(soln_prealloc=64 * 16384, output_prealloc=1 << 26, bank_size=2048,
                    output_page_bytes=1 << 16) as e:
         for name in ["_4_road4", "_5_road5", "_4_square"]:
             q = fm.Query.load_g(str(fixture_path(name)))
@@ -244,14 +258,105 @@ def test_small_bank_pool_recycles_banks(er):
             assert st.banks_reused > 0 or st.banks_allocated <= 64
 
 
+def test_large_queries_at_the_smallest_bank():
+    """Walk queries of 12 to 32 vertices (the plan limit) with the smallest
+    bank (2,048 8-byte units: one step of 2 x 64 children of 32 words), on
+    a graph with 32 edge labels where long labelled walks have few matches:
+    one launch each and all in one batch launch give the oracle's count and
+    sorted table."""
+    g = fm.Graph.erdos_renyi(2000, 8000, 32, seed=0xF0F6).upload()
+    og = _oracle_graph(g)
+    with fm.Engine(soln_prealloc=1 << 28, output_prealloc=1 << 28, bank_size=2048, output_page_bytes=1 << 16) as e:
+        chosen = []
+        for n, shape, cyc in [(12, 1, 2), (16, 1, 2), (18, 3, 3), (20, 3, 2), (24, 3, 4), (32, 3, 5), (32, 0, 0)]:
+            for seed in range(20):
+                q = fm.Query.random_walk(g, n, shape=shape, min_cycles=cyc, seed=1000 * n + seed)
+                o = e.match(g, q, table=False, max_count=1 << 18, max_partials=1 << 22, timeout_ms=TIMEOUT_MS,
+                            allow_errors=True).query(0)
+                if o.status == 0:
+                    chosen.append(q)
+                    break
+        assert [q.vertices for q in chosen] == [12, 16, 18, 20, 24, 32, 32]
+        refs = [orc.match(og, q.vertices, q.edges(), threads=4, order=q.plan(g).order) for q in chosen]
+        for q, ref in zip(chosen, refs):
+            r = e.match(g, q, timeout_ms=TIMEOUT_MS)
+            assert r.query(0).status == 0 and r.count(0) == ref.count > 0, q.vertices
+            assert np.array_equal(r.table(0), ref.table), q.vertices
+            r.close()
+        rb = e.match_batch(g, chosen, timeout_ms=TIMEOUT_MS)
+        for i, ref in enumerate(refs):
+            assert rb.query(i).status == 0 and rb.count(i) == ref.count, i
+            assert np.array_equal(rb.table(i), ref.table), i
+        rb.close()
+
+
+def test_wide_step_fits_the_smallest_bank():
+    """A step of 2 x 64 children of 19 words (2,432 words) into a fresh bank
+    of the smallest size (4,096 words): a labelled 17-edge path whose end is
+    a hub of 1,000 leaves, each leaf with one tail edge. The 20-vertex query
+    (the path, a leaf, its tail) reaches the hub at depth 18, where one
+    parent has 1,000 candidates. Count and table equal the oracle's (1,000
+    embeddings); with banks of half this size the step would not fit."""
+    und = [(i, i + 1, i + 1) for i in range(17)]
+    leaves = range(18, 1018)
+    und += [(17, v, 50) for v in leaves] + [(v, v + 1000, 60) for v in leaves]
+    s = np.array([a for a, b, _ in und] + [b for a, b, _ in und], np.uint32)
+    d = np.array([b for a, b, _ in und] + [a for a, b, _ in und], np.uint32)
+    lab = np.array([x for _, _, x in und] * 2, np.uint32)
+    V = 2018
+    g = fm.Graph.from_edges(V, s, d, lab).upload()
+    og = orc.OracleGraph(V, s, d, lab)
+    qe = [(i, i + 1, i + 1) for i in range(17)] + [(17, 18, 50), (18, 19, 60)]
+    q = fm.Query.from_edges(20, [(a, b, x) for a, b, x in qe] + [(b, a, x) for a, b, x in qe])
+    ref = orc.match(og, 20, q.edges())
+    assert ref.count == 1000
+    with fm.Engine(soln_prealloc=1 << 26, output_prealloc=1 << 26, bank_size=2048, output_page_bytes=1 << 16) as e:
+        for plan in (fm.PLAN_COST, fm.PLAN_REFERENCE):
+            r = e.match(g, q, timeout_ms=TIMEOUT_MS, plan=plan)
+            assert r.query(0).status == 0 and r.count(0) == 1000, plan
+            assert np.array_equal(r.table(0), ref.table), plan
+            r.close()
+
+
+def test_bank_size_below_one_step_is_refused():
+    with pytest.raises(fm.ForkMatchError) as err:
+        fm.Engine(soln_prealloc=1 << 24, output_prealloc=1 << 24, bank_size=2047)
+    assert err.value.code == 300
+
+
+def test_prealloc_budget_is_enforced(er):
+    """PREALLOC bounds the arenas plus the graph: beyond it engine creation,
+    a launch and a server fail with status 1 (the reference's exit status
+    when its slab overflows); at exactly the budget they run."""
+    g, _ = er
+    kw = dict(soln_prealloc=1 << 24, output_prealloc=1 << 24, queue_capacity=1 << 12)
+    with fm.Engine(prealloc=fm.PREALLOC_UNLIMITED, **kw) as e:
+        arenas = e.info().arena_bytes
+    need = arenas + g.info().device_bytes
+    with pytest.raises(fm.ForkMatchError) as err:
+        fm.Engine(prealloc=arenas - 1, **kw)
+    assert err.value.code == 1 and "PREALLOC" in err.value.message
+    q = fm.Query.load_g(str(fixture_path("_2_road2")))
+    with fm.Engine(prealloc=need - 1, **kw) as e:
+        with pytest.raises(fm.ForkMatchError) as err:
+            e.match(g, q)
+        assert err.value.code == 1 and "PREALLOC" in err.value.message
+        with pytest.raises(fm.ForkMatchError) as err:
+            fm.Server(e, g)
+        assert err.value.code == 1
+    with fm.Engine(prealloc=need, **kw) as e:
+        assert e.match(g, q).count(0) > 0
+
+
 def test_exhausted_bank_pool_fails_cleanly():
-    # one 8 KiB bank: the depth-1 rows of a 5-path on 8000 same-label edges
-    # fill it, and expanding them needs a second bank that never comes
+    # one 16 KiB bank (the smallest): the depth-1 rows of a 5-path on 8000
+    # same-label edges fill it, and expanding them needs a second bank that
+    # never comes
     g0 = fm.Graph.erdos_renyi(1000, 4000, 1, seed=9)
     s, d, _ = g0.edge_list()
     g = fm.Graph.from_edges(1000, s, d, np.ones_like(s)).upload()
     q = fm.Query.load_g(str(fixture_path("_5_road5")))
-    with fm.Engine(soln_prealloc=8192, output_prealloc=1 << 24, bank_size=1024, output_page_bytes=1 << 16) as e:
+    with fm.Engine(soln_prealloc=16384, output_prealloc=1 << 24, bank_size=2048, output_page_bytes=1 << 16) as e:
         r = e.match(g, q, table=False, timeout_ms=TIMEOUT_MS, grid_waves=4, allow_errors=True)
         assert r.query(0).status == 303  # FM_ERR_POOL_EXHAUSTED
         # the engine recovers for the next launch

@@ -136,3 +136,51 @@ def test_tile_plan_counts():
     # 400 >= 32 * 10: one skew tile; 10000 merge-path elements: 3 tiles; empty: none
     assert begin == [0, 1, 4, 4]
     assert tile_pair == [0, 1, 1, 1]
+
+
+def device_plan(lens, cap):
+    """fm_isect_plan_kernel: tile_begin by a scan of the tile counts (each
+    entry at most cap), the pairs with tiles ordered by the bit length of
+    their tile count, largest first, and the overflow flag (no work) when
+    the tiles exceed cap."""
+    counts = []
+    for la, lb in lens:
+        na, nb = sorted((la, lb))
+        counts.append(0 if na == 0 else (-(-na // IB) if nb >= ISKEW * na else -(-(na + nb) // XT)))
+    begin, acc = [], 0
+    for c in counts:
+        begin.append(min(acc, cap))
+        acc += c
+    begin.append(min(acc, cap))
+    over = acc > cap
+    buckets = {}
+    for p, c in enumerate(counts):
+        if c:
+            buckets.setdefault(32 - int(c).bit_length(), []).append(p)
+    order = [p for b in sorted(buckets) for p in buckets[b]]
+    return begin, order, (0 if over else len(order)), over
+
+
+def test_device_plan_equals_the_host_plan():
+    rng = np.random.default_rng(9)
+    pairs = []
+    for _ in range(300):
+        na, nb = (int(x) for x in rng.integers(0, 20000, 2))
+        if rng.random() < 0.2:
+            nb = na * 40
+        pairs.append((np.zeros(na), np.zeros(nb)))
+    lens = [(len(a), len(b)) for a, b in pairs]
+    host_begin, host_tile_pair = plan(pairs)
+    begin, order, nwork, over = device_plan(lens, len(host_tile_pair))
+    assert begin == host_begin and not over and nwork == len(order)
+    # every pair with tiles once, largest tile count (by bit length) first
+    assert sorted(order) == sorted(set(host_tile_pair))
+    bits = [int(begin[p + 1] - begin[p]).bit_length() for p in order]
+    assert bits == sorted(bits, reverse=True)
+
+
+def test_device_plan_flags_too_many_tiles_and_stays_in_the_array():
+    lens = [(5000, 6000)] * 10
+    begin, order, nwork, over = device_plan(lens, 20)
+    assert over and nwork == 0
+    assert all(b <= 20 for b in begin)  # cut[t] / cut2[t] are only read for t < cap

@@ -267,3 +267,20 @@ def test_invalid_matching_order_is_rejected():
         orc.plan(4, e, order=[0, 2, 1, 3])  # 2 does not touch 0
     with pytest.raises(ValueError):
         orc.plan(4, e, order=[0, 1, 1, 3])
+
+
+def test_timed_count_is_the_count_when_it_finishes_and_stops_when_it_does_not():
+    """The CPU baseline's time-budgeted enumeration: with room to finish it
+    counts exactly what the oracle counts; with a budget far below the work
+    it stops early with a partial count."""
+    V = 300
+    s, d, l = single_label_graph(V, 3000, 5)
+    og = orc.OracleGraph(V, s, d, l)
+    path4 = np.array([(0, 1, 1), (1, 0, 1), (1, 2, 1), (2, 1, 1), (2, 3, 1), (3, 2, 1)], np.uint32)
+    full = orc.match(og, 4, path4, table=False).count
+    cnt, secs, stopped = orc.timed_count(og, 4, path4, 2, 60.0)
+    assert (cnt, stopped) == (full, False) and secs >= 0
+    star = np.array([(0, k, 1) for k in range(1, 7)] + [(k, 0, 1) for k in range(1, 7)], np.uint32)
+    cnt, secs, stopped = orc.timed_count(og, 7, star, 1, 0.05)
+    # sum over v of d(d-1)...(d-5): far more than 0.05 s of enumeration
+    assert stopped and cnt > 0 and secs < 5

@@ -3,7 +3,7 @@ batches of 64 parents, 64-candidate steps over the flattened driver rows,
 work splitting (parent halves and candidate halves) and yields that requeue
 the remainder, with `skip` / `last_end` bounding a task's first and last
 parent. Fuzzed with random row lengths and random split / yield decisions,
-the invariant is that every (parent, candidate) pair is processed exactly
+the invariants are that every (parent, candidate) pair is processed exactly
 once and that no u32 arithmetic wraps. This is where a split that left a
 task ending inside its last batch used to underflow a parent count."""
 import random
@@ -19,27 +19,43 @@ def u32(x):
     return x
 
 
-def process(task, lens, rng, queue, seen, p_split, p_yield):
+def process(task, lens, rng, queue, seen, p_split, p_yield, counted=None):
     first, count, skip, last_end = task
     cnt_local, last = count, last_end
     b0 = 0
     while b0 < cnt_local:
         nb = min(WAVE, u32(cnt_local - b0))
-        dstart, dlen = [], []
+        dstart, dlen, acct = [], [], []
         for lane in range(nb):
             p = first + b0 + lane
             endv = lens[p]
             if b0 + lane == cnt_local - 1:
                 endv = min(endv, last)
-            startv = min(skip, endv) if (b0 == 0 and lane == 0) else 0
+            first_par = b0 == 0 and lane == 0
+            startv = min(skip, endv) if first_par else 0
             dstart.append(startv)
             dlen.append(u32(endv - startv))
+            # the instrumented launch's flags: bit 0 the range starts at
+            # candidate 0, bit 1 the driver row is empty
+            acct.append((0 if (first_par and skip) else 1) | (2 if lens[p] == 0 else 0))
         cum = []
         acc = 0
         for x in dlen:
             acc += x
             cum.append(acc)
         T = acc
+
+        def account(t_fin):
+            # engine.hip process_task `account`: a parent is counted by the
+            # task whose processed candidates [0, t_fin) hold its candidate 0
+            if counted is None:
+                return
+            for lane in range(nb):
+                hi, lo = cum[lane], (cum[lane - 1] if lane else 0)
+                fl = acct[lane]
+                if (fl & 1) and ((hi <= t_fin) if (fl & 2) else (lo < hi and lo < t_fin)):
+                    p = first + b0 + lane
+                    counted[p] = counted.get(p, 0) + 1
 
         def parent_of(c):
             lo, hi = 0, nb - 1
@@ -72,6 +88,7 @@ def process(task, lens, rng, queue, seen, p_split, p_yield):
                 excl = cum[lo - 1] if lo else 0
                 queue.append((first + b0 + lo, u32(cnt_local - b0 - lo),
                               u32(base - excl) + (skip if (b0 == 0 and lo == 0) else 0), last))
+                account(base)
                 return
             for lane in range(WAVE):
                 tt = base + lane
@@ -83,6 +100,7 @@ def process(task, lens, rng, queue, seen, p_split, p_yield):
                     seen.add(key)
             base += WAVE
             step += 1
+        account(T)
         b0 += WAVE
 
 
@@ -93,6 +111,7 @@ def test_every_candidate_exactly_once(seed):
     lens = [rng.choice([0, 1, 3, 10, 64, 65, 200, 700, 5000]) for _ in range(nparents)]
     queue = [(0, nparents, 0, ALL)]
     seen = set()
+    counted = {}
     p_split, p_yield = rng.choice([(0.0, 0.0), (0.5, 0.0), (0.0, 0.2), (0.7, 0.1), (1.0, 0.05)])
     guard = 0
     while queue:
@@ -100,6 +119,75 @@ def test_every_candidate_exactly_once(seed):
         assert guard < 1_000_000
         t = queue.pop(rng.randrange(len(queue)))
         assert t[1] > 0
-        process(t, lens, rng, queue, seen, p_split, p_yield)
+        process(t, lens, rng, queue, seen, p_split, p_yield, counted)
     want = {(p, c) for p in range(nparents) for c in range(lens[p])}
     assert seen == want
+    # the algorithmic counters: every parent's row words and lookups are
+    # counted exactly once however the work was split and requeued
+    assert counted == {p: 1 for p in range(nparents)}
+
+
+def claim_roots(grid, active, total_chunks, rng, first_before_queue=True):
+    """Model of the launch start (engine.hip match_waves / claim_root): the
+    waves below `active` start at once and own chunk (workgroup id) as their
+    first claim, without an atomic; root_cursor starts at `active`, so later
+    claims (every wave, dormant ones included) take the chunks past the
+    first round. Waves run in a random interleaving and a wave may find a
+    queued task before its first claim. Returns the claimed chunk ids."""
+    cursor = active
+    claimed = []
+    first = [w < active for w in range(grid)]
+    queued = 0
+    live = list(range(grid))
+    while live:
+        w = rng.choice(live)
+        took = False
+        if first_before_queue and first[w]:
+            first[w] = False
+            if w < total_chunks:
+                claimed.append(w)
+                took = True
+        if not took and queued > 0 and rng.random() < 0.5:
+            queued -= 1  # a task pushed by an earlier wave
+            took = True
+        if not took and not first_before_queue and first[w]:
+            first[w] = False
+            if w < total_chunks:
+                claimed.append(w)
+                took = True
+        # a claim is only attempted if no task was popped (the old order
+        # cleared the first-claim flag before that)
+        if not first_before_queue:
+            first[w] = False
+        if not took:
+            if cursor < total_chunks:
+                claimed.append(cursor)
+                cursor += 1
+                took = True
+        if took:
+            queued += rng.randrange(0, 3)  # the chunk's children
+        else:
+            live.remove(w)
+    return claimed
+
+
+@pytest.mark.parametrize("seed", range(30))
+def test_every_root_chunk_claimed_exactly_once(seed):
+    rng = random.Random(seed)
+    grid = rng.choice([1, 4, 16, 64])
+    active = rng.choice([grid, max(1, grid // 2)])
+    total = rng.choice([0, 1, active - 1, active, active + 1, 5 * grid])
+    total = max(0, total)
+    claimed = claim_roots(grid, active, total, rng)
+    assert sorted(claimed) == list(range(total))
+
+
+def test_popping_before_the_first_claim_loses_a_chunk():
+    """The order the claim must not have: a starting wave that pops a queued
+    task first and then claims from the cursor never takes its own chunk."""
+    lost = 0
+    for seed in range(50):
+        rng = random.Random(seed)
+        claimed = claim_roots(16, 16, 64, rng, first_before_queue=False)
+        lost += len(set(range(64)) - set(claimed))
+    assert lost > 0
