@@ -536,12 +536,13 @@ class DeviceBuffer:
 
 
 def intersect_pairs(engine: Engine, data, a_off, a_len, b_off, b_len, out_off=None, out_words: int = 0,
-                    buffers: dict | None = None) -> dict:
+                    buffers: dict | None = None, noalloc: bool = False) -> dict:
     """|A_i n B_i| (and the intersections when out_off is given) on the device.
 
     Lists are slices of `data` (u32, each slice sorted ascending). Returns
     {"counts", "out", "kernel_ms"}. `buffers` may hold device copies from an
-    earlier call to skip the uploads (keys data/a_off/a_len/b_off/b_len)."""
+    earlier call to skip the uploads (keys data/a_off/a_len/b_off/b_len).
+    noalloc: run fm_intersect_pairs_noalloc (the library's fixed scratch)."""
     npairs = len(a_len)
     bufs = buffers if buffers is not None else {}
     if "data" not in bufs:
@@ -554,7 +555,7 @@ def intersect_pairs(engine: Engine, data, a_off, a_len, b_off, b_len, out_off=No
     oo = DeviceBuffer.from_array(np.asarray(out_off, np.uint64)) if out_off is not None else None
     out = DeviceBuffer(4 * max(out_words, 1)) if out_off is not None else None
     ms = C.c_double()
-    fn = L.lib().fm_intersect_pairs
+    fn = L.lib().fm_intersect_pairs_noalloc if noalloc else L.lib().fm_intersect_pairs
     _check(fn(engine._h, bufs["data"].ptr, bufs["a_off"].ptr, bufs["a_len"].ptr, bufs["b_off"].ptr, bufs["b_len"].ptr,
               npairs, counts.ptr, oo.ptr if oo else None, out.ptr if out else None, C.byref(ms)))
     res = {"counts": counts.to_array(np.uint32, npairs), "kernel_ms": ms.value, "buffers": bufs}

@@ -120,6 +120,7 @@ SIGNATURES = {
     "fm_server_stats_get": (C.c_int, [vp, P(ServerStats)]),
     "fm_server_destroy": (None, [vp]),
     "fm_intersect_pairs": (C.c_int, [vp, vp, vp, vp, vp, vp, u32, vp, vp, vp, P(f64)]),
+    "fm_intersect_pairs_noalloc": (C.c_int, [vp, vp, vp, vp, vp, vp, u32, vp, vp, vp, P(f64)]),
     "fm_device_alloc": (C.c_int, [u64, P(vp)]),
     "fm_device_free": (C.c_int, [vp]),
     "fm_copy_to_device": (C.c_int, [vp, vp, u64]),

@@ -29,6 +29,7 @@
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <thread>
 #include <vector>
@@ -2447,6 +2448,8 @@ static unsigned long long deadline_ticks(uint32_t timeout_ms, double wall_hz) {
     return (unsigned long long)((double)timeout_ms * (wall_hz / 1000.0));
 }
 
+static int isect_scratch_init(int device);
+
 // A launch counter summed over its shards.
 static unsigned long long stat_sum(const Ctl &c, uint32_t i) {
     unsigned long long v = 0;
@@ -2830,6 +2833,7 @@ int fm_engine_create(const fm_engine_config *cfg, fm_engine **out) {
         return rc;
     };
     if (hipSetDevice(e->device) != hipSuccess) return cleanup_fail(fail(FM_ERR_DEVICE, "hipSetDevice failed"));
+    if (int rc = isect_scratch_init(e->device)) return cleanup_fail(rc);
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, e->device) != hipSuccess)
         return cleanup_fail(fail(FM_ERR_DEVICE, "hipGetDeviceProperties failed"));
@@ -3706,6 +3710,37 @@ void fm_server_destroy(fm_server *s) {
 // ---------------------------------------------------------------------------
 // Intersection host side.
 
+// The no-allocation entry point's work array (claim counter, pair order,
+// tile starts, cuts): a fixed device array, each GPU holding its own copy
+// until the process ends. Host code uses the address hipGetSymbolAddress
+// returns for the device, taken once per device at engine creation, never
+// the array's name.
+constexpr uint32_t NOALLOC_WORDS = FM_NOALLOC_SCRATCH_BYTES / 4;
+static_assert(ISECT_HDR + 2ull * FM_NOALLOC_MAX_PAIRS + 1 < NOALLOC_WORDS, "the pair arrays fit the scratch");
+static __device__ __align__(256) uint32_t scratch[NOALLOC_WORDS];
+constexpr int FM_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static uint32_t *g_scratch_ptr[FM_MAX_DEVICES];
+
+// Called by fm_engine_create with the engine's device current.
+static int isect_scratch_init(int device) {
+    if (device < 0 || device >= FM_MAX_DEVICES) return fail(FM_ERR_INVALID_ARG, "device %d out of range", device);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return FM_OK;
+    void *ptr = nullptr;
+    const hipError_t rc = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (rc != hipSuccess || !ptr)
+        return fail(FM_ERR_DEVICE, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(rc));
+    g_scratch_ptr[device] = (uint32_t *)ptr;
+    return FM_OK;
+}
+
+static uint32_t *isect_scratch_of(int device) {
+    if (device < 0 || device >= FM_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch_ptr[device];
+}
+
 // Words of the work array for npairs pairs and cap tiles.
 static uint64_t isect_work_words(uint32_t npairs, uint64_t cap) {
     return ISECT_HDR + 2ull * npairs + 1 + 2 * cap;
@@ -3781,6 +3816,25 @@ int fm_intersect_pairs(fm_engine *e, const uint32_t *data, const uint64_t *a_off
     }
     return isect_run(e, (uint32_t *)e->isect_buf, (uint32_t)tiles, data, a_off, a_len, b_off, b_len, npairs, counts,
                      out_off, out, kernel_ms, nullptr);
+}
+
+int fm_intersect_pairs_noalloc(fm_engine *e, const uint32_t *data, const uint64_t *a_off, const uint32_t *a_len,
+                               const uint64_t *b_off, const uint32_t *b_len, uint32_t npairs, uint32_t *counts,
+                               const uint64_t *out_off, uint32_t *out, double *kernel_ms) {
+    if (!e) return fail(FM_ERR_INVALID_ARG, "null argument");
+    if (npairs > FM_NOALLOC_MAX_PAIRS)
+        return fail(FM_ERR_INVALID_ARG, "%u pairs exceed the no-allocation limit of %u pairs", npairs,
+                    (uint32_t)FM_NOALLOC_MAX_PAIRS);
+    uint32_t *work = isect_scratch_of(e->device);
+    if (!work) return fail(FM_ERR_DEVICE, "no scratch address for device %d", e->device);
+    const uint32_t cap = (uint32_t)((NOALLOC_WORDS - isect_work_words(npairs, 0)) / 2);
+    bool over = false;
+    int rc = isect_run(e, work, cap, data, a_off, a_len, b_off, b_len, npairs, counts, out_off, out, kernel_ms, &over);
+    if (rc) return rc;
+    if (over)
+        return fail(FM_ERR_INVALID_ARG, "the pairs need more than the %u tiles of the no-allocation scratch (%u bytes)",
+                    cap, (uint32_t)FM_NOALLOC_SCRATCH_BYTES);
+    return FM_OK;
 }
 
 int fm_device_alloc(uint64_t bytes, void **out) {

@@ -432,6 +432,28 @@ int fm_intersect_pairs(fm_engine *e, const uint32_t *data,
                        const uint64_t *out_off, uint32_t *out,
                        double *kernel_ms);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc. fm_intersect_pairs with its work array (claim counter,
+ * pair order, tile starts and cuts) in a fixed device array of the library
+ * (FM_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends) instead of
+ * engine memory; it calls no hipMalloc, hipFree or other allocating routine
+ * (the engine, created beforehand, provides the device and stream). Limits:
+ * FM_NOALLOC_MAX_PAIRS pairs, and the pairs' tiles (3,840 merge-path
+ * elements each, or 256 elements of the shorter list of a pair whose longer
+ * list is 32x longer) must fit the array: about 458k tiles at 65,536 pairs.
+ * Beyond either limit it returns FM_ERR_INVALID_ARG (counts are then zero).
+ * Every call on one GPU uses that GPU's one array: calls that overlap on one
+ * GPU (from several engines or threads) race. */
+#define FM_NOALLOC_MAX_PAIRS (1u << 16)
+#define FM_NOALLOC_SCRATCH_BYTES (4u << 20)
+int fm_intersect_pairs_noalloc(fm_engine *e, const uint32_t *data,
+                               const uint64_t *a_off, const uint32_t *a_len,
+                               const uint64_t *b_off, const uint32_t *b_len,
+                               uint32_t npairs, uint32_t *counts,
+                               const uint64_t *out_off, uint32_t *out,
+                               double *kernel_ms);
+
 /* Device buffers for the microkernel and its tests (hipMalloc/hipMemcpy). */
 int fm_device_alloc(uint64_t bytes, void **out);
 int fm_device_free(void *p);
