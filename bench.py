@@ -5,11 +5,11 @@ beside it. An R-MAT data graph (a, b, c, d = .57, .19, .19, .05), 2^20
 vertices, 2^24 undirected edges written as 2^25 directed edges, edge labels
 Zipf(1.2) over 0..86, resident in HBM.
  - Plain (the headline): 32 seeded random-walk queries of 6-8 vertices, each
-   matched in its own launch. 16 are trees, enumerated in count mode (the
-   reference's WriteSolnFlag = false, src/environment.cuh:65) and screened to
-   at most 2^31 embeddings; 16 have >= 2 independent cycles and are matched
-   with their full embedding table written to HBM (at most 2^26 embeddings
-   each). One step = all 32 launches. value = embeddings/s over the timed
+   matched in its own launch. 16 are tree-like (a walk tree or a tree plus
+   one chord), enumerated in count mode (the reference's WriteSolnFlag =
+   false, src/environment.cuh:65) and screened to at most 2^31 embeddings;
+   16 have >= 2 independent cycles and are matched with their full embedding
+   table written to HBM (at most 2^26 embeddings each). One step = all 32 launches. value = embeddings/s over the timed
    steps (whole job, inputs resident before the timed region).
  - Batched (the `batched` object): 1,024 queries of 4..12 vertices (shape
    caps 40% trees, 40% cycle+chord, 20% near-cliques, each with a table of at
@@ -85,11 +85,13 @@ def make_candidates(g, cfg: int, seed: int, count: int):
             out.append((3, 10, q))
             continue
         if cfg == 2:
-            # half trees (matched count-only), half with >= 2 independent
-            # cycles (matched with tables)
-            shape = 0 if len(out) % 2 == 0 else 1
+            # half tree-like (at most one cycle: a walk tree, or the growth
+            # tree plus one chord; matched count-only), half with >= 2
+            # independent cycles (matched with tables)
+            kind = len(out) % 4
+            shape = (0, 1, 3, 1)[kind]
             n = int(rng.integers(6, 9))
-            cyc = 2 if shape == 1 else 0
+            cyc = 2 if shape == 1 else (1 if shape == 3 else 0)
         else:
             u = rng.random()
             shape = 0 if u < 0.4 else (1 if u < 0.8 else 2)
@@ -452,19 +454,21 @@ BENCH_PREALLOC = 270 * 10 ** 9
 
 
 def select_plain(engine, g, seed: int, nq: int, budget: int, tree_budget: int):
-    """Half trees (count mode, at most tree_budget embeddings), half with
-    >= 2 independent cycles (tables, at most budget), from seeded rounds of
-    candidates. Returns (chosen, counts, tables)."""
+    """Half tree-like queries (at most one cycle; count mode, at most
+    tree_budget embeddings), half with >= 2 independent cycles (tables, at
+    most budget), from seeded rounds of candidates. Returns (chosen, counts,
+    tables)."""
     half = nq - nq // 2
     trees, tcounts, cyc, ccounts = [], [], [], []
-    # about 1 walk tree of 6-8 vertices in 100 has at most 2^31 embeddings on
-    # the 2^20-vertex graph, 1 cyclic query in 30 at most 2^26
-    for rnd in range(48):
+    # on the 2^20-vertex graph about 1 walk tree of 6-8 vertices in 300 has
+    # at most 2^31 embeddings (most unicyclic walks do), and 1 query with
+    # >= 2 cycles in 30 at most 2^26
+    for rnd in range(24):
         if len(trees) >= half and len(cyc) >= nq - half:
             break
         cands = make_candidates(g, 2, seed + 7919 * rnd, 4 * nq)
         if len(trees) < half:
-            k, c = screen(engine, g, [x for x in cands if x[0] == 0], tree_budget, half - len(trees),
+            k, c = screen(engine, g, [x for x in cands if x[0] in (0, 3)], tree_budget, half - len(trees),
                           max_partials=1 << 28)
             trees += k
             tcounts += c
@@ -527,22 +531,41 @@ def run_plain(engine, g, queries, tables, counts, steps: int, warmup: int, valid
             "parity_checks_passed": bool(ok)}
 
 
-def run_batched(g, seed: int, nq: int, budget: int, steps: int, warmup: int):
+def run_batched(g, seed: int, nq: int, budget: int, steps: int, warmup: int, save: str = "", load: str = ""):
     """The batched form: nq queries of 4..12 vertices in one persistent
     launch (knobs from the autotuner) against the same queries one launch
-    each. Returns the leg's results and its queries."""
-    from forkmatch_amd.autotune import autotune
+    each. Returns the leg's results and its queries. save/load: the query
+    set, its counts, algorithmic bytes and tuned knobs (a loaded set runs
+    no screening, instrumented or tuning launches)."""
+    from forkmatch_amd.autotune import TuneResult, autotune
     # 1,024 tables resident at once: a larger table arena
     engine = fm.Engine(output_prealloc=96 << 30, prealloc=BENCH_PREALLOC)
     t1 = time.time()
-    chosen, counts = select_mixed(engine, g, 3, seed, nq, budget)
-    queries = [c[2] for c in chosen]
-    log(f"batched leg: {len(queries)} queries selected in {time.time() - t1:.1f} s")
-    ri = engine.match_batch(g, queries, table=False, instrument=True, timeout_ms=300000)
-    balg = [b_alg(ri.query(i), chosen[i][1], table=True) for i in range(len(queries))]
-    ri.close()
-    space = {"root_chunk": [1024, 4096], "task_rows": [1024, 4096], "grid_frac": [1.0, 1.5]}
-    tune = autotune(engine, g, queries, space=space, table=False, log=log)
+    if load:
+        man = json.loads((Path(load) / "manifest.json").read_text())
+        chosen = [(m["shape"], m["n"], fm.Query.load_g(str(Path(load) / m["file"]))) for m in man["queries"]]
+        counts = [m["count"] for m in man["queries"]]
+        balg = [m["balg"] for m in man["queries"]]
+        queries = [c[2] for c in chosen]
+        tune = TuneResult(man["best"], man["best_ms"], [])
+    else:
+        chosen, counts = select_mixed(engine, g, 3, seed, nq, budget)
+        queries = [c[2] for c in chosen]
+        log(f"batched leg: {len(queries)} queries selected in {time.time() - t1:.1f} s")
+        ri = engine.match_batch(g, queries, table=False, instrument=True, timeout_ms=300000)
+        balg = [b_alg(ri.query(i), chosen[i][1], table=True) for i in range(len(queries))]
+        ri.close()
+        space = {"root_chunk": [1024, 4096], "task_rows": [1024, 4096], "grid_frac": [1.0, 1.5]}
+        tune = autotune(engine, g, queries, space=space, table=False, log=log)
+        if save:
+            d = Path(save)
+            d.mkdir(parents=True, exist_ok=True)
+            items = []
+            for i, (shape, n, q) in enumerate(chosen):
+                q.write_g(str(d / f"q{i:04d}.g"))
+                items.append({"file": f"q{i:04d}.g", "shape": shape, "n": n, "count": counts[i], "balg": balg[i]})
+            (d / "manifest.json").write_text(json.dumps({"config": 3, "seed": seed, "budget": budget, "queries": items,
+                                                         "best": tune.best, "best_ms": tune.best_ms}))
     best = tune.best
     log(f"autotune: best {best} {tune.best_ms:.2f} ms over {len(tune.trials)} configurations")
     # plain: one launch per query, with the library's single-query defaults
@@ -651,9 +674,11 @@ def main():
     graph_desc = f"rmat scale {gi.vertices.bit_length() - 1}, {gi.edges} directed edges, {gi.labels} labels"
 
     if args.config == 3:
-        res, chosen = run_batched(g, args.seed, args.queries or 1024, args.budget, args.steps, args.warmup)
-        with fm.Engine() as engine:
-            res["intersect_roofline"] = intersect_bench(engine)
+        res, chosen = run_batched(g, args.seed, args.queries or 1024, args.budget, args.steps, args.warmup,
+                                  save=args.save_queries, load=args.load_queries)
+        if not args.load_queries:
+            with fm.Engine() as engine:
+                res["intersect_roofline"] = intersect_bench(engine)
         cpu = None
         if not args.no_cpu:
             cpu = cpu_baselines(g, chosen[:32], args.cpu_seconds)
@@ -736,7 +761,7 @@ def main():
     if args.config == 2:
         pr = run_plain(engine, g, queries, tables, counts, args.steps, args.warmup, validate=manifest is None)
         ntree = sum(1 for t in tables if not t)
-        workload = f"rmat20_plain_{len(queries)}q_{ntree}trees_count_{len(queries) - ntree}cyclic_table"
+        workload = f"rmat20_plain_{len(queries)}q_{ntree}treelike_count_{len(queries) - ntree}cyclic_table"
         dom_ms = float(np.sum(pr["kernel_ms"]))
         nl = len(queries)
         achieved = sum(balg) / (dom_ms / 1e3) / 1e9
@@ -753,7 +778,9 @@ def main():
             "wave_busy_frac": pr["wave_busy_frac"],
             "parity_checks_passed": pr["parity_checks_passed"],
             "embeddings_per_step": embeddings,
-            "count_only_trees": pr["count_only_trees"],
+            "count_only_treelike": pr["count_only_trees"],
+            "treelike_shapes": {"tree": sum(1 for c, t in zip(chosen, tables) if not t and c[0] == 0),
+                                "one_cycle": sum(1 for c, t in zip(chosen, tables) if not t and c[0] == 3)},
             "cyclic_tables": pr["cyclic_tables"],
             "launch_per_step": pr["per_step"],
             # per launch: algorithmic bytes (mean over the step's launches)
