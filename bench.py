@@ -555,7 +555,11 @@ def run_batched(g, seed: int, nq: int, budget: int, steps: int, warmup: int, sav
         ri = engine.match_batch(g, queries, table=False, instrument=True, timeout_ms=300000)
         balg = [b_alg(ri.query(i), chosen[i][1], table=True) for i in range(len(queries))]
         ri.close()
-        space = {"root_chunk": [1024, 4096], "task_rows": [1024, 4096], "grid_frac": [1.0, 1.5]}
+        # 1,024 queries: 12 waves per CU (grid_frac 1.5) was faster than 8 in
+        # every earlier sweep, and root chunks of 1,024 roots tie with 16-64
+        # waves per query, so the sweep covers the other knobs
+        space = {"waves_per_query": [16, 64], "tile_parents": [32, 64], "task_rows": [1024, 4096],
+                 "grid_frac": [1.5]}
         tune = autotune(engine, g, queries, space=space, table=False, log=log)
         if save:
             d = Path(save)

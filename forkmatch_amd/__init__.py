@@ -376,8 +376,10 @@ class Engine:
 
     @staticmethod
     def _opts(mode, instrument, max_rows, max_count, timeout_ms, grid_waves, task_rows, root_chunk,
-              max_partials=0, plan=0, symmetry=False):
+              max_partials=0, plan=0, symmetry=False, waves_per_query=0, tile_parents=0):
         o = L.MatchOpts()
+        o.waves_per_query = waves_per_query
+        o.tile_parents = tile_parents
         o.max_partials = max_partials
         o.plan = plan
         o.symmetry = 1 if symmetry else 0
@@ -397,9 +399,16 @@ class Engine:
     def match_batch(self, graph: Graph, queries, table: bool = True, instrument: bool = False,
                     max_rows: int = 0, max_count: int = 0, timeout_ms: int = 0, grid_waves: int = 0,
                     task_rows: int = 0, root_chunk: int = 0, max_partials: int = 0,
-                    allow_errors: bool = False, plan: int = 0, symmetry: bool = False) -> Result:
+                    allow_errors: bool = False, plan: int = 0, symmetry: bool = False, waves_per_query: int = 0,
+                    tile_parents: int = 0) -> Result:
+        """Matches the queries in one persistent launch. Launch knobs (0 =
+        the library's default): grid_waves, task_rows (parents per queued
+        task: the fork threshold), root_chunk (roots per claim),
+        waves_per_query (a query's roots cut into this many claims, in place
+        of root_chunk) and tile_parents (parents staged in LDS per step)."""
         o = self._opts(FM_MODE_TABLE if table else FM_MODE_COUNT, instrument, max_rows, max_count, timeout_ms,
-                       grid_waves, task_rows, root_chunk, max_partials, plan, symmetry)
+                       grid_waves, task_rows, root_chunk, max_partials, plan, symmetry, waves_per_query,
+                       tile_parents)
         arr = (C.c_void_p * len(queries))(*[q._h for q in queries])
         h = C.c_void_p()
         rc = L.lib().fm_match_batch(self._h, graph._h, arr, len(queries), C.byref(o), C.byref(h))

@@ -42,7 +42,8 @@ class EngineInfo(C.Structure):
 class MatchOpts(C.Structure):
     _fields_ = [("mode", u32), ("instrument", u32), ("max_rows", u64), ("max_count", u64),
                 ("timeout_ms", u32), ("grid_waves", u32), ("task_rows", u32), ("root_chunk", u32),
-                ("plan", u32), ("max_partials", u64), ("symmetry", u32), ("reserved", u32 * 3)]
+                ("plan", u32), ("max_partials", u64), ("symmetry", u32), ("waves_per_query", u32),
+                ("tile_parents", u32), ("reserved", u32 * 1)]
 
 
 class QueryResult(C.Structure):

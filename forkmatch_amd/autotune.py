@@ -1,12 +1,18 @@
 """Autotuner for the batched persistent launch.
 
-The launch shape of ``fm_match_batch`` has three runtime knobs:
-- ``root_chunk``: roots a wave claims at a time. This sets how many waves
-  one query's roots spread over; it is the "wavefronts per query" knob.
+The launch shape of ``fm_match_batch`` has these runtime knobs:
+- ``waves_per_query``: the number of claims a query's root list is cut into,
+  i.e. how many waves a query fans out to when it starts (deeper work then
+  spreads by splitting). ``root_chunk`` is the same knob as a fixed number of
+  roots per claim for every query; a configuration sets one or the other.
+- ``tile_parents``: the LDS tile, parents staged per extension step with
+  their requirement rows (at most 64; wide rows stage fewer).
 - ``task_rows``: parents per queued task. This is the fork threshold, the
-  replacement of the reference's soft bank limit (SubMaxMulSpill).
-- ``grid_waves``: persistent waves. Fewer waves leave more LDS and L2 per
-  wave; more waves hide more memory latency.
+  replacement of the reference's soft bank limit (SubMaxMulSpill,
+  src/environment.cuh).
+- ``grid_waves``: persistent waves, as ``grid_frac`` of the engine's default
+  grid. Fewer waves leave more LDS and L2 per wave; more waves hide more
+  memory latency.
 
 Each candidate configuration is run on the real workload in count mode and
 timed with the kernel's HIP events. The counts of every run must agree, so
@@ -27,10 +33,13 @@ class TuneResult:
 
 
 DEFAULT_SPACE = {
-    "root_chunk": [256, 1024, 4096],
-    "task_rows": [256, 1024, 4096],
-    "grid_frac": [0.5, 1.0, 1.5],
+    "waves_per_query": [16, 64, 256],
+    "tile_parents": [32, 64],
+    "task_rows": [1024, 4096],
+    "grid_frac": [1.0, 1.5],
 }
+
+KNOBS = ("root_chunk", "waves_per_query", "tile_parents", "task_rows")
 
 
 def autotune(engine, graph, queries, space: dict | None = None, table: bool = False, reps: int = 1,
@@ -45,8 +54,8 @@ def autotune(engine, graph, queries, space: dict | None = None, table: bool = Fa
         cfg = dict(zip(keys, combo))
         # grid_frac is relative to the engine's default grid (8 waves per CU);
         # the library caps it at the resident waves
-        opts = {"root_chunk": cfg.get("root_chunk", 0), "task_rows": cfg.get("task_rows", 0),
-                "grid_waves": max(1, int(grid * cfg.get("grid_frac", 1.0)))}
+        opts = {k: cfg[k] for k in KNOBS if k in cfg}
+        opts["grid_waves"] = max(1, int(grid * cfg.get("grid_frac", 1.0)))
         ms = []
         for _ in range(reps):
             r = engine.match_batch(graph, queries, table=table, timeout_ms=timeout_ms, **opts)

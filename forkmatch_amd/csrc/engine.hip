@@ -101,7 +101,7 @@ struct DQuery {
     uint32_t max_pages;
     uint32_t npages;  // pages claimed
     uint32_t root_chunk_begin;
-    uint32_t root_chunks;
+    uint32_t chunk_roots;  // roots per root claim of this query
     uint32_t pad;
 };
 
@@ -262,6 +262,7 @@ struct KParams {
     uint32_t total_root_chunks;
     uint32_t root_chunk;
     uint32_t task_rows;
+    uint32_t tile_parents;  // parents staged in LDS per batch (1..WAVE)
     uint32_t table;
     uint32_t instrument;
     uint32_t qcap;
@@ -722,13 +723,14 @@ __device__ bool claim_root(const KParams &P, TaskDesc &t, unsigned long long *st
         else hi = mid - 1;
     }
     const DPlan &pl = P.plans[lo];
+    const uint32_t per = P.qs[lo].chunk_roots;
     uint32_t local = (uint32_t)k - P.qs[lo].root_chunk_begin;
-    uint32_t first = local * P.root_chunk;
+    uint32_t first = local * per;
     t.q = lo;
     t.depth = 0;
     t.bank = ROOT_BANK;
     t.begin = first;
-    t.count = min(P.root_chunk, pl.nroots - first);
+    t.count = min(per, pl.nroots - first);
     t.skip = 0;
     t.last_end = ALL_CANDIDATES;
     stats[ST_ROOTS]++;
@@ -955,8 +957,9 @@ struct WaveOut {
 };
 
 // LDS words holding a batch's parent rows and requirement rows (start,
-// length); a batch takes min(64, POOL_WORDS / (W + 2k)) parents of width W
-// with k requirement rows each, so the pool fits any plan (W <= 32, k <= 16).
+// length); a batch takes min(tile_parents, POOL_WORDS / (W + 2k)) parents of
+// width W with k requirement rows each (tile_parents <= 64), so the pool fits
+// any plan (W <= 32, k <= 16).
 constexpr uint32_t POOL_WORDS = 2048;
 static_assert(POOL_WORDS / (MAXN + 2 * MAXR) >= 1, "pool too small for the widest batch");
 
@@ -1091,7 +1094,7 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
     const uint32_t k = pl.req_begin[D + 1] - rb - nself;  // anchored requirements
 
     // batch layout in the LDS pool
-    const uint32_t NB = min(WAVE, POOL_WORDS / (W + 2 * k));
+    const uint32_t NB = min(P.tile_parents, POOL_WORDS / (W + 2 * k));
     uint32_t *const par = s.pool;
     uint32_t *const rows_st = s.pool + NB * W;
     uint32_t *const rows_len = rows_st + NB * k;
@@ -3029,6 +3032,9 @@ int fm_match_batch(fm_engine *e, const fm_graph *g, const fm_query *const *qs, u
     if (opts) o = *opts;
     if (o.timeout_ms == 0) o.timeout_ms = 120000;
     if (o.task_rows == 0) o.task_rows = nq == 1 ? SubMaxMulSpill : FM_BATCH_TASK_ROWS;
+    if (o.tile_parents > WAVE)
+        return fail(FM_ERR_INVALID_ARG, "tile_parents %u is above %u", o.tile_parents, WAVE);
+    if (o.tile_parents == 0) o.tile_parents = WAVE;
     if (int rc = check_prealloc(e, g)) return rc;
     const bool default_chunk = o.root_chunk == 0;
     if (o.root_chunk == 0) o.root_chunk = TooLargeThreshold;
@@ -3068,7 +3074,7 @@ int fm_match_batch(fm_engine *e, const fm_graph *g, const fm_query *const *qs, u
         if (!qs[i]) return fail(FM_ERR_INVALID_ARG, "null query %u", i);
         int rc = make_dplan(g, qs[i]->h, o.instrument != 0, o.plan, o.symmetry, plans[i]);
         if (rc) return rc;
-        if (nq == 1 && default_chunk) {
+        if (nq == 1 && default_chunk && !o.waves_per_query) {
             // One query: about 8 root chunks per starting wave, between 64
             // roots (one batch) and TooLargeThreshold, so that waves that
             // draw hub roots do not hold the root depth back. Measured on
@@ -3091,9 +3097,15 @@ int fm_match_batch(fm_engine *e, const fm_graph *g, const fm_query *const *qs, u
         Q.max_pages = (o.mode == FM_MODE_TABLE) ? (uint32_t)std::min<uint64_t>(want_pages, e->npages) : 0;
         Q.max_count = o.max_count;
         Q.max_partials = o.max_partials;
+        // roots per claim: root_chunk, or the query's roots cut into
+        // waves_per_query claims
+        const uint32_t nroots = plans[i].nroots;
+        Q.chunk_roots = o.waves_per_query
+                            ? std::max<uint32_t>(1, (uint32_t)(((uint64_t)nroots + o.waves_per_query - 1) /
+                                                               o.waves_per_query))
+                            : o.root_chunk;
         Q.root_chunk_begin = (uint32_t)chunks;
-        Q.root_chunks = (plans[i].nroots + o.root_chunk - 1) / o.root_chunk;
-        chunks += Q.root_chunks;
+        chunks += (nroots + (uint64_t)Q.chunk_roots - 1) / Q.chunk_roots;
     }
     if (chunks >= (1ull << 32)) return fail(FM_ERR_INVALID_ARG, "too many root chunks");
     // ---- device buffers ----
@@ -3140,6 +3152,7 @@ int fm_match_batch(fm_engine *e, const fm_graph *g, const fm_query *const *qs, u
     P.total_root_chunks = (uint32_t)chunks;
     P.root_chunk = o.root_chunk;
     P.task_rows = o.task_rows;
+    P.tile_parents = o.tile_parents;
     P.table = o.mode == FM_MODE_TABLE ? 1 : 0;
     P.instrument = o.instrument ? 1 : 0;
     P.qcap = e->qcap;
@@ -3380,6 +3393,7 @@ static int server_launch(fm_server *s) {
     P.total_root_chunks = 0;
     P.root_chunk = s->cfg.root_chunk;
     P.task_rows = s->cfg.task_rows;
+    P.tile_parents = WAVE;
     P.table = s->cfg.table ? 1 : 0;
     P.qcap = e->qcap;
     P.fcap = e->fcap;

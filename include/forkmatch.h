@@ -296,7 +296,15 @@ typedef struct fm_match_opts {
      * reference counterpart: the reference's rule (ccsr.cuh:342-676,
      * smsm.cu:187-227) is inexact for C5, C6 and the bowtie. */
     uint32_t symmetry;
-    uint32_t reserved[3];
+    /* Waves a query's roots fan out to when it starts: its root list is cut
+     * into this many claims (root_chunk is then ignored; 0: root_chunk
+     * roots per claim). Deeper work spreads further by splitting. */
+    uint32_t waves_per_query;
+    /* LDS tile: parent rows staged per extension step, with their
+     * requirement rows (1..64; 0: 64; wide rows may stage fewer, as the
+     * pool holds 2,048 words). */
+    uint32_t tile_parents;
+    uint32_t reserved[1];
 } fm_match_opts;
 
 /* One query in one launch (the reference's match()). */

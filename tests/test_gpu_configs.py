@@ -168,7 +168,8 @@ def test_config3_batched_rmat20_equals_oracle(rmat20):
     trees, 40% cycle+chord, 20% near-cliques; shapes that do not fill their
     share are topped up by the others) on the full 2^20-vertex graph: every
     count and every sorted table equals the oracle's, with the default batch
-    knobs and with another task size, root chunk and grid."""
+    knobs and with other values of every autotuner knob (task size, root
+    chunk, grid, waves per query, LDS tile of parents)."""
     g, og = rmat20
     nq = 128
     caps = {0: 51, 1: 51, 2: 26}
@@ -187,7 +188,9 @@ def test_config3_batched_rmat20_equals_oracle(rmat20):
     t0 = time.time()
     refs = [oracle_ref(og, g, q) for q in queries]
     log(f"config 3: oracle in {time.time() - t0:.1f} s")
-    for knobs in ({}, {"task_rows": 1024, "root_chunk": 4096, "grid_waves": 2048}):
+    for knobs in ({}, {"task_rows": 1024, "root_chunk": 4096, "grid_waves": 2048},
+                  {"waves_per_query": 3, "tile_parents": 7},
+                  {"waves_per_query": 1000, "tile_parents": 32, "task_rows": 64}):
         rb = engine.match_batch(g, queries, timeout_ms=TIMEOUT_MS, **knobs)
         for i, ref in enumerate(refs):
             out = rb.query(i)

@@ -198,6 +198,39 @@ def test_instrumented_counters_match_oracle(engine, er, plan):
         assert out.partials == ref.partials, name
 
 
+def test_launch_knobs_keep_tables_and_counters(engine):
+    """The autotuner's knobs change how the work is cut, never the result.
+    A batch of walk queries on a skewed R-MAT graph gives the oracle's
+    tables for every LDS tile (1, 5 or 64 parents staged per step) and for
+    waves per query from 1 (a query's roots in one claim) to more claims
+    than roots; one query's instrumented counters equal the oracle's for
+    every tile. A tile above 64 parents is refused."""
+    g = fm.Graph.rmat(11, 12000, 12, zipf_s=1.2, seed=21).upload()
+    og = _oracle_graph(g)
+    qs = [fm.Query.random_walk(g, n, shape=shape, min_cycles=1 if shape == 1 else 0, seed=seed)
+          for n, shape, seed in [(5, 0, 401), (6, 0, 404), (6, 1, 401), (7, 1, 401), (7, 2, 401), (9, 2, 401),
+                                 (4, 4, 3), (12, 2, 402)]]
+    refs = [orc.match(og, q.vertices, q.edges(), threads=4) for q in qs]
+    assert sum(1 for ref in refs if ref.count > 0) >= 6
+    for knobs in ({"tile_parents": 1}, {"tile_parents": 5, "waves_per_query": 1},
+                  {"tile_parents": 64, "waves_per_query": 1 << 20}, {"waves_per_query": 7, "task_rows": 16}):
+        r = engine.match_batch(g, qs, timeout_ms=TIMEOUT_MS, **knobs)
+        for i, ref in enumerate(refs):
+            assert r.count(i) == ref.count, (knobs, i)
+            assert np.array_equal(r.table(i), ref.table), (knobs, i)
+        r.close()
+    q = qs[3]
+    ref = orc.match(og, q.vertices, q.edges(), table=False, instrument=True, order=q.plan(g).order)
+    for tp in (1, 5, 64):
+        out = engine.match(g, q, table=False, instrument=True, tile_parents=tp, timeout_ms=TIMEOUT_MS).query(0)
+        assert out.count == ref.count, tp
+        assert out.alg_row_bytes == 4 * ref.row_words, tp
+        assert out.partials == ref.partials, tp
+    with pytest.raises(fm.ForkMatchError) as err:
+        engine.match(g, q, tile_parents=65, timeout_ms=TIMEOUT_MS)
+    assert err.value.code == 300
+
+
 def test_cost_and_reference_plans_give_the_same_table(engine):
     """Random-walk queries on a skewed R-MAT graph with Zipf labels: both
     matching orders give the oracle's sorted table."""
