@@ -1226,72 +1226,102 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
             pi_out = lo;
             x_out = g.col[s.dstart[lo] + (tt - excl)];
         };
+        // Work splitting: when waves are idle, donate half of what is
+        // left: the upper half of the remaining parents when at least
+        // two batches remain, else the upper half of this batch's
+        // remaining candidates together with the parents after them (a
+        // hub parent's long driver row is split across waves).
+        auto maybe_split = [&](uint32_t base) __attribute__((always_inline)) {
+            uint32_t want = 0;
+            // donate only while idle waves outnumber the waiting work
+            // (queued tasks and unclaimed root chunks): with thousands
+            // of waves idle, every busy wave would otherwise cut its
+            // work into pieces at every check
+            if (lane == 0) {
+                long long idle_waves = 0;
+                for (uint32_t h = 0; h < HUNGRY_SHARDS; h++) idle_waves += ld_rlx(&P.ctl->hungry[h].n);
+                if (idle_waves > 0) {
+                    const unsigned long long rc = ld_rlx(&P.ctl->root_cursor);
+                    const long long roots_left =
+                        rc < P.total_root_chunks ? (long long)(P.total_root_chunks - rc) : 0;
+                    want = idle_waves > max(0, ld_rlx(&P.ctl->queued)) + roots_left ? 1u : 0u;
+                }
+            }
+            want = bcast(want, 0);
+            run_rows = want ? min(P.task_rows, WAVE) : P.task_rows;
+            if (want) {
+                // a candidate split may already have ended the task inside this batch
+                const uint32_t rem_par = cnt_local > b0 + nb ? cnt_local - (b0 + nb) : 0;
+                if (rem_par >= 2 * WAVE) {
+                    const uint32_t keep = rem_par / 2, first = b0 + nb + keep;
+                    TaskDesc r{t.q, d, t.bank, root ? t.begin + first : t.begin + first * W, rem_par - keep, 0,
+                               last_end};
+                    push_tasks(P, lvl_self, 1, r, t.bank, stats);
+                    cnt_local = first;
+                    last_end = ALL_CANDIDATES;
+                    if (lane == 0) stats[ST_SPLIT]++;
+                } else if (T - base >= 8 * WAVE) {
+                    const uint32_t mid = base + ((T - base) / (2 * WAVE)) * WAVE;
+                    uint32_t lo = 0, hi = nb - 1;
+                    while (lo < hi) {
+                        uint32_t m2 = (lo + hi) >> 1;
+                        if (s.cum[m2] > mid) hi = m2;
+                        else lo = m2 + 1;
+                    }
+                    const uint32_t excl = lo ? s.cum[lo - 1] : 0;
+                    // driver-row offset of candidate `mid` in parent lo
+                    const uint32_t abs_mid = (mid - excl) + ((b0 == 0 && lo == 0) ? t.skip : 0);
+                    TaskDesc r{t.q, d, t.bank, root ? t.begin + b0 + lo : t.begin + (b0 + lo) * W,
+                               cnt_local - b0 - lo, abs_mid, last_end};
+                    push_tasks(P, lvl_self, 1, r, t.bank, stats);
+                    // this task now ends inside parent lo
+                    cnt_local = b0 + lo + 1;
+                    last_end = abs_mid;
+                    T = mid;
+                    if (lane == 0) stats[ST_SPLIT]++;
+                }
+            }
+        };
+        // Count-only last level whose vertex has one requirement (the last
+        // vertex of the order is a leaf: trees, and cycles closed at an
+        // earlier depth): a candidate needs no membership probe, only the
+        // check that it is not a vertex of its parent (LDS). Every lane then
+        // keeps LEAF_U driver-row loads in flight per iteration, where the
+        // general loop below has one step prefetched. Every candidate is
+        // still loaded and checked one by one.
+        const bool leaf_count =
+            final_level && !P.table && k == 1 && nself == 0 && pl.sc_begin[D] == pl.sc_begin[D + 1];
+        if (leaf_count) {
+            constexpr uint32_t LEAF_U = 8;
+            uint32_t hits = 0;
+            for (uint32_t base = 0, it = 0; base < T; base += LEAF_U * WAVE, it++) {
+                if ((it & 1u) == 0) maybe_split(base);  // every 1,024 candidates (T stays > base)
+                uint32_t xs[LEAF_U], ps[LEAF_U];
+#pragma unroll
+                for (uint32_t u = 0; u < LEAF_U; u++) locate(base + u * WAVE, ps[u], xs[u]);
+#pragma unroll
+                for (uint32_t u = 0; u < LEAF_U; u++) {
+                    const uint32_t *const prow = &par[ps[u] * W];
+                    bool ok = base + u * WAVE + lane < T;
+                    for (uint32_t a = 0; a < W; a++) ok &= prow[a] != xs[u];
+                    hits += ok ? 1u : 0u;
+                }
+            }
+            const unsigned long long nh = wave_sum_u64(hits);
+            if (P.instrument && lane == 0) s.inst[4 + D] += nh;
+            cnt_acc += nh * (pl.naut > 1 ? pl.naut : 1u);
+        }
         // Each step takes STEP candidates: one per lane, or with PAIRED two
         // (base + lane and base + 64 + lane) whose probes are in flight
         // together.
         constexpr uint32_t STEP = PAIRED ? 2 * WAVE : WAVE;
         uint32_t pf_pi = 0, pf_x = 0, pf_pi2 = 0, pf_x2 = 0;
-        if (T) {
+        if (T && !leaf_count) {
             locate(0, pf_pi, pf_x);
             if constexpr (PAIRED) locate(WAVE, pf_pi2, pf_x2);
         }
-        for (uint32_t base = 0, step = 0; base < T; base += STEP, step++) {
-            // Work splitting: when waves are idle, donate half of what is
-            // left: the upper half of the remaining parents when at least
-            // two batches remain, else the upper half of this batch's
-            // remaining candidates together with the parents after them (a
-            // hub parent's long driver row is split across waves).
-            if ((step & (PAIRED ? 1u : 3u)) == 0) {  // every 256 candidates
-                uint32_t want = 0;
-                // donate only while idle waves outnumber the waiting work
-                // (queued tasks and unclaimed root chunks): with thousands
-                // of waves idle, every busy wave would otherwise cut its
-                // work into pieces at every check
-                if (lane == 0) {
-                    long long idle_waves = 0;
-                    for (uint32_t h = 0; h < HUNGRY_SHARDS; h++) idle_waves += ld_rlx(&P.ctl->hungry[h].n);
-                    if (idle_waves > 0) {
-                        const unsigned long long rc = ld_rlx(&P.ctl->root_cursor);
-                        const long long roots_left =
-                            rc < P.total_root_chunks ? (long long)(P.total_root_chunks - rc) : 0;
-                        want = idle_waves > max(0, ld_rlx(&P.ctl->queued)) + roots_left ? 1u : 0u;
-                    }
-                }
-                want = bcast(want, 0);
-                run_rows = want ? min(P.task_rows, WAVE) : P.task_rows;
-                if (want) {
-                    // a candidate split may already have ended the task inside this batch
-                    const uint32_t rem_par = cnt_local > b0 + nb ? cnt_local - (b0 + nb) : 0;
-                    if (rem_par >= 2 * WAVE) {
-                        const uint32_t keep = rem_par / 2, first = b0 + nb + keep;
-                        TaskDesc r{t.q, d, t.bank, root ? t.begin + first : t.begin + first * W, rem_par - keep, 0,
-                                   last_end};
-                        push_tasks(P, lvl_self, 1, r, t.bank, stats);
-                        cnt_local = first;
-                        last_end = ALL_CANDIDATES;
-                        if (lane == 0) stats[ST_SPLIT]++;
-                    } else if (T - base >= 8 * WAVE) {
-                        const uint32_t mid = base + ((T - base) / (2 * WAVE)) * WAVE;
-                        uint32_t lo = 0, hi = nb - 1;
-                        while (lo < hi) {
-                            uint32_t m2 = (lo + hi) >> 1;
-                            if (s.cum[m2] > mid) hi = m2;
-                            else lo = m2 + 1;
-                        }
-                        const uint32_t excl = lo ? s.cum[lo - 1] : 0;
-                        // driver-row offset of candidate `mid` in parent lo
-                        const uint32_t abs_mid = (mid - excl) + ((b0 == 0 && lo == 0) ? t.skip : 0);
-                        TaskDesc r{t.q, d, t.bank, root ? t.begin + b0 + lo : t.begin + (b0 + lo) * W,
-                                   cnt_local - b0 - lo, abs_mid, last_end};
-                        push_tasks(P, lvl_self, 1, r, t.bank, stats);
-                        // this task now ends inside parent lo
-                        cnt_local = b0 + lo + 1;
-                        last_end = abs_mid;
-                        T = mid;
-                        if (lane == 0) stats[ST_SPLIT]++;
-                    }
-                }
-            }
+        for (uint32_t base = 0, step = 0; base < T && !leaf_count; base += STEP, step++) {
+            if ((step & (PAIRED ? 1u : 3u)) == 0) maybe_split(base);  // every 256 candidates
             if (!final_level) {
                 const uint32_t need = STEP * (D + 1);
                 static_assert(2 * WAVE * MAXN <= 2 * MIN_BANK_SIZE, "a fresh bank holds one step of children");
