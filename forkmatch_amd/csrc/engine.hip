@@ -1955,14 +1955,23 @@ __device__ __forceinline__ uint32_t isect_tiles(uint32_t la, uint32_t lb) {
                                 : (uint32_t)(((unsigned long long)na + nb + XT - 1) / XT);
 }
 
+// Size class of a tile count nt >= 1, 0..127: 4 * (bit length - 1) plus the
+// two bits after the leading one, so the classes grow by about 19% each.
+__device__ __forceinline__ uint32_t isect_size_class(uint32_t nt) {
+    const uint32_t b = 31 - __clz(nt);
+    const uint32_t frac = b >= 2 ? (nt >> (b - 2)) & 3u : (nt << (2 - b)) & 3u;
+    return 4 * b + frac;
+}
+
 // One block of 1,024 threads. tile_begin by a scan of the tile counts, the
-// pairs with tiles in order of their bit length of tiles, largest first;
+// pairs with tiles in order of their tile count (quarter-octave buckets:
+// the bit length and the next two bits), largest first;
 // counts cleared; work[0] = 0 (claim counter), work[1] = pairs with tiles,
 // work[2] = tiles, work[3] = 1 (and no work) when the tiles exceed cap.
 __global__ __launch_bounds__(1024) void fm_isect_plan_kernel(const uint32_t *a_len, const uint32_t *b_len,
                                                              uint32_t npairs, uint32_t *counts, uint32_t *work,
                                                              uint32_t cap) {
-    constexpr uint32_t NB = 33;  // bit lengths of a tile count (< 2^32)
+    constexpr uint32_t NB = 128;  // size classes of a tile count (< 2^32)
     __shared__ uint32_t hist[NB], start[NB];
     __shared__ unsigned long long s_part[1024 / WAVE];
     __shared__ unsigned long long s_carry;
@@ -1976,7 +1985,7 @@ __global__ __launch_bounds__(1024) void fm_isect_plan_kernel(const uint32_t *a_l
         const uint32_t p = p0 + tid;
         const uint32_t nt = p < npairs ? isect_tiles(a_len[p], b_len[p]) : 0u;
         if (p < npairs) counts[p] = 0;
-        if (nt) atomicAdd(&hist[(NB - 1) - (31 - __clz(nt))], 1u);
+        if (nt) atomicAdd(&hist[(NB - 1) - isect_size_class(nt)], 1u);
         // block scan of the tile counts (64-bit: the total may pass 2^32)
         unsigned long long v = nt;
 #pragma unroll
@@ -2013,7 +2022,7 @@ __global__ __launch_bounds__(1024) void fm_isect_plan_kernel(const uint32_t *a_l
     __syncthreads();
     for (uint32_t p = tid; p < npairs; p += 1024) {
         const uint32_t nt = isect_tiles(a_len[p], b_len[p]);
-        if (nt) order[atomicAdd(&start[(NB - 1) - (31 - __clz(nt))], 1u)] = p;
+        if (nt) order[atomicAdd(&start[(NB - 1) - isect_size_class(nt)], 1u)] = p;
     }
 }
 
@@ -2055,12 +2064,15 @@ __global__ __launch_bounds__(IB) void fm_isect_cut_kernel(const uint32_t *__rest
 // prefetch register makes the compiler wait for the loads at the loop join,
 // exposing their latency on every tile). The matches leave through LDS in
 // coalesced stores at the pair's running offset.
-__global__ __launch_bounds__(IB) void fm_isect_kernel(const uint32_t *__restrict__ data, const uint64_t *a_off,
+__global__ __launch_bounds__(IB) __attribute__((amdgpu_waves_per_eu(5, 5))) void fm_isect_kernel(const uint32_t *__restrict__ data, const uint64_t *a_off,
                                                       const uint32_t *a_len, const uint64_t *b_off,
                                                       const uint32_t *b_len, uint32_t *counts,
                                                       const uint64_t *out_off, uint32_t *out, uint32_t npairs,
                                                       uint32_t *work, uint32_t cap) {
-    __shared__ uint32_t sAB[XT + 1];  // a tile's A slice, then its B slice (+ the next B element)
+    // a tile's A slice, then its B slice (+ the next B element), each
+    // starting on a chunk of IB words
+    constexpr uint32_t XR = XVT + 2;  // ceil(ta / IB) + ceil((tb + 1) / IB) <= XVT + 2
+    __shared__ uint32_t sAB[XR * IB];
     // A match takes two merge steps (its A element, then the equal B
     // element), so a stretch holds at most XM matches. They are parked at
     // tid * (XM + 1) (an odd stride: conflict-free banks) and then moved to
@@ -2132,37 +2144,35 @@ __global__ __launch_bounds__(IB) void fm_isect_kernel(const uint32_t *__restrict
                 __syncthreads();  // s_out and s_cut are rewritten by the next tile
             }
         } else {
-            uint32_t reg[XVT + 1];
+            // A tile's LDS image: its A slice from word 0 and its B slice
+            // (with the next B element) from word xa * IB, xa = ceil(ta / IB).
+            // Register x holds word x * IB + tid of the image, so a register
+            // is a whole chunk of A or of B: its load address is a scalar
+            // base plus tid (clamped to the list), and its LDS store needs no
+            // test. Words past a slice hold clamped copies that the merge
+            // never takes.
+            uint32_t reg[XR];
             uint32_t nc0 = 0, nc1 = 0;  // cuts of the tile in reg
-            // Tile t's elements into reg: every load is unconditional and in
-            // bounds (lanes past the tile read B's last element; the store
-            // below skips them).
             auto load_tile = [&](uint32_t t) {
                 nc0 = s_cut[t - cbase];
                 nc1 = s_cut[t - cbase + 1];
-                const uint32_t d0 = (t - tb) * XT;
-                const uint32_t b0 = d0 - nc0, ta = nc1 - nc0;
+                const uint32_t b0 = (t - tb) * XT - nc0;
+                const uint32_t xa = (nc1 - nc0 + IB - 1) / IB;
 #pragma unroll
-                for (uint32_t x = 0; x <= XVT; x++) {
-                    const uint32_t e = tid + x * IB;
-                    const uint32_t *src = e < ta ? q.A + (nc0 + e) : q.B + min(b0 + (e - ta), q.nb - 1);
-                    reg[x] = *src;
+                for (uint32_t x = 0; x < XR; x++) {
+                    const bool in_a = x < xa;  // the same for the whole block
+                    const uint32_t *const src = in_a ? q.A : q.B;
+                    const uint32_t first = in_a ? nc0 + x * IB : b0 + (x - xa) * IB;
+                    reg[x] = src[min(first + tid, (in_a ? q.na : q.nb) - 1)];
                 }
             };
-            // reg (tile t, cuts c0 and c1) into sAB
-            auto store_tile = [&](uint32_t t, uint32_t c0, uint32_t c1) {
-                const uint32_t d0 = (t - tb) * XT, d1 = min(d0 + XT, total_path);
-                const uint32_t b0 = d0 - c0, b1 = d1 - c1;
-                const uint32_t n = (c1 - c0) + (b1 - b0) + (b1 < q.nb ? 1u : 0u);
+            auto store_tile = [&]() {
 #pragma unroll
-                for (uint32_t x = 0; x <= XVT; x++) {
-                    const uint32_t e = tid + x * IB;
-                    if (e < n) sAB[e] = reg[x];
-                }
+                for (uint32_t x = 0; x < XR; x++) sAB[x * IB + tid] = reg[x];
             };
             load_tile(tb);
             uint32_t c0 = nc0, c1 = nc1;
-            store_tile(tb, c0, c1);
+            store_tile();
             __syncthreads();
             for (uint32_t t = tb; t < te; t++) {
                 // t + 1 - cbase <= XCUT: the window holds the next tile's
@@ -2172,40 +2182,40 @@ __global__ __launch_bounds__(IB) void fm_isect_kernel(const uint32_t *__restrict
                 const uint32_t d0 = (t - tb) * XT, d1 = min(d0 + XT, total_path);
                 const uint32_t b0 = d0 - c0, b1 = d1 - c1;
                 const uint32_t ta = c1 - c0, tbn = b1 - b0, tbx = tbn + (b1 < q.nb ? 1u : 0u);
+                const uint32_t *const sA = sAB, *const sB = sAB + ((ta + IB - 1) / IB) * IB;
                 const uint32_t dl = min(tid * XVT, ta + tbn);
-                const uint32_t steps = min(XVT, ta + tbn - dl);
-                const uint32_t i0 = merge_split(sAB, ta, sAB + ta, tbn, dl);
+                const uint32_t i0 = merge_split(sA, ta, sB, tbn, dl);
                 const uint32_t j0 = dl - i0;
-                // Merges the stretch; emit(x) for each match. Consuming A[i], an
-                // equal B element is the next one on B's side. Branch-free
-                // steps: the lanes of a wave take A or B steps in any mix, so
-                // each step advances one side by a predicate and reloads both
-                // current elements (clamped indices, always in the tile's LDS
-                // slice; a value past a side's end is never compared).
+                // Merges the stretch; emit(match, x) at every step. Consuming
+                // A[i], an equal B element is the next one on B's side.
+                // Branch-free steps: the lanes of a wave take A or B steps in
+                // any mix, so each step advances one side by a predicate and
+                // reloads both current elements. A thread whose stretch ends
+                // at the tile's end (ta + tb) runs its remaining steps from
+                // (ta, tb): B steps only, which match nothing, so every
+                // thread runs XVT steps. Indices stay below ta + XVT and
+                // tb + XVT, inside the image.
                 auto merge_stretch = [&](auto emit) {
-                    const uint32_t *sA = sAB, *sB = sAB + ta;
-                    const uint32_t alast = ta ? ta - 1 : 0u, blast = tbx ? tbx - 1 : 0u;
                     uint32_t ii = i0, jj = j0;
-                    uint32_t av = sA[min(ii, alast)], bv = sB[min(jj, blast)];
+                    uint32_t av = sA[ii], bv = sB[jj];
 #pragma unroll
                     for (uint32_t st = 0; st < XVT; st++) {
-                        const bool live = st < steps;
                         const bool take_a = ii < ta && (jj >= tbn || av <= bv);
-                        if (live && take_a && jj < tbx && bv == av) emit(av);
-                        ii += (live && take_a) ? 1u : 0u;
-                        jj += (live && !take_a) ? 1u : 0u;
-                        av = sA[min(ii, alast)];
-                        bv = sB[min(jj, blast)];
+                        emit(take_a && jj < tbx && bv == av, av);
+                        ii += take_a ? 1u : 0u;
+                        jj += take_a ? 0u : 1u;
+                        av = sA[ii];
+                        bv = sB[jj];
                     }
                 };
                 uint32_t c = 0;
-                // strictly increasing lists give at most XM; the clamp keeps
-                // any input inside the buffers
-                if (o) merge_stretch([&](uint32_t x) {
-                    if (c < XM) s_out[tid * (XM + 1) + c] = x;
-                    c++;
+                // strictly increasing lists give at most XM matches per
+                // stretch; the clamp keeps any input inside the buffers
+                if (o) merge_stretch([&](bool m, uint32_t x) {
+                    s_out[tid * (XM + 1) + min(c, XM)] = x;
+                    c += m ? 1u : 0u;
                 });
-                else merge_stretch([&](uint32_t) { c++; });
+                else merge_stretch([&](bool m, uint32_t) { c += m ? 1u : 0u; });
                 c = min(c, XM);
                 uint32_t tot;
                 const uint32_t at = block_excl_scan(c, s_wave, tot);  // its barriers order the parking
@@ -2226,7 +2236,7 @@ __global__ __launch_bounds__(IB) void fm_isect_kernel(const uint32_t *__restrict
                 if (has_next) {
                     c0 = nc0;
                     c1 = nc1;
-                    store_tile(t + 1, c0, c1);
+                    store_tile();
                 } else if (t + 1 < te) {
                     // the next tile is past the cut window: refill, then load it
                     fill_cuts(t + 1);
@@ -2234,7 +2244,7 @@ __global__ __launch_bounds__(IB) void fm_isect_kernel(const uint32_t *__restrict
                     load_tile(t + 1);
                     c0 = nc0;
                     c1 = nc1;
-                    store_tile(t + 1, c0, c1);
+                    store_tile();
                 }
                 __syncthreads();
             }

@@ -413,6 +413,16 @@ def test_intersection_microkernel(engine):
         a_off.append(pos); a_len.append(len(a)); lists.append(a); pos += len(a)
         b_off.append(pos); b_len.append(len(b)); lists.append(b); pos += len(b)
         expected.append(np.intersect1d(a, b))
+    # equal lists (every element matches: the most matches per stretch) and
+    # lists that hold the largest and smallest u32 values
+    top = np.uint32(0xFFFFFFFF)
+    for a, b in [(np.arange(9000, dtype=np.uint32), np.arange(9000, dtype=np.uint32)),
+                 (np.array([0, 5, 7, top], np.uint32), np.array([0, 6, 7, top], np.uint32)),
+                 (np.arange(top - np.uint32(6000), top, dtype=np.uint32).astype(np.uint32)[::2].copy(),
+                  np.append(np.arange(top - np.uint32(7000), top, dtype=np.uint32)[::3], top).astype(np.uint32))]:
+        a_off.append(pos); a_len.append(len(a)); lists.append(a); pos += len(a)
+        b_off.append(pos); b_len.append(len(b)); lists.append(b); pos += len(b)
+        expected.append(np.intersect1d(a, b))
     data = np.concatenate(lists).astype(np.uint32)
     out_off = np.cumsum([0] + [min(x, y) for x, y in zip(a_len, b_len)]).astype(np.uint64)
     res = fm.intersect_pairs(engine, data, np.array(a_off, np.uint64), np.array(a_len, np.uint32),
@@ -422,6 +432,10 @@ def test_intersection_microkernel(engine):
     for i, ex in enumerate(expected):
         assert counts[i] == len(ex)
         assert np.array_equal(out[out_off[i]: out_off[i] + counts[i]], ex)
+    # counts only (no output arrays)
+    res = fm.intersect_pairs(engine, data, np.array(a_off, np.uint64), np.array(a_len, np.uint32),
+                             np.array(b_off, np.uint64), np.array(b_len, np.uint32))
+    assert [int(c) for c in res["counts"]] == [len(ex) for ex in expected]
 
 
 def test_hub_rows_are_split_across_waves(engine):

@@ -1,13 +1,16 @@
 """CPU model of the tiled intersection kernels (engine.hip:
 fm_isect_cut_kernel, fm_isect_kernel): the host tile plan (merge tiles of
 3840 merge-path elements, skew tiles of 256 elements of the shorter list when
-the longer one is >= 32x longer), each tile's cut, the per-thread 15-element
-stretches found by binary search in the tile's LDS slices, the match check
-against the next B element (staged one past the tile), and the output offset
-of each tile as the sum of the counts of the pair's earlier tiles (the running
-offset of the block that streams the pair). Checked against numpy on random
-and edge-case pairs, including matches straddling tile and thread
-boundaries."""
+the longer one is >= 32x longer), each tile's cut, the tile's LDS image (the
+A slice from word 0 and the B slice from the next 256-word chunk, loaded a
+chunk of one list per register at clamped indices), the per-thread 15-element
+stretches found by binary search in the image, the branch-free steps that
+every thread runs to the full 15 (past its stretch only B steps, which match
+nothing), the match check against the next B element (staged one past the
+tile), and the output offset of each tile as the sum of the counts of the
+pair's earlier tiles (the running offset of the block that streams the pair).
+Checked against numpy on random and edge-case pairs, including matches
+straddling tile and thread boundaries."""
 import numpy as np
 import pytest
 
@@ -59,38 +62,41 @@ def tile_matches(A, B, k, cuts, te_minus_tb):
     b0, b1 = d0 - a0, d1 - a1
     assert 0 <= a1 - a0 <= XT and 0 <= b1 - b0 <= XT
     sA = A[a0:a1]
-    sB = B[b0:b1 + (1 if b1 < nb else 0)]
-    ta, tb, tbx = a1 - a0, b1 - b0, len(sB)
-    # the element loads of the tile: thread tid, register x reads element
-    # e = tid + x * IB from A, or else from B at a clamped index
-    for x in range(XVT + 1):
-        e = np.arange(IB) + x * IB
-        ia = a0 + e[e < ta]
-        ib = np.minimum(b0 + e[e >= ta] - ta, nb - 1)
-        assert ((ia >= 0) & (ia < na)).all() and ((ib >= 0) & (ib < nb)).all()
-    # the LDS slice: A part, B part, then stale words up to XT + 1
-    sAB = np.full(XT + 1, 0xFFFFFFFF, dtype=np.int64)
-    sAB[:ta] = sA
-    sAB[ta:ta + tbx] = sB
-    alast, blast = max(ta - 1, 0), max(tbx - 1, 0)
+    ta, tb = a1 - a0, b1 - b0
+    tbx = tb + (1 if b1 < nb else 0)
+    # the LDS image: register x of thread tid holds word x * IB + tid; the A
+    # slice from word 0, the B slice (+ the next B element) from word
+    # xa * IB; every load is a chunk of one list at a clamped index
+    xa = -(-ta // IB)
+    XR = XVT + 2
+    assert xa + -(-tbx // IB) <= XR
+    img = np.empty(XR * IB, dtype=np.int64)
+    for x in range(XR):
+        e = np.arange(IB)
+        if x < xa:
+            img[x * IB:(x + 1) * IB] = A[np.minimum(a0 + x * IB + e, na - 1)]
+        else:
+            img[x * IB:(x + 1) * IB] = B[np.minimum(b0 + (x - xa) * IB + e, nb - 1)]
+    assert np.array_equal(img[:ta], sA) and np.array_equal(img[xa * IB:xa * IB + tbx], B[b0:b0 + tbx])
+    sB_at = xa * IB
     out = []
     for tid in range(IB):
         dl = min(tid * XVT, ta + tb)
-        steps = min(XVT, ta + tb - dl)
-        i = merge_split(sA, ta, sB, tb, dl)
+        i = merge_split(img, ta, img[sB_at:], tb, dl)
         j = dl - i
-        # branch-free steps as in the kernel: both current elements are read
-        # at clamped indices after every step
-        av, bv = sAB[min(i, alast)], sAB[ta + min(j, blast)]
+        # every thread runs XVT branch-free steps: one whose stretch ends at
+        # the tile's end continues from (ta, tb) with B steps only
+        av, bv = img[i], img[sB_at + j]
         for st in range(XVT):
-            live = st < steps
             take_a = i < ta and (j >= tb or av <= bv)
-            if live and take_a and j < tbx and bv == av:
+            if take_a and j < tbx and bv == av:
                 out.append(int(av))
-            i += 1 if live and take_a else 0
-            j += 1 if live and not take_a else 0
-            assert min(i, alast) <= XT and ta + min(j, blast) <= XT
-            av, bv = sAB[min(i, alast)], sAB[ta + min(j, blast)]
+            i += 1 if take_a else 0
+            j += 0 if take_a else 1
+            assert i <= ta and sB_at + j < XR * IB  # reads stay in the image
+            if not take_a and j > tb:
+                assert st >= min(XVT, ta + tb - dl)  # only past the stretch
+            av, bv = img[i], img[sB_at + j]
     return out
 
 
@@ -138,9 +144,31 @@ def test_tile_plan_counts():
     assert tile_pair == [0, 1, 1, 1]
 
 
+def size_class(nt):
+    """isect_size_class: 4 * (bit length - 1) + the two bits after the
+    leading one."""
+    b = int(nt).bit_length() - 1
+    frac = (nt >> (b - 2)) & 3 if b >= 2 else (nt << (2 - b)) & 3
+    return 4 * b + frac
+
+
+def test_size_classes_are_ordered():
+    # monotone in the tile count, within 19% of each other inside a class,
+    # and within 0..127 for every u32 count
+    prev = -1
+    for nt in list(range(1, 5000)) + [2**31 - 1, 2**31, 2**32 - 1]:
+        c = size_class(nt)
+        assert 0 <= c <= 127 and c >= prev
+        prev = c
+    for c in range(8, 120):
+        members = [nt for nt in range(1, 1 << 14) if size_class(nt) == c]
+        if members:
+            assert max(members) < 1.25 * min(members)
+
+
 def device_plan(lens, cap):
     """fm_isect_plan_kernel: tile_begin by a scan of the tile counts (each
-    entry at most cap), the pairs with tiles ordered by the bit length of
+    entry at most cap), the pairs with tiles ordered by the size class of
     their tile count, largest first, and the overflow flag (no work) when
     the tiles exceed cap."""
     counts = []
@@ -156,7 +184,7 @@ def device_plan(lens, cap):
     buckets = {}
     for p, c in enumerate(counts):
         if c:
-            buckets.setdefault(32 - int(c).bit_length(), []).append(p)
+            buckets.setdefault(127 - size_class(c), []).append(p)
     order = [p for b in sorted(buckets) for p in buckets[b]]
     return begin, order, (0 if over else len(order)), over
 
@@ -173,10 +201,10 @@ def test_device_plan_equals_the_host_plan():
     host_begin, host_tile_pair = plan(pairs)
     begin, order, nwork, over = device_plan(lens, len(host_tile_pair))
     assert begin == host_begin and not over and nwork == len(order)
-    # every pair with tiles once, largest tile count (by bit length) first
+    # every pair with tiles once, largest tile count (by size class) first
     assert sorted(order) == sorted(set(host_tile_pair))
-    bits = [int(begin[p + 1] - begin[p]).bit_length() for p in order]
-    assert bits == sorted(bits, reverse=True)
+    classes = [size_class(int(begin[p + 1] - begin[p])) for p in order]
+    assert classes == sorted(classes, reverse=True)
 
 
 def test_device_plan_flags_too_many_tiles_and_stays_in_the_array():
