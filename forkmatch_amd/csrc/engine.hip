@@ -514,7 +514,7 @@ __device__ __forceinline__ void row_of(const DGraph &g, uint32_t req, uint32_t v
 // its deadline, so no wait can outlive the launch.
 // Records why the launch timed out (first caller wins): diag[0] = site,
 // diag[1..3] = site-specific values.
-__device__ void timeout_at(const Ctl *c, uint32_t site, uint32_t a, uint32_t b, uint32_t d) {
+__device__ __forceinline__ void timeout_at(const Ctl *c, uint32_t site, uint32_t a, uint32_t b, uint32_t d) {
     uint32_t zero = 0;
     Ctl *w = const_cast<Ctl *>(c);
     if (__hip_atomic_compare_exchange_strong(&w->error, &zero, (uint32_t)FM_ERR_TIMEOUT, __ATOMIC_RELAXED,
@@ -596,7 +596,7 @@ __device__ uint32_t bank_alloc(const KParams &P, unsigned long long *stats) {
     return NO_BANK;
 }
 
-__device__ void bank_release(const KParams &P, uint32_t id) {
+__device__ __forceinline__ void bank_release(const KParams &P, uint32_t id) {
     if (atom_add(&P.bank_ref[id], -1) == 1) {
         Ctl *c = P.ctl;
         uint32_t pos = atom_add(&c->f_tail, 1u);
@@ -677,7 +677,7 @@ __device__ void push_tasks(const KParams &P, uint32_t lvl, uint32_t m, const Tas
 }
 
 // Lane 0 only: takes the task of the lowest level (fewest remaining depths).
-__device__ bool pop_task(const KParams &P, TaskDesc &t) {
+__device__ __forceinline__ bool pop_task(const KParams &P, TaskDesc &t) {
     Ctl *c = P.ctl;
     for (uint32_t lvl = 0; lvl < P.nlevels; lvl++) {
         if (ld_rlx(&c->lq[lvl].avail) <= 0) continue;
@@ -707,7 +707,7 @@ __device__ bool pop_task(const KParams &P, TaskDesc &t) {
 // host starts root_cursor at the grid size, so later claims take the chunks
 // past the first round (thousands of waves starting together would otherwise
 // queue on the cursor).
-__device__ bool claim_root(const KParams &P, TaskDesc &t, unsigned long long *stats, bool first_claim) {
+__device__ __forceinline__ bool claim_root(const KParams &P, TaskDesc &t, unsigned long long *stats, bool first_claim) {
     Ctl *c = P.ctl;
     unsigned long long k = blockIdx.x;
     if (!first_claim) {
@@ -749,7 +749,7 @@ struct WaveTab {
 };
 
 // Lane 0 only.
-__device__ void tab_release(const KParams &P, WaveTab &w) {
+__device__ __forceinline__ void tab_release(const KParams &P, WaveTab &w) {
     if (w.page == NO_PAGE) return;
     DQuery &Q = P.qs[w.q];
     st_rlx(&P.page_fill[w.page], w.fill);
@@ -949,6 +949,21 @@ __device__ void emit_rows(const KParams &P, WaveTab &wt, uint32_t q, uint32_t n,
 }
 
 // ---- the matcher --------------------------------------------------------------
+// Copies n words from global memory into LDS, eight loads per lane in
+// flight per round (a plain copy loop waits for each load before its store).
+__device__ __forceinline__ void copy_to_lds(uint32_t *dst, const uint32_t *src, uint32_t n) {
+    constexpr uint32_t U = 8;
+    const uint32_t lane = lane_id();
+    for (uint32_t w0 = 0; w0 < n; w0 += U * WAVE) {
+        uint32_t v[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) v[u] = src[min(w0 + u * WAVE + lane, n - 1)];
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++)
+            if (w0 + u * WAVE + lane < n) dst[w0 + u * WAVE + lane] = v[u];
+    }
+}
+
 
 struct WaveOut {
     uint32_t bank;       // current output bank or NO_BANK
@@ -1009,11 +1024,174 @@ __device__ void flush_run(const KParams &P, WaveOut &o, uint32_t q, uint32_t D, 
 }
 
 
+// Count-only last level whose vertex has one anchored requirement, no
+// self-loop requirement and no order constraint, for a task below the root
+// level of a launch that gathers no counters (a leaf of the matching order:
+// trees, and cycles closed at an earlier depth). A candidate then needs no
+// membership probe, only the check that it is not a vertex of its parent.
+// Batches stage up to LEAF_TILE x tile_parents parents (LEAF_TILE per lane),
+// so the staging of a batch (parent rows, one row lookup per parent, the
+// scan) is paid once per 256 parents instead of 64; every lane keeps
+// LEAF_U driver-row loads in flight. Every candidate is still loaded and
+// checked one by one. Splits and the first/last parent ranges follow
+// process_task.
+constexpr uint32_t LEAF_TILE = 4;
+constexpr uint32_t LEAF_MAX = LEAF_TILE * WAVE;  // parents per batch
+static_assert(LEAF_MAX * 2 < POOL_WORDS, "leaf batch arrays fit the pool");
+
+__device__ __forceinline__ void process_leaf_task(const KParams &P, const TaskDesc &t, Lds &s, unsigned long long *stats,
+                                  unsigned long long deadline) {
+    const uint32_t lane = lane_id();
+    const uint32_t qi = qslot(t.q);
+    const DPlan &pl = P.plans[qi];
+    DQuery &Q = P.qs[qi];
+    const DGraph &g = P.g;
+    const uint32_t n = pl.n, d = t.depth, W = d + 1, D = d + 1;
+    const uint32_t req = pl.req[pl.req_begin[D]];
+    const uint32_t lvl_self = n - 2 - d;
+    // pool: cum[LEAF_MAX] | dstart[LEAF_MAX] | parent rows
+    uint32_t *const cum = s.pool;
+    uint32_t *const dstart = s.pool + LEAF_MAX;
+    uint32_t *const par = s.pool + 2 * LEAF_MAX;
+    const uint32_t NB = min(LEAF_TILE * P.tile_parents, (POOL_WORDS - 2 * LEAF_MAX) / W);
+    const uint32_t naut = pl.naut > 1 ? pl.naut : 1u;
+    unsigned long long cnt_acc = 0;
+    uint32_t cnt_local = t.count;    // shrinks when the upper parents are donated
+    uint32_t last_end = t.last_end;  // end of the last parent's driver range
+    for (uint32_t b0 = 0; b0 < cnt_local; b0 += NB) {
+        uint32_t stop = 0;
+        if (lane == 0) stop = (ld_rlx(&P.ctl->abort) || ld_rlx(&Q.status)) ? 1u : 0u;
+        if (bcast(stop, 0)) break;
+        if (wall_clock64() > deadline) {
+            if (lane == 0) timeout_at(P.ctl, 2, t.q, t.depth, t.count);
+            break;
+        }
+        const uint32_t nb = min(NB, cnt_local - b0);
+        copy_to_lds(par, P.banks + (size_t)t.bank * P.bank_words + t.begin + (size_t)b0 * W, nb * W);
+        __syncthreads();
+        // driver range of each parent (LEAF_TILE per lane) and their scan;
+        // entries past the batch hold its total
+        uint32_t total = 0;
+#pragma unroll
+        for (uint32_t r = 0; r < LEAF_TILE; r++) {
+            const uint32_t p = r * WAVE + lane;
+            uint32_t dl = 0;
+            if (p < nb) {
+                uint32_t st, len;
+                row_of(g, req, par[p * W + req_anchor(req)], st, len);
+                uint32_t endv = len;
+                if (b0 + p == cnt_local - 1) endv = min(endv, last_end);
+                const uint32_t startv = (b0 == 0 && p == 0) ? min(t.skip, endv) : 0u;
+                dstart[p] = st + startv;
+                dl = endv - startv;
+            }
+            const uint32_t incl = wave_incl_scan(dl);
+            cum[p] = total + incl;
+            total += bcast(incl, WAVE - 1);
+        }
+        uint32_t T = total;
+        __syncthreads();
+        // parent of candidate tt: the number of entries with cum <= tt, in
+        // four rounds of four independent LDS reads
+        auto locate = [&](uint32_t from, uint32_t &pi_out, uint32_t &x_out) {
+            const uint32_t tt = min(from + lane, T - 1);
+            uint32_t lo64 = 0;
+#pragma unroll
+            for (uint32_t j = 0; j < 4; j++) lo64 += cum[64 * j + 63] <= tt ? 64u : 0u;
+            uint32_t lo = lo64;
+#pragma unroll
+            for (uint32_t j = 0; j < 4; j++) lo += cum[lo64 + 16 * j + 15] <= tt ? 16u : 0u;
+            uint32_t lo4 = lo;
+#pragma unroll
+            for (uint32_t j = 0; j < 4; j++) lo4 += cum[lo + 4 * j + 3] <= tt ? 4u : 0u;
+            const uint4 c4 = *reinterpret_cast<const uint4 *>(&cum[lo4]);
+            uint32_t excl = lo4 ? cum[lo4 - 1] : 0u;
+            lo = lo4;
+            if (c4.x <= tt) { lo++; excl = c4.x; }
+            if (c4.y <= tt) { lo++; excl = c4.y; }
+            if (c4.z <= tt) { lo++; excl = c4.z; }
+            if (c4.w <= tt) { lo++; excl = c4.w; }
+            pi_out = lo;
+            x_out = g.col[dstart[lo] + (tt - excl)];
+        };
+        constexpr uint32_t LEAF_U = 8;
+        uint32_t hits = 0;
+        for (uint32_t base = 0, it = 0; base < T; base += LEAF_U * WAVE, it++) {
+            // work splitting while waves are idle (as in process_task), every
+            // 1,024 candidates
+            if ((it & 1u) == 0) {
+                uint32_t want = 0;
+                if (lane == 0) {
+                    long long idle_waves = 0;
+                    for (uint32_t h = 0; h < HUNGRY_SHARDS; h++) idle_waves += ld_rlx(&P.ctl->hungry[h].n);
+                    if (idle_waves > 0) {
+                        const unsigned long long rc = ld_rlx(&P.ctl->root_cursor);
+                        const long long roots_left =
+                            rc < P.total_root_chunks ? (long long)(P.total_root_chunks - rc) : 0;
+                        want = idle_waves > max(0, ld_rlx(&P.ctl->queued)) + roots_left ? 1u : 0u;
+                    }
+                }
+                if (bcast(want, 0)) {
+                    const uint32_t rem_par = cnt_local > b0 + nb ? cnt_local - (b0 + nb) : 0;
+                    if (rem_par >= 2 * WAVE) {
+                        // the upper half of the parents after this batch
+                        const uint32_t keep = rem_par / 2, first = b0 + nb + keep;
+                        TaskDesc r{t.q, d, t.bank, t.begin + first * W, rem_par - keep, 0, last_end};
+                        push_tasks(P, lvl_self, 1, r, t.bank, stats);
+                        cnt_local = first;
+                        last_end = ALL_CANDIDATES;
+                        if (lane == 0) stats[ST_SPLIT]++;
+                    } else if (T - base >= 8 * WAVE) {
+                        // the upper half of this batch's remaining candidates
+                        // and the parents after them
+                        const uint32_t mid = base + ((T - base) / (2 * WAVE)) * WAVE;
+                        uint32_t lo = 0, hi = nb - 1;
+                        while (lo < hi) {
+                            const uint32_t m2 = (lo + hi) >> 1;
+                            if (cum[m2] > mid) hi = m2;
+                            else lo = m2 + 1;
+                        }
+                        const uint32_t excl = lo ? cum[lo - 1] : 0u;
+                        const uint32_t abs_mid = (mid - excl) + ((b0 == 0 && lo == 0) ? t.skip : 0u);
+                        TaskDesc r{t.q, d, t.bank, t.begin + (b0 + lo) * W, cnt_local - b0 - lo, abs_mid, last_end};
+                        push_tasks(P, lvl_self, 1, r, t.bank, stats);
+                        cnt_local = b0 + lo + 1;
+                        last_end = abs_mid;
+                        T = mid;
+                        if (lane == 0) stats[ST_SPLIT]++;
+                    }
+                }
+            }
+            uint32_t xs[LEAF_U], ps[LEAF_U];
+#pragma unroll
+            for (uint32_t u = 0; u < LEAF_U; u++) locate(base + u * WAVE, ps[u], xs[u]);
+#pragma unroll
+            for (uint32_t u = 0; u < LEAF_U; u++) {
+                const uint32_t *const prow = &par[ps[u] * W];
+                bool ok = base + u * WAVE + lane < T;
+                for (uint32_t a = 0; a < W; a++) ok &= prow[a] != xs[u];
+                hits += ok ? 1u : 0u;
+            }
+        }
+        cnt_acc += wave_sum_u64(hits) * naut;
+        if (lane == 0 && cnt_acc >= (1ull << 20)) {
+            const unsigned long long now_cnt = atom_add(&Q.count, cnt_acc) + cnt_acc;
+            cnt_acc = 0;
+            if (Q.max_count && now_cnt > Q.max_count) query_fail(P, qi, FM_ERR_COUNT_LIMIT);
+        }
+        __syncthreads();
+    }
+    if (lane == 0 && cnt_acc) {
+        const unsigned long long now_cnt = atom_add(&Q.count, cnt_acc) + cnt_acc;
+        if (Q.max_count && now_cnt > Q.max_count) query_fail(P, qi, FM_ERR_COUNT_LIMIT);
+    }
+}
+
 // Extends one task by one depth. Returns after pushing children tasks; the
 // caller completes the task (bank reference and pending count). PAIRED takes
 // two candidates per lane per step (more loads in flight, more registers).
 template <bool PAIRED>
-__device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOut &o, WaveTab &wt,
+__device__ __forceinline__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOut &o, WaveTab &wt,
                              unsigned long long *stats, unsigned long long &starve_since,
                              unsigned long long deadline) {
     const uint32_t lane = lane_id();
@@ -1092,6 +1270,11 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
     const uint32_t rb = pl.req_begin[D];
     const uint32_t nself = pl.nself[D];
     const uint32_t k = pl.req_begin[D + 1] - rb - nself;  // anchored requirements
+    if (final_level && !P.table && !P.instrument && !root && k == 1 && nself == 0 &&
+        pl.sc_begin[D] == pl.sc_begin[D + 1]) {
+        process_leaf_task(P, t, s, stats, deadline);
+        return;
+    }
 
     // batch layout in the LDS pool
     const uint32_t NB = min(P.tile_parents, POOL_WORDS / (W + 2 * k));
@@ -1123,8 +1306,7 @@ __device__ void process_task(const KParams &P, const TaskDesc &t, Lds &s, WaveOu
                 par[lane] = pl.root_base == ALL_VERTICES ? idx : g.roots[pl.root_base + idx];
             }
         } else {
-            const uint32_t *src = P.banks + (size_t)t.bank * P.bank_words + t.begin + (size_t)b0 * W;
-            for (uint32_t w = lane; w < nb * W; w += WAVE) par[w] = src[w];
+            copy_to_lds(par, P.banks + (size_t)t.bank * P.bank_words + t.begin + (size_t)b0 * W, nb * W);
         }
         __syncthreads();
         // ---- requirement rows per parent; driver = shortest ----
@@ -1518,7 +1700,7 @@ __device__ __forceinline__ void st_sys(uint32_t *p, uint32_t v) {
 
 // Lane 0: publishes a finished query to the host. Every task of the query
 // released its rows before its pending decrement, which this wave acquired.
-__device__ void complete_query(const KParams &P, uint32_t q) {
+__device__ __forceinline__ void complete_query(const KParams &P, uint32_t q) {
     DQuery &Q = P.qs[q];
     SrvDone *d = &P.srv_done[q];
     const unsigned long long count = ld_rlx(&Q.count);
@@ -1539,7 +1721,7 @@ __device__ void complete_query(const KParams &P, uint32_t q) {
 }
 
 // Lane 0: claims the next submission; returns its slot or NO_PAGE.
-__device__ uint32_t claim_submission(const KParams &P) {
+__device__ __forceinline__ uint32_t claim_submission(const KParams &P) {
     Ctl *c = P.ctl;
     // The two relaxed loads may complete in either order, so the tail read
     // can be older than the head: the head may already be past it. Only a
@@ -1561,7 +1743,7 @@ __device__ uint32_t claim_submission(const KParams &P) {
 // All lanes: admits the query in `slot`: recycles the pages of the slot's
 // previous query, copies the plan into device memory, initialises the
 // query state and queues its root chunks.
-__device__ void admit(const KParams &P, uint32_t slot, unsigned long long *stats) {
+__device__ __forceinline__ void admit(const KParams &P, uint32_t slot, unsigned long long *stats) {
     const uint32_t lane = lane_id();
     DQuery &Q = P.qs[slot];
     Ctl *c = P.ctl;

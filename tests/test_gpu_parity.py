@@ -462,6 +462,51 @@ def test_hub_rows_are_split_across_waves(engine):
     assert split > 0
 
 
+def test_count_only_leaf_level_equals_oracle(engine):
+    """Count-only queries whose last vertex is a leaf of the matching order
+    run their last level in batches of up to 256 parents: counts equal the
+    oracle's for trees and one-chord walks on a skewed R-MAT graph, for LDS
+    tiles of 1, 5 and 64 parents (leaf batches of 4, 20 and 256), and for
+    star queries on a hub graph whose long driver rows are split between
+    waves (splits asserted)."""
+    g = fm.Graph.rmat(11, 12000, 12, zipf_s=1.2, seed=21).upload()
+    og = _oracle_graph(g)
+    qs = [fm.Query.random_walk(g, n, shape=shape, seed=seed)
+          for n, shape, seed in [(3, 0, 1), (5, 0, 401), (6, 0, 24), (7, 0, 33), (6, 3, 21), (7, 3, 24), (8, 3, 24),
+                                 (5, 3, 20)]]
+    refs = [orc.match(og, q.vertices, q.edges(), table=False, threads=4).count for q in qs]
+    assert sum(1 for c in refs if c > 10000) >= 3
+    for tp in (1, 5, 64):
+        for q, ref in zip(qs, refs):
+            r = engine.match(g, q, table=False, tile_parents=tp, timeout_ms=TIMEOUT_MS)
+            assert r.query(0).status == 0 and r.count(0) == ref, (tp, q.vertices)
+            r.close()
+        rb = engine.match_batch(g, qs, table=False, tile_parents=tp, timeout_ms=TIMEOUT_MS)
+        assert [rb.count(i) for i in range(len(qs))] == refs, tp
+        rb.close()
+    # stars on a hub joined to 3000 leaves (a ring among the leaves): a star
+    # with k leaves has sum over v of deg(v) (deg(v) - 1) ... (deg(v) - k + 1)
+    # embeddings (hub degree 3000, leaf degree 3)
+    V = 3001
+    leaves = np.arange(1, V, dtype=np.uint32)
+    s = np.concatenate([np.zeros(V - 1, np.uint32), leaves, leaves, np.roll(leaves, 1)])
+    d = np.concatenate([leaves, np.zeros(V - 1, np.uint32), np.roll(leaves, 1), leaves])
+    lab = np.zeros(len(s), np.uint32)
+    hg = fm.Graph.from_edges(V, s, d, lab).upload()
+    split = 0
+    for n in (3, 4):
+        e = [x for v in range(1, n) for x in ((0, v, 0), (v, 0, 0))]
+        q = fm.Query.from_edges(n, e)
+        k = n - 1
+        ref = int(np.prod(np.arange(3000 - k + 1, 3001, dtype=np.int64))) + 3000 * int(np.prod(np.arange(3 - k + 1, 4)))
+        if n == 3:
+            assert ref == orc.match(orc.OracleGraph(V, s, d, lab), n, q.edges(), table=False).count
+        r = engine.match(hg, q, table=False, task_rows=64, timeout_ms=TIMEOUT_MS)
+        split += r.stats().tasks_split
+        assert r.count(0) == ref, n
+    assert split > 0
+
+
 @pytest.mark.parametrize("slice_ms,plan", [(0, fm.PLAN_COST), (1, fm.PLAN_REFERENCE)])
 def test_serving_results_equal_alone_and_in_the_mix(er, slice_ms, plan):
     """Continuous batching: queries submitted while others run are admitted
