@@ -198,6 +198,7 @@ class Query:
 
     def __init__(self, handle: C.c_void_p):
         self._h = handle
+        self.count_only = False
 
     @classmethod
     def load_g(cls, path: str) -> "Query":
@@ -272,6 +273,13 @@ class Query:
 
     def write_g(self, path: str) -> None:
         _check(L.lib().fm_query_write_g(self._h, str(path).encode()))
+
+    def set_count_only(self, count_only: bool = True) -> "Query":
+        """In a table-mode launch, count this query without storing its
+        rows (its table stays empty)."""
+        _check(L.lib().fm_query_set_count_only(self._h, 1 if count_only else 0))
+        self.count_only = bool(count_only)
+        return self
 
     def close(self) -> None:
         if self._h:

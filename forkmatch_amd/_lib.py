@@ -102,6 +102,7 @@ SIGNATURES = {
     "fm_query_plan": (C.c_int, [vp, P(PlanInfo), P(u32), P(u32), P(u32), P(u32), P(u32)]),
     "fm_query_plan_on": (C.c_int, [vp, vp, u32, P(PlanInfo), P(u32), P(u32), P(u32), P(u32), P(u32)]),
     "fm_query_symmetry": (C.c_int, [vp, u32, P(u32), P(u32), u32, P(u32), P(u32), u32]),
+    "fm_query_set_count_only": (C.c_int, [vp, u32]),
     "fm_query_destroy": (None, [vp]),
     "fm_engine_create": (C.c_int, [P(EngineConfig), P(vp)]),
     "fm_engine_destroy": (None, [vp]),

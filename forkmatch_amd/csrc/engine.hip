@@ -102,6 +102,7 @@ struct DQuery {
     uint32_t npages;  // pages claimed
     uint32_t root_chunk_begin;
     uint32_t chunk_roots;  // roots per root claim of this query
+    uint32_t count_only;   // 1: counted without rows in a table-mode launch
     uint32_t pad;
 };
 
@@ -326,7 +327,7 @@ __device__ __forceinline__ bool cas_u32(uint32_t *p, uint32_t expect, uint32_t d
 __device__ __forceinline__ void backoff() { __builtin_amdgcn_s_sleep(2); }
 
 struct Ctl;
-__device__ bool spin_ok(const Ctl *c, uint32_t &iter, uint32_t site, uint32_t a, uint32_t b);
+__device__ __forceinline__ bool spin_ok(const Ctl *c, uint32_t &iter, uint32_t site, uint32_t a, uint32_t b);
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
@@ -527,7 +528,7 @@ __device__ __forceinline__ void timeout_at(const Ctl *c, uint32_t site, uint32_t
     __hip_atomic_store(&w->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ bool spin_ok(const Ctl *c, uint32_t &iter, uint32_t site, uint32_t a, uint32_t b) {
+__device__ __forceinline__ bool spin_ok(const Ctl *c, uint32_t &iter, uint32_t site, uint32_t a, uint32_t b) {
     backoff();
     if ((++iter & 255u) != 0) return true;
     if (__hip_atomic_load(&c->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
@@ -543,7 +544,7 @@ __device__ bool spin_ok(const Ctl *c, uint32_t &iter, uint32_t site, uint32_t a,
 // Waits until *p == want, polling with relaxed loads (no cache invalidation
 // per poll) and acquiring once at the end. false: the launch is aborting.
 // `site` names the wait in a timeout report.
-__device__ bool wait_eq(const uint32_t *p, uint32_t want, const Ctl *c, uint32_t site) {
+__device__ __forceinline__ bool wait_eq(const uint32_t *p, uint32_t want, const Ctl *c, uint32_t site) {
     uint32_t it = 0;
     uint32_t v;
     while ((v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) != want)
@@ -552,14 +553,14 @@ __device__ bool wait_eq(const uint32_t *p, uint32_t want, const Ctl *c, uint32_t
     return true;
 }
 
-__device__ void raise_error(const KParams &P, uint32_t code) {
+__device__ __forceinline__ void raise_error(const KParams &P, uint32_t code) {
     uint32_t zero = 0;
     __hip_atomic_compare_exchange_strong(&P.ctl->error, &zero, code, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                          __HIP_MEMORY_SCOPE_AGENT);
     st_rlx(&P.ctl->abort, 1u);
 }
 
-__device__ void query_fail(const KParams &P, uint32_t q, uint32_t code) {
+__device__ __forceinline__ void query_fail(const KParams &P, uint32_t q, uint32_t code) {
     uint32_t zero = 0;
     __hip_atomic_compare_exchange_strong(&P.qs[q].status, &zero, code, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                          __HIP_MEMORY_SCOPE_AGENT);
@@ -567,7 +568,7 @@ __device__ void query_fail(const KParams &P, uint32_t q, uint32_t code) {
 
 // ---- bank pool (lane 0 only) ----------------------------------------------
 
-__device__ uint32_t bank_alloc(const KParams &P, unsigned long long *stats) {
+__device__ __forceinline__ uint32_t bank_alloc(const KParams &P, unsigned long long *stats) {
     Ctl *c = P.ctl;
     if (ld_rlx(&c->f_avail) > 0) {
         int32_t a = atom_add(&c->f_avail, -1);
@@ -631,7 +632,7 @@ struct TaskDesc {
 
 // Pushes `m` tasks (lane i < m describes task i) at priority level lvl.
 // All lanes call. Row data the tasks reference must be written before.
-__device__ void push_tasks(const KParams &P, uint32_t lvl, uint32_t m, const TaskDesc &t, uint32_t bank,
+__device__ __forceinline__ void push_tasks(const KParams &P, uint32_t lvl, uint32_t m, const TaskDesc &t, uint32_t bank,
                            unsigned long long *stats) {
     if (m == 0) return;
     const uint32_t lane = lane_id();
@@ -764,7 +765,7 @@ __device__ __forceinline__ void tab_release(const KParams &P, WaveTab &w) {
 
 // Lane 0 only: publishes the rows appended since the last sync (page fill
 // and the query's row count) and keeps the page.
-__device__ void tab_sync(const KParams &P, WaveTab &w) {
+__device__ __forceinline__ void tab_sync(const KParams &P, WaveTab &w) {
     if (w.page == NO_PAGE || w.fill == w.fill0) return;
     st_rlx(&P.page_fill[w.page], w.fill);
     atom_add(&P.qs[w.q].rows, (unsigned long long)(w.fill - w.fill0));
@@ -775,13 +776,13 @@ __device__ void tab_sync(const KParams &P, WaveTab &w) {
 // for the wave's next task of the same query, so a query uses about
 // rows / page_rows + waves pages however small its tasks are. Serving
 // tables are dense (emit_rows_dense) and hold no page in the wave.
-__device__ void tab_task_end(const KParams &P, WaveTab &w) {
+__device__ __forceinline__ void tab_task_end(const KParams &P, WaveTab &w) {
     if (!P.serving) tab_sync(P, w);
 }
 
 // Lane 0 only. Takes the query's spare page (if allowed) or claims a fresh
 // one; false when the query's page budget or the arena is exhausted.
-__device__ bool tab_acquire(const KParams &P, WaveTab &w, uint32_t q, uint32_t cap, bool allow_spare,
+__device__ __forceinline__ bool tab_acquire(const KParams &P, WaveTab &w, uint32_t q, uint32_t cap, bool allow_spare,
                             unsigned long long *stats) {
     DQuery &Q = P.qs[q];
     w.q = q;
@@ -808,7 +809,7 @@ __device__ bool tab_acquire(const KParams &P, WaveTab &w, uint32_t q, uint32_t c
 
 // Lane 0, serving: a page of the arena, a recycled one first; NO_PAGE when
 // the arena is exhausted.
-__device__ uint32_t srv_page_alloc(const KParams &P, unsigned long long *stats) {
+__device__ __forceinline__ uint32_t srv_page_alloc(const KParams &P, unsigned long long *stats) {
     Ctl *c = P.ctl;
     uint32_t id = NO_PAGE;
     if (ld_rlx(&c->pf_avail) > 0) {
@@ -832,7 +833,7 @@ __device__ uint32_t srv_page_alloc(const KParams &P, unsigned long long *stats) 
 
 // Lane 0, serving: the page of slot entry k once its owner has resolved it
 // (a page id or FAILED_PAGE).
-__device__ uint32_t wait_page(const KParams &P, uint32_t q, uint32_t k) {
+__device__ __forceinline__ uint32_t wait_page(const KParams &P, uint32_t q, uint32_t k) {
     const uint32_t *e = &P.slot_pages[(size_t)q * P.max_slot_pages + k];
     uint32_t it = 0, v;
     while ((v = __hip_atomic_load(e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) == NO_PAGE)
@@ -848,7 +849,7 @@ __device__ uint32_t wait_page(const KParams &P, uint32_t q, uint32_t k) {
 // slot entry. When the arena runs out the query's table ends at the start
 // of the first page that failed (trunc_at): every row below it is stored.
 template <class F>
-__device__ void emit_rows_dense(const KParams &P, uint32_t q, uint32_t n, bool ok, uint32_t rank, uint32_t cnt,
+__device__ __forceinline__ void emit_rows_dense(const KParams &P, uint32_t q, uint32_t n, bool ok, uint32_t rank, uint32_t cnt,
                                 unsigned long long *stats, F write_row) {
     const uint32_t lane = lane_id();
     DQuery &Q = P.qs[q];
@@ -906,7 +907,7 @@ __device__ void emit_rows_dense(const KParams &P, uint32_t q, uint32_t n, bool o
 // All lanes. Appends the rows of the lanes with ok set (rank = their order)
 // to query q's table; write_row(dst) stores one row of n words at dst.
 template <class F>
-__device__ void emit_rows(const KParams &P, WaveTab &wt, uint32_t q, uint32_t n, bool ok, uint32_t rank, uint32_t cnt,
+__device__ __forceinline__ void emit_rows(const KParams &P, WaveTab &wt, uint32_t q, uint32_t n, bool ok, uint32_t rank, uint32_t cnt,
                           unsigned long long *stats, F write_row) {
     if (P.serving) {
         emit_rows_dense(P, q, n, ok, rank, cnt, stats, write_row);
@@ -995,7 +996,7 @@ struct Lds {
 
 // Pushes the rows written since run_begin as tasks of at most `per`
 // parents each (depth D, row width D + 1).
-__device__ void flush_run(const KParams &P, WaveOut &o, uint32_t q, uint32_t D, uint32_t lvl, uint32_t per,
+__device__ __forceinline__ void flush_run(const KParams &P, WaveOut &o, uint32_t q, uint32_t D, uint32_t lvl, uint32_t per,
                           unsigned long long *stats) {
     const uint32_t W = D + 1;
     uint32_t rows = (o.used - o.run_begin) / W;
@@ -1199,6 +1200,7 @@ __device__ __forceinline__ void process_task(const KParams &P, const TaskDesc &t
     const DPlan &pl = P.plans[qi];
     DQuery &Q = P.qs[qi];
     if (ld_rlx(&Q.status) != 0) return;
+    const bool table = P.table && !Q.count_only;  // this query stores rows
     const uint32_t n = pl.n;
     const uint32_t d = t.depth;
     const uint32_t W = d + 1;  // parent width
@@ -1253,12 +1255,12 @@ __device__ __forceinline__ void process_task(const KParams &P, const TaskDesc &t
             uint32_t rank = __popcll(mask & ((1ull << lane) - 1ull));
             cnt_acc += cnt;
             if (P.instrument && lane == 0) s.inst[4 + 0] += cnt;
-            if (P.table && cnt)
+            if (table && cnt)
                 emit_rows(P, wt, qi, 1, ok, rank, cnt, stats, [&](uint32_t *dst) { dst[0] = v; });
         }
         if (lane == 0) {
             atom_add(&Q.count, cnt_acc);
-            if (P.table) tab_task_end(P, wt);
+            if (table) tab_task_end(P, wt);
         }
         return;
     }
@@ -1270,7 +1272,7 @@ __device__ __forceinline__ void process_task(const KParams &P, const TaskDesc &t
     const uint32_t rb = pl.req_begin[D];
     const uint32_t nself = pl.nself[D];
     const uint32_t k = pl.req_begin[D + 1] - rb - nself;  // anchored requirements
-    if (final_level && !P.table && !P.instrument && !root && k == 1 && nself == 0 &&
+    if (final_level && !table && !P.instrument && !root && k == 1 && nself == 0 &&
         pl.sc_begin[D] == pl.sc_begin[D + 1]) {
         process_leaf_task(P, t, s, stats, deadline);
         return;
@@ -1472,7 +1474,7 @@ __device__ __forceinline__ void process_task(const KParams &P, const TaskDesc &t
         // general loop below has one step prefetched. Every candidate is
         // still loaded and checked one by one.
         const bool leaf_count =
-            final_level && !P.table && k == 1 && nself == 0 && pl.sc_begin[D] == pl.sc_begin[D + 1];
+            final_level && !table && k == 1 && nself == 0 && pl.sc_begin[D] == pl.sc_begin[D + 1];
         if (leaf_count) {
             constexpr uint32_t LEAF_U = 8;
             uint32_t hits = 0;
@@ -1629,7 +1631,7 @@ __device__ __forceinline__ void process_task(const KParams &P, const TaskDesc &t
             if (final_level) {
                 const uint32_t naut = pl.naut > 1 ? pl.naut : 1u;
                 cnt_acc += (unsigned long long)(cna + cnb) * naut;
-                if (P.table) {
+                if (table) {
                     // all lanes call (wave-uniform conditions)
                     auto emit = [&](bool ok, uint32_t rank, uint32_t cnt, const uint32_t *prow, uint32_t x) {
                         if (naut == 1) {
@@ -1682,7 +1684,7 @@ __device__ __forceinline__ void process_task(const KParams &P, const TaskDesc &t
     }
     if (!final_level && o.bank != NO_BANK) flush_run(P, o, t.q, D, lvl_children, run_rows, stats);
     // the query's stored-row count is complete when its last task finishes
-    if (final_level && P.table && lane == 0) tab_task_end(P, wt);
+    if (final_level && table && lane == 0) tab_task_end(P, wt);
     if (lane == 0 && cnt_acc) {
         unsigned long long now_cnt = atom_add(&Q.count, cnt_acc) + cnt_acc;
         if (Q.max_count && now_cnt > Q.max_count) query_fail(P, qi, FM_ERR_COUNT_LIMIT);
@@ -3327,11 +3329,13 @@ int fm_match_batch(fm_engine *e, const fm_graph *g, const fm_query *const *qs, u
         maxn = std::max(maxn, n);
         Q.page_rows = (uint32_t)(e->page_words / n);
         uint64_t cap_rows = (uint64_t)e->npages * Q.page_rows;
-        Q.max_rows = (o.mode == FM_MODE_TABLE) ? (o.max_rows ? std::min<uint64_t>(o.max_rows, cap_rows) : cap_rows) : 0;
+        const bool qtable = o.mode == FM_MODE_TABLE && !qs[i]->count_only;
+        Q.count_only = (o.mode == FM_MODE_TABLE && qs[i]->count_only) ? 1u : 0u;
+        Q.max_rows = qtable ? (o.max_rows ? std::min<uint64_t>(o.max_rows, cap_rows) : cap_rows) : 0;
         // pages are filled by several waves at once, so a query may hold one
         // partly filled page per wave beyond the full ones
         uint64_t want_pages = (Q.max_rows + Q.page_rows - 1) / Q.page_rows + (o.max_rows ? grid : 0);
-        Q.max_pages = (o.mode == FM_MODE_TABLE) ? (uint32_t)std::min<uint64_t>(want_pages, e->npages) : 0;
+        Q.max_pages = qtable ? (uint32_t)std::min<uint64_t>(want_pages, e->npages) : 0;
         Q.max_count = o.max_count;
         Q.max_partials = o.max_partials;
         // roots per claim: root_chunk, or the query's roots cut into
@@ -3450,7 +3454,8 @@ int fm_match_batch(fm_engine *e, const fm_graph *g, const fm_query *const *qs, u
     }
     for (auto &Q : r->qs) {
         if (Q.status == FM_ERR_COUNT_LIMIT) Q.flags |= FM_RESULT_ABORTED;
-        if (o.mode == FM_MODE_TABLE && Q.count > std::min<uint64_t>(Q.rows, Q.max_rows)) Q.flags |= FM_RESULT_TRUNCATED;
+        if (o.mode == FM_MODE_TABLE && !Q.count_only && Q.count > std::min<uint64_t>(Q.rows, Q.max_rows))
+            Q.flags |= FM_RESULT_TRUNCATED;
     }
     st.total_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();

@@ -148,6 +148,7 @@ struct fm_graph {
 
 struct fm_query {
     fm::HostQuery h;
+    uint32_t count_only = 0;  // counted without rows in a table-mode launch
 };
 
 namespace fm {

@@ -1467,6 +1467,12 @@ int fm_gen_query_walk(const fm_graph *g, uint32_t vertices, int shape, uint32_t 
     return FM_OK;
 }
 
+int fm_query_set_count_only(fm_query *q, uint32_t count_only) {
+    if (!q) return fail(FM_ERR_INVALID_ARG, "null argument");
+    q->count_only = count_only ? 1u : 0u;
+    return FM_OK;
+}
+
 int fm_query_load_g(const char *path, fm_query **out) {
     if (!path || !out) return fail(FM_ERR_INVALID_ARG, "null argument");
     std::string buf;

@@ -208,6 +208,13 @@ int fm_query_plan_on(const fm_graph *g, const fm_query *q, uint32_t plan,
                      fm_plan_info *info, uint32_t *order, uint32_t *req_begin,
                      uint32_t *req_anchor, uint32_t *req_label,
                      uint32_t *req_dir);
+/* In a table-mode launch (fm_match, fm_match_batch), count q's embeddings
+ * without storing its rows (count_only = 1), as a count-mode launch does:
+ * the reference's WriteSolnFlag = false (src/environment.cuh:65) for one
+ * query of a batch, so that queries with too many embeddings for a table
+ * run beside queries with tables. Its result has rows = 0 and no
+ * FM_RESULT_TRUNCATED flag. No effect in a count-mode launch or a server. */
+int fm_query_set_count_only(fm_query *q, uint32_t count_only);
 void fm_query_destroy(fm_query *q);
 
 /* ---- engine -------------------------------------------------------------

@@ -462,6 +462,35 @@ def test_hub_rows_are_split_across_waves(engine):
     assert split > 0
 
 
+def test_count_only_queries_in_a_table_batch(engine, er):
+    """Queries marked count-only in a table-mode batch are counted as in a
+    count-mode launch (rows 0, not truncated), while the others in the same
+    launch get their full tables: every count and table equals the
+    oracle's, alone and batched."""
+    g, og = er
+    qs = [fm.Query.random_walk(g, 3 + i % 5, shape=i % 3, min_cycles=1 if i % 3 == 1 else 0, seed=700 + i)
+          for i in range(12)]
+    refs = [orc.match(og, q.vertices, q.edges()) for q in qs]
+    for i, q in enumerate(qs):
+        q.set_count_only(i % 2 == 0)
+    rb = engine.match_batch(g, qs, timeout_ms=TIMEOUT_MS)
+    for i, ref in enumerate(refs):
+        out = rb.query(i)
+        assert out.status == 0 and out.count == ref.count, i
+        if i % 2 == 0:
+            assert out.rows == 0 and not out.truncated, i
+        else:
+            assert out.rows == ref.count and np.array_equal(rb.table(i), ref.table), i
+    rb.close()
+    r = engine.match(g, qs[0], timeout_ms=TIMEOUT_MS)
+    assert r.count(0) == refs[0].count and r.query(0).rows == 0
+    r.close()
+    qs[0].set_count_only(False)
+    r = engine.match(g, qs[0], timeout_ms=TIMEOUT_MS)
+    assert r.query(0).rows == refs[0].count and np.array_equal(r.table(0), refs[0].table)
+    r.close()
+
+
 def test_count_only_leaf_level_equals_oracle(engine):
     """Count-only queries whose last vertex is a leaf of the matching order
     run their last level in batches of up to 256 parents: counts equal the
