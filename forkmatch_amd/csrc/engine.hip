@@ -2042,7 +2042,7 @@ __global__ void fm_init_slots_kernel(Slot *ring, uint32_t qcap, uint32_t levels,
 //  - fm_isect_kernel: persistent 256-thread blocks claim whole pairs in that
 //    order and stream their tiles: a merge tile is staged in LDS (plus the
 //    next B element, so a match on the tile's last A element is seen), each
-//    thread merges a 15-element stretch of the path, a block scan orders the
+//    thread merges a 19-element stretch of the path, a block scan orders the
 //    matches, and they leave through LDS in coalesced stores. Every list
 //    element is read from HBM once.
 //
@@ -2054,8 +2054,15 @@ constexpr uint32_t IB = 256;              // threads per block
 // Merge-path elements per thread. Odd: the threads' stretches start about
 // XVT / 2 words apart in each LDS slice, and an odd stride spreads a wave's
 // accesses over all LDS banks (an even one folds them onto a few).
+// On 2,048 pairs of 32k-element lists (total time of the three kernels):
+// 19 elements at 4 waves per SIMD 217 us; 15 at 5 waves 230 us, 17 222-225,
+// 21 at 4 waves 220, 11 and 13 255-260, 21 and 25 at 3 waves 246-256.
 #ifndef FM_ISECT_VT
-#define FM_ISECT_VT 15
+#define FM_ISECT_VT 19
+#endif
+// waves per SIMD the main kernel is compiled for (its VGPR budget)
+#ifndef FM_ISECT_WAVES
+#define FM_ISECT_WAVES 4
 #endif
 constexpr uint32_t XVT = FM_ISECT_VT;
 constexpr uint32_t XT = IB * XVT;         // merge-path elements per merge tile
@@ -2263,7 +2270,7 @@ __global__ __launch_bounds__(IB) void fm_isect_cut_kernel(const uint32_t *__rest
 // prefetch register makes the compiler wait for the loads at the loop join,
 // exposing their latency on every tile). The matches leave through LDS in
 // coalesced stores at the pair's running offset.
-__global__ __launch_bounds__(IB) __attribute__((amdgpu_waves_per_eu(5, 5))) void fm_isect_kernel(const uint32_t *__restrict__ data, const uint64_t *a_off,
+__global__ __launch_bounds__(IB) __attribute__((amdgpu_waves_per_eu(FM_ISECT_WAVES, FM_ISECT_WAVES))) void fm_isect_kernel(const uint32_t *__restrict__ data, const uint64_t *a_off,
                                                       const uint32_t *a_len, const uint64_t *b_off,
                                                       const uint32_t *b_len, uint32_t *counts,
                                                       const uint64_t *out_off, uint32_t *out, uint32_t npairs,
@@ -4020,8 +4027,8 @@ static int isect_run(fm_engine *e, uint32_t *work, uint32_t cap, const uint32_t 
     // Blocks take whole pairs, largest first. With more pairs than resident
     // blocks, every block takes the same number of pairs (rounds), so that a
     // large pair is followed by a small one: on 2,048 pairs of 32k-96k
-    // elements, 1,024 blocks end within 2% of each other where 1,280 end
-    // 17% apart.
+    // elements (3,840-element tiles), 1,024 blocks end within 2% of each
+    // other where 1,280 end 17% apart.
     const uint64_t resident = std::max<uint64_t>(1, (uint64_t)blocks * e->cu_count);
     const uint64_t rounds = (npairs + resident - 1) / resident;
     const uint32_t grid = (uint32_t)((npairs + rounds - 1) / rounds);

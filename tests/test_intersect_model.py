@@ -1,11 +1,11 @@
 """CPU model of the tiled intersection kernels (engine.hip:
 fm_isect_cut_kernel, fm_isect_kernel): the host tile plan (merge tiles of
-3840 merge-path elements, skew tiles of 256 elements of the shorter list when
+4864 merge-path elements, skew tiles of 256 elements of the shorter list when
 the longer one is >= 32x longer), each tile's cut, the tile's LDS image (the
 A slice from word 0 and the B slice from the next 256-word chunk, loaded a
-chunk of one list per register at clamped indices), the per-thread 15-element
+chunk of one list per register at clamped indices), the per-thread 19-element
 stretches found by binary search in the image, the branch-free steps that
-every thread runs to the full 15 (past its stretch only B steps, which match
+every thread runs to the full 19 (past its stretch only B steps, which match
 nothing), the match check against the next B element (staged one past the
 tile), and the output offset of each tile as the sum of the counts of the
 pair's earlier tiles (the running offset of the block that streams the pair).
@@ -14,7 +14,7 @@ straddling tile and thread boundaries."""
 import numpy as np
 import pytest
 
-IB, XVT, ISKEW = 256, 15, 32
+IB, XVT, ISKEW = 256, 19, 32
 XT = IB * XVT
 
 
@@ -170,7 +170,7 @@ def test_size_classes_are_ordered():
 def test_equal_pair_counts_per_block_balance_the_blocks():
     # the host's grid rule (every block takes the same number of pairs)
     # with blocks taking pairs largest first in size-class order, on the
-    # benchmark's pair sizes: the busiest block ends within 3% of the mean
+    # benchmark's pair sizes: the busiest block ends within 5% of the mean
     import heapq
     rng = np.random.default_rng(7)
     nt = -(-(rng.integers(16384, 49152, 2048) + rng.integers(16384, 49152, 2048)) // XT)
@@ -183,7 +183,7 @@ def test_equal_pair_counts_per_block_balance_the_blocks():
     for p in order:
         x, i = heapq.heappop(h)
         heapq.heappush(h, (x + int(nt[p]), i))
-    assert max(x for x, _ in h) < 1.03 * nt.sum() / grid
+    assert max(x for x, _ in h) < 1.05 * nt.sum() / grid
 
 
 def device_plan(lens, cap):
