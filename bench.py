@@ -313,15 +313,15 @@ def intersect_bench(engine, pairs: int = 2048, mean_len: int = 32768, reps: int 
         ms.append(r["kernel_ms"])
     nbytes = 4 * (int(la.sum()) + int(lb.sum())) + 4 * inter
     gbs = nbytes / (np.mean(ms) / 1e3) / 1e9
-    out = {"kernel": "fm_isect_plan_kernel + fm_isect_kernel", "pairs": pairs, "elements": int(lens.sum()),
+    out = {"kernel": "fm_isx_plan_kernel + fm_isx_kernel", "pairs": pairs, "elements": int(lens.sum()),
            "bytes": nbytes, "kernel_ms": float(np.mean(ms)), "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
            "frac": gbs / HBM_PEAK_GBS, "checked": bool(ok), "traffic": None}
     # HBM bytes per launch of both kernels, measured with rocprofv3 FETCH_SIZE /
     # WRITE_SIZE on this workload and calibrated against a copy of known size
-    tpath = ROOT / "profiles" / "fm_isect_kernel_intersect_2048pairs_32k_traffic.json"
+    tpath = ROOT / "profiles" / "fm_isx_kernel_intersect_2048pairs_32k_traffic.json"
     if tpath.exists() and pairs == 2048 and mean_len == 32768 and seed == 7:
         tr = json.loads(tpath.read_text())
-        if tr.get("algorithmic_bytes_per_launch") == nbytes and tr.get("kernel") == "fm_isect_kernel":
+        if tr.get("algorithmic_bytes_per_launch") == nbytes and tr.get("kernel") == "fm_isx_kernel":
             out["traffic"] = tr["hbm_bytes_per_launch"]
             out["traffic_source"] = str(tpath.relative_to(ROOT))
     return out

@@ -46,14 +46,17 @@ def build_library() -> Path:
     host_obj = objdir / "host.o"
     engine_obj = objdir / "engine.o"
     gbuild_obj = objdir / "gbuild.o"
+    isect_obj = objdir / "isect.o"
     _run([HIPCC, "-x", "c++", *common, "-c", str(CSRC / "host.cpp"), "-o", str(host_obj)])
     _run([HIPCC, f"--offload-arch={ARCH}", *common, *DEVICE_FLAGS, "-c", str(CSRC / "engine.hip"),
           "-o", str(engine_obj)])
     _run([HIPCC, f"--offload-arch={ARCH}", *common, *DEVICE_FLAGS, "-c", str(CSRC / "gbuild.hip"),
           "-o", str(gbuild_obj)])
+    _run([HIPCC, f"--offload-arch={ARCH}", *common, *DEVICE_FLAGS, "-c", str(CSRC / "isect.hip"),
+          "-o", str(isect_obj)])
     tmp = LIB.with_suffix(".so.tmp")
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), str(host_obj),
-          str(engine_obj), str(gbuild_obj), "-lpthread"])
+          str(engine_obj), str(gbuild_obj), str(isect_obj), "-lpthread"])
     os.replace(tmp, LIB)
     tmp_cli = CLI.with_suffix(".tmp")
     _run([HIPCC, "-x", "c++", *common, str(CSRC / "cli.cpp"), "-o", str(tmp_cli), f"-L{PKG}", "-lforkmatch",

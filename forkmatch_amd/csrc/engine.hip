@@ -35,6 +35,7 @@
 #include <vector>
 
 #include "fm_internal.hpp"
+#include "isect.hpp"
 
 namespace fm {
 
@@ -2024,80 +2025,8 @@ __global__ void fm_init_slots_kernel(Slot *ring, uint32_t qcap, uint32_t levels,
 }
 
 // ---------------------------------------------------------------------------
-// Intersection microkernel: |A n B| and the ordered intersection for a batch
-// of pairs of sorted u32 lists (A = the shorter list of a pair).
-//
-// Each pair is cut into tiles by its list lengths:
-//  - merge tiles (lengths within 32x): tile k covers merge-path diagonals
-//    [k*XT, (k+1)*XT) of merge(A, B);
-//  - skew tiles (longer >= 32x shorter): tile k covers A[k*IB, (k+1)*IB) and
-//    the range of B between its first and last value.
-// The plan is made on the device, so a call needs no host copy of the
-// lengths and no memory beyond one work array (`work`):
-//  - fm_isect_plan_kernel (one block): every pair's first tile (a scan of
-//    the tile counts), the pairs with tiles ordered largest first, counts
-//    and the claim counter cleared;
-//  - fm_isect_cut_kernel: one block per pair finds the cuts of its tiles
-//    (one merge-path or range search per tile, all in parallel);
-//  - fm_isect_kernel: persistent 256-thread blocks claim whole pairs in that
-//    order and stream their tiles: a merge tile is staged in LDS (plus the
-//    next B element, so a match on the tile's last A element is seen), each
-//    thread merges a 19-element stretch of the path, a block scan orders the
-//    matches, and they leave through LDS in coalesced stores. Every list
-//    element is read from HBM once.
-//
-// work (u32 words): [0] claim counter, [1] pairs with tiles, [2] tiles in
-// all, [3] set when the tiles exceed the array, then from word ISECT_HDR:
-// order[npairs], tile_begin[npairs + 1], cut[cap], cut2[cap].
-constexpr uint32_t ISECT_HDR = 64;
-constexpr uint32_t IB = 256;              // threads per block
-// Merge-path elements per thread. Odd: the threads' stretches start about
-// XVT / 2 words apart in each LDS slice, and an odd stride spreads a wave's
-// accesses over all LDS banks (an even one folds them onto a few).
-// On 2,048 pairs of 32k-element lists (total time of the three kernels):
-// 19 elements at 4 waves per SIMD 217 us; 15 at 5 waves 230 us, 17 222-225,
-// 21 at 4 waves 220, 11 and 13 255-260, 21 and 25 at 3 waves 246-256.
-#ifndef FM_ISECT_VT
-#define FM_ISECT_VT 19
-#endif
-// waves per SIMD the main kernel is compiled for (its VGPR budget)
-#ifndef FM_ISECT_WAVES
-#define FM_ISECT_WAVES 4
-#endif
-constexpr uint32_t XVT = FM_ISECT_VT;
-constexpr uint32_t XT = IB * XVT;         // merge-path elements per merge tile
-constexpr uint32_t ISKEW = 32;            // longer >= ISKEW x shorter: skew tiles
-
-// Number of A elements among the first `diag` elements of merge(A, B), with
-// A taken first on ties.
-__device__ __forceinline__ uint32_t merge_split(const uint32_t *A, uint32_t na, const uint32_t *B, uint32_t nb,
-                                                uint32_t diag) {
-    uint32_t lo = diag > nb ? diag - nb : 0, hi = min(diag, na);
-    while (lo < hi) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (A[mid] <= B[diag - 1 - mid]) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
-}
-
-__device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t *B, uint32_t lo, uint32_t hi, uint32_t v) {
-    while (lo < hi) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (B[mid] < v) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
-}
-
-__device__ __forceinline__ uint32_t upper_bound_u32(const uint32_t *B, uint32_t lo, uint32_t hi, uint32_t v) {
-    while (lo < hi) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (B[mid] <= v) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
-}
+// Block-wide helpers of the .g parser kernels.
+constexpr uint32_t IB = 256;  // threads per block
 
 // Exclusive block scan over IB threads; total returned through *total.
 __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *s_wave, uint32_t &total) {
@@ -2114,354 +2043,6 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *s_wave
     total = tot;
     return pre + incl - v;
 }
-
-struct IsectPair {
-    const uint32_t *A, *B;
-    uint32_t na, nb;
-};
-
-__device__ __forceinline__ IsectPair isect_pair(const uint32_t *data, const uint64_t *a_off, const uint32_t *a_len,
-                                                const uint64_t *b_off, const uint32_t *b_len, uint32_t p) {
-    IsectPair r{data + a_off[p], data + b_off[p], a_len[p], b_len[p]};
-    if (r.na > r.nb) {
-        const uint32_t *tp = r.A;
-        r.A = r.B;
-        r.B = tp;
-        const uint32_t tn = r.na;
-        r.na = r.nb;
-        r.nb = tn;
-    }
-    return r;
-}
-
-__device__ __forceinline__ bool isect_skewed(uint32_t na, uint32_t nb) {
-    return (unsigned long long)nb >= (unsigned long long)ISKEW * na;
-}
-
-// The tiles of pair p: none without an element in the shorter list.
-__device__ __forceinline__ uint32_t isect_tiles(uint32_t la, uint32_t lb) {
-    const uint32_t na = min(la, lb), nb = max(la, lb);
-    if (na == 0) return 0;
-    return isect_skewed(na, nb) ? (na + IB - 1) / IB
-                                : (uint32_t)(((unsigned long long)na + nb + XT - 1) / XT);
-}
-
-// Size class of a tile count nt >= 1, 0..511: 16 * (bit length - 1) plus
-// the four bits after the leading one (exact below 32 tiles, within about
-// 6% above), so that blocks taking pairs largest first in class order end
-// within a few percent of each other.
-constexpr uint32_t ISECT_CLASSES = 512;
-__device__ __forceinline__ uint32_t isect_size_class(uint32_t nt) {
-    const uint32_t b = 31 - __clz(nt);
-    const uint32_t frac = b >= 4 ? (nt >> (b - 4)) & 15u : (nt << (4 - b)) & 15u;
-    return 16 * b + frac;
-}
-
-// One block of 1,024 threads. tile_begin by a scan of the tile counts, the
-// pairs with tiles in order of their tile count (size classes: the bit
-// length and the next four bits), largest first;
-// counts cleared; work[0] = 0 (claim counter), work[1] = pairs with tiles,
-// work[2] = tiles, work[3] = 1 (and no work) when the tiles exceed cap.
-__global__ __launch_bounds__(1024) void fm_isect_plan_kernel(const uint32_t *a_len, const uint32_t *b_len,
-                                                             uint32_t npairs, uint32_t *counts, uint32_t *work,
-                                                             uint32_t cap) {
-    constexpr uint32_t NB = ISECT_CLASSES;  // size classes of a tile count (< 2^32)
-    __shared__ uint32_t hist[NB], start[NB];
-    __shared__ unsigned long long s_part[1024 / WAVE];
-    __shared__ unsigned long long s_carry;
-    const uint32_t tid = threadIdx.x, lane = lane_id(), wid = tid / WAVE;
-    uint32_t *const order = work + ISECT_HDR;
-    uint32_t *const tile_begin = order + npairs;
-    if (tid < NB) hist[tid] = 0;
-    if (tid == 0) s_carry = 0;
-    __syncthreads();
-    for (uint32_t p0 = 0; p0 < npairs; p0 += 1024) {
-        const uint32_t p = p0 + tid;
-        const uint32_t nt = p < npairs ? isect_tiles(a_len[p], b_len[p]) : 0u;
-        if (p < npairs) counts[p] = 0;
-        if (nt) atomicAdd(&hist[(NB - 1) - isect_size_class(nt)], 1u);
-        // block scan of the tile counts (64-bit: the total may pass 2^32)
-        unsigned long long v = nt;
-#pragma unroll
-        for (int o = 1; o < (int)WAVE; o <<= 1) {
-            const unsigned long long t = __shfl_up(v, o, WAVE);
-            if (lane >= (uint32_t)o) v += t;
-        }
-        if (lane == WAVE - 1) s_part[wid] = v;
-        __syncthreads();
-        unsigned long long pre = s_carry, tot = s_carry;
-        for (uint32_t w = 0; w < 1024 / WAVE; w++) {
-            if (w < wid) pre += s_part[w];
-            tot += s_part[w];
-        }
-        const unsigned long long begin = pre + v - nt;
-        if (p < npairs) tile_begin[p] = begin <= cap ? (uint32_t)begin : cap;
-        __syncthreads();
-        if (tid == 0) s_carry = tot;
-        __syncthreads();
-    }
-    if (wid == 0) {
-        // exclusive scan of the class histogram, NB / WAVE classes per lane
-        static_assert(NB % WAVE == 0, "classes split evenly over a wave");
-        constexpr uint32_t PER = NB / WAVE;
-        uint32_t h[PER], sum = 0;
-#pragma unroll
-        for (uint32_t j = 0; j < PER; j++) {
-            h[j] = hist[lane * PER + j];
-            sum += h[j];
-        }
-        uint32_t acc = wave_incl_scan(sum) - sum;
-#pragma unroll
-        for (uint32_t j = 0; j < PER; j++) {
-            start[lane * PER + j] = acc;
-            acc += h[j];
-        }
-        if (lane == WAVE - 1) {
-            const unsigned long long total = s_carry;
-            tile_begin[npairs] = total <= cap ? (uint32_t)total : cap;
-            work[0] = 0;
-            work[1] = total <= cap ? acc : 0u;
-            work[2] = total <= cap ? (uint32_t)total : cap;
-            work[3] = total <= cap ? 0u : 1u;
-        }
-    }
-    __syncthreads();
-    for (uint32_t p = tid; p < npairs; p += 1024) {
-        const uint32_t nt = isect_tiles(a_len[p], b_len[p]);
-        if (nt) order[atomicAdd(&start[(NB - 1) - isect_size_class(nt)], 1u)] = p;
-    }
-}
-
-// Blocks take pairs (p = blockIdx.x, + gridDim.x, ...) and find the cuts of
-// their tiles, one thread per tile. Merge tiles: cut = A elements before the
-// tile's first diagonal. Skew tiles: [cut, cut2) = the range of B that holds
-// the values of the tile's A chunk.
-__global__ __launch_bounds__(IB) void fm_isect_cut_kernel(const uint32_t *__restrict__ data, const uint64_t *a_off,
-                                                          const uint32_t *a_len, const uint64_t *b_off,
-                                                          const uint32_t *b_len, uint32_t npairs, const uint32_t *work,
-                                                          uint32_t cap) {
-    if (__hip_atomic_load(&work[3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
-    const uint32_t *const tile_begin = work + ISECT_HDR + npairs;
-    uint32_t *const cut = const_cast<uint32_t *>(tile_begin) + npairs + 1;
-    uint32_t *const cut2 = cut + cap;
-    for (uint32_t p = blockIdx.x; p < npairs; p += gridDim.x) {
-        const uint32_t tb = tile_begin[p], te = tile_begin[p + 1];
-        if (tb == te) continue;
-        const IsectPair q = isect_pair(data, a_off, a_len, b_off, b_len, p);
-        for (uint32_t t = tb + threadIdx.x; t < te; t += IB) {
-            const uint32_t k = t - tb;
-            if (isect_skewed(q.na, q.nb)) {
-                const uint32_t first = k * IB, last = min(first + IB, q.na) - 1;
-                const uint32_t lo = lower_bound_u32(q.B, 0, q.nb, q.A[first]);
-                cut[t] = lo;
-                cut2[t] = upper_bound_u32(q.B, lo, q.nb, q.A[last]);
-            } else {
-                cut[t] = merge_split(q.A, q.na, q.B, q.nb, k * XT);
-                cut2[t] = 0;
-            }
-        }
-    }
-}
-
-// Persistent blocks claim whole pairs (largest first, `order`) and stream
-// their tiles. A merge tile's elements are loaded into registers while the
-// previous tile is merged and stored to LDS at the end of that iteration, so
-// no register is carried from one iteration to the next (a loop-carried
-// prefetch register makes the compiler wait for the loads at the loop join,
-// exposing their latency on every tile). The matches leave through LDS in
-// coalesced stores at the pair's running offset.
-__global__ __launch_bounds__(IB) __attribute__((amdgpu_waves_per_eu(FM_ISECT_WAVES, FM_ISECT_WAVES))) void fm_isect_kernel(const uint32_t *__restrict__ data, const uint64_t *a_off,
-                                                      const uint32_t *a_len, const uint64_t *b_off,
-                                                      const uint32_t *b_len, uint32_t *counts,
-                                                      const uint64_t *out_off, uint32_t *out, uint32_t npairs,
-                                                      uint32_t *work, uint32_t cap) {
-    // a tile's A slice, then its B slice (+ the next B element), each
-    // starting on a chunk of IB words
-    constexpr uint32_t XR = XVT + 2;  // ceil(ta / IB) + ceil((tb + 1) / IB) <= XVT + 2
-    __shared__ uint32_t sAB[XR * IB];
-    // A match takes two merge steps (its A element, then the equal B
-    // element), so a stretch holds at most XM matches. They are parked at
-    // tid * (XM + 1) (an odd stride: conflict-free banks) and then moved to
-    // their block order in the same buffer.
-    constexpr uint32_t XM = (XVT + 1) / 2;
-    __shared__ uint32_t s_out[IB * (XM + 1)];
-    __shared__ uint32_t s_wave[IB / WAVE];
-    __shared__ uint32_t s_w;
-    // cuts of the pair's tiles, XCUT at a time (no dependent global load
-    // before a tile's element loads)
-    constexpr uint32_t XCUT = 2 * IB;
-    __shared__ uint32_t s_cut[XCUT + 1], s_cut2[XCUT];
-    const uint32_t tid = threadIdx.x;
-    uint32_t *const ctr = work;
-    const uint32_t *const order = work + ISECT_HDR;
-    const uint32_t *const tile_begin = order + npairs;
-    const uint32_t *const cut = tile_begin + npairs + 1;
-    const uint32_t *const cut2 = cut + cap;
-    // loop controls in scalar registers (readfirstlane): every barrier below
-    // is taken by the whole block on a scalar branch
-    const uint32_t nwork =
-        __builtin_amdgcn_readfirstlane(__hip_atomic_load(&work[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    while (true) {
-        if (tid == 0) s_w = atomicAdd(ctr, 1u);
-        __syncthreads();
-        const uint32_t w = __builtin_amdgcn_readfirstlane(s_w);
-        if (w >= nwork) break;
-        const uint32_t p = order[w];
-        const IsectPair q = isect_pair(data, a_off, a_len, b_off, b_len, p);
-        const uint32_t tb = tile_begin[p], te = tile_begin[p + 1];
-        const uint32_t total_path = q.na + q.nb;
-        uint32_t *const o = out ? out + out_off[p] : nullptr;
-        uint32_t cbase = tb;  // tile of s_cut[0]
-        auto fill_cuts = [&](uint32_t from) {
-            cbase = from;
-            for (uint32_t i = tid; i <= XCUT; i += IB) {
-                const uint32_t t = from + i;
-                if (i < XCUT) s_cut2[i] = t < te ? cut2[t] : 0u;
-                s_cut[i] = t < te ? cut[t] : q.na;  // past the pair's last tile: all of A
-            }
-        };
-        fill_cuts(tb);
-        __syncthreads();
-        uint32_t found = 0;
-        if (isect_skewed(q.na, q.nb)) {
-            // skew tiles: each thread looks one A element up in the tile's
-            // range of B
-            for (uint32_t t = tb; t < te; t++) {
-                if (t - cbase == XCUT) {
-                    fill_cuts(t);
-                    __syncthreads();
-                }
-                const uint32_t c0 = s_cut[t - cbase], c1 = s_cut2[t - cbase];
-                const uint32_t first = (t - tb) * IB, cn = min(IB, q.na - first);
-                uint32_t c = 0, x = 0;
-                if (tid < cn) {
-                    x = q.A[first + tid];
-                    const uint32_t at = lower_bound_u32(q.B, c0, c1, x);
-                    c = at < c1 && q.B[at] == x ? 1u : 0u;
-                }
-                uint32_t tot;
-                const uint32_t at = block_excl_scan(c, s_wave, tot);  // its barriers order s_out
-                if (o) {
-                    if (c) s_out[at] = x;
-                    __syncthreads();
-                    for (uint32_t i = tid; i < tot; i += IB) o[found + i] = s_out[i];
-                }
-                found += tot;
-                __syncthreads();  // s_out and s_cut are rewritten by the next tile
-            }
-        } else {
-            // A tile's LDS image: its A slice from word 0 and its B slice
-            // (with the next B element) from word xa * IB, xa = ceil(ta / IB).
-            // Register x holds word x * IB + tid of the image, so a register
-            // is a whole chunk of A or of B: its load address is a scalar
-            // base plus tid (clamped to the list), and its LDS store needs no
-            // test. Words past a slice hold clamped copies that the merge
-            // never takes.
-            uint32_t reg[XR];
-            uint32_t nc0 = 0, nc1 = 0;  // cuts of the tile in reg
-            auto load_tile = [&](uint32_t t) {
-                nc0 = s_cut[t - cbase];
-                nc1 = s_cut[t - cbase + 1];
-                const uint32_t b0 = (t - tb) * XT - nc0;
-                const uint32_t xa = (nc1 - nc0 + IB - 1) / IB;
-#pragma unroll
-                for (uint32_t x = 0; x < XR; x++) {
-                    const bool in_a = x < xa;  // the same for the whole block
-                    const uint32_t *const src = in_a ? q.A : q.B;
-                    const uint32_t first = in_a ? nc0 + x * IB : b0 + (x - xa) * IB;
-                    reg[x] = src[min(first + tid, (in_a ? q.na : q.nb) - 1)];
-                }
-            };
-            auto store_tile = [&]() {
-#pragma unroll
-                for (uint32_t x = 0; x < XR; x++) sAB[x * IB + tid] = reg[x];
-            };
-            load_tile(tb);
-            uint32_t c0 = nc0, c1 = nc1;
-            store_tile();
-            __syncthreads();
-            for (uint32_t t = tb; t < te; t++) {
-                // t + 1 - cbase <= XCUT: the window holds the next tile's
-                // cuts unless it starts past it (then it is refilled below)
-                const bool has_next = t + 1 < te && t + 1 - cbase < XCUT;
-                load_tile(has_next ? t + 1 : t);  // in flight during the merge
-                const uint32_t d0 = (t - tb) * XT, d1 = min(d0 + XT, total_path);
-                const uint32_t b0 = d0 - c0, b1 = d1 - c1;
-                const uint32_t ta = c1 - c0, tbn = b1 - b0, tbx = tbn + (b1 < q.nb ? 1u : 0u);
-                const uint32_t *const sA = sAB, *const sB = sAB + ((ta + IB - 1) / IB) * IB;
-                const uint32_t dl = min(tid * XVT, ta + tbn);
-                const uint32_t i0 = merge_split(sA, ta, sB, tbn, dl);
-                const uint32_t j0 = dl - i0;
-                // Merges the stretch; emit(match, x) at every step. Consuming
-                // A[i], an equal B element is the next one on B's side.
-                // Branch-free steps: the lanes of a wave take A or B steps in
-                // any mix, so each step advances one side by a predicate and
-                // reloads both current elements. A thread whose stretch ends
-                // at the tile's end (ta + tb) runs its remaining steps from
-                // (ta, tb): B steps only, which match nothing, so every
-                // thread runs XVT steps. Indices stay below ta + XVT and
-                // tb + XVT, inside the image.
-                auto merge_stretch = [&](auto emit) {
-                    uint32_t ii = i0, jj = j0;
-                    uint32_t av = sA[ii], bv = sB[jj];
-#pragma unroll
-                    for (uint32_t st = 0; st < XVT; st++) {
-                        const bool take_a = ii < ta && (jj >= tbn || av <= bv);
-                        emit(take_a && jj < tbx && bv == av, av);
-                        ii += take_a ? 1u : 0u;
-                        jj += take_a ? 0u : 1u;
-                        av = sA[ii];
-                        bv = sB[jj];
-                    }
-                };
-                uint32_t c = 0;
-                // strictly increasing lists give at most XM matches per
-                // stretch; the clamp keeps any input inside the buffers
-                if (o) merge_stretch([&](bool m, uint32_t x) {
-                    s_out[tid * (XM + 1) + min(c, XM)] = x;
-                    c += m ? 1u : 0u;
-                });
-                else merge_stretch([&](bool m, uint32_t) { c += m ? 1u : 0u; });
-                c = min(c, XM);
-                uint32_t tot;
-                const uint32_t at = block_excl_scan(c, s_wave, tot);  // its barriers order the parking
-                if (o) {
-                    // parked matches to block order, then coalesced stores
-                    uint32_t m[XM];
-#pragma unroll
-                    for (uint32_t x = 0; x < XM; x++) m[x] = x < c ? s_out[tid * (XM + 1) + x] : 0u;
-                    __syncthreads();
-#pragma unroll
-                    for (uint32_t x = 0; x < XM; x++)
-                        if (x < c) s_out[at + x] = m[x];
-                    __syncthreads();
-                    for (uint32_t x = tid; x < tot; x += IB) o[found + x] = s_out[x];
-                }
-                found += tot;
-                __syncthreads();  // sAB and s_out are rewritten
-                if (has_next) {
-                    c0 = nc0;
-                    c1 = nc1;
-                    store_tile();
-                } else if (t + 1 < te) {
-                    // the next tile is past the cut window: refill, then load it
-                    fill_cuts(t + 1);
-                    __syncthreads();
-                    load_tile(t + 1);
-                    c0 = nc0;
-                    c1 = nc1;
-                    store_tile();
-                }
-                __syncthreads();
-            }
-        }
-        if (tid == 0) counts[p] = found;
-        __syncthreads();  // s_cut is refilled for the next pair
-    }
-}
-
-
-
 
 // ---------------------------------------------------------------------------
 // .g data file parser on the device (the reference parses on the GPU too:
@@ -3973,13 +3554,14 @@ void fm_server_destroy(fm_server *s) {
 // ---------------------------------------------------------------------------
 // Intersection host side.
 
-// The no-allocation entry point's work array (claim counter, pair order,
-// tile starts, cuts): a fixed device array, each GPU holding its own copy
-// until the process ends. Host code uses the address hipGetSymbolAddress
-// returns for the device, taken once per device at engine creation, never
-// the array's name.
+// The no-allocation entry point's work array (claim counter, pairs' first
+// segments, per-segment status words): a fixed device array, each GPU
+// holding its own copy until the process ends. It starts zeroed (a static
+// device array), which the status words' call epochs need. Host code uses
+// the address hipGetSymbolAddress returns for the device, taken once per
+// device at engine creation, never the array's name.
 constexpr uint32_t NOALLOC_WORDS = FM_NOALLOC_SCRATCH_BYTES / 4;
-static_assert(ISECT_HDR + 2ull * FM_NOALLOC_MAX_PAIRS + 1 < NOALLOC_WORDS, "the pair arrays fit the scratch");
+static_assert(isx_work_words(FM_NOALLOC_MAX_PAIRS, 0) < NOALLOC_WORDS, "the pair arrays fit the scratch");
 static __device__ __align__(256) uint32_t scratch[NOALLOC_WORDS];
 constexpr int FM_MAX_DEVICES = 64;
 static std::mutex g_scratch_mu;
@@ -4004,14 +3586,9 @@ static uint32_t *isect_scratch_of(int device) {
     return g_scratch_ptr[device];
 }
 
-// Words of the work array for npairs pairs and cap tiles.
-static uint64_t isect_work_words(uint32_t npairs, uint64_t cap) {
-    return ISECT_HDR + 2ull * npairs + 1 + 2 * cap;
-}
-
-// The plan, cut and main kernels on the engine's stream with `work` holding
-// isect_work_words(npairs, cap) words; allocates nothing. When the pairs
-// need more than cap tiles the kernels do no work and *over is set.
+// The plan and main kernels on the engine's stream with `work` holding
+// isx_work_words(npairs, cap) words; allocates nothing. When the pairs need
+// more than cap segments the kernels do no work and *over is set.
 static int isect_run(fm_engine *e, uint32_t *work, uint32_t cap, const uint32_t *data, const uint64_t *a_off,
                      const uint32_t *a_len, const uint64_t *b_off, const uint32_t *b_len, uint32_t npairs,
                      uint32_t *counts, const uint64_t *out_off, uint32_t *out, double *kernel_ms, bool *over) {
@@ -4021,38 +3598,20 @@ static int isect_run(fm_engine *e, uint32_t *work, uint32_t cap, const uint32_t 
     if (over) *over = false;
     if (npairs == 0) return FM_OK;
     HIP_TRY(hipSetDevice(e->device));
-    int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fm_isect_kernel, IB, 0) != hipSuccess || blocks <= 0)
-        blocks = 4;
-    // Blocks take whole pairs, largest first. With more pairs than resident
-    // blocks, every block takes the same number of pairs (rounds), so that a
-    // large pair is followed by a small one: on 2,048 pairs of 32k-96k
-    // elements (3,840-element tiles), 1,024 blocks end within 2% of each
-    // other where 1,280 end 17% apart.
-    const uint64_t resident = std::max<uint64_t>(1, (uint64_t)blocks * e->cu_count);
-    const uint64_t rounds = (npairs + resident - 1) / resident;
-    const uint32_t grid = (uint32_t)((npairs + rounds - 1) / rounds);
-    const uint32_t cut_grid = std::min<uint32_t>(npairs, 4096);
+    const uint32_t grid = (uint32_t)std::max(1, isx_blocks_per_cu() * e->cu_count);
     HIP_TRY(hipEventRecord(e->ev0, e->stream));
-    hipLaunchKernelGGL(fm_isect_plan_kernel, dim3(1), dim3(1024), 0, e->stream, a_len, b_len, npairs, counts, work,
-                       cap);
-    HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(fm_isect_cut_kernel, dim3(cut_grid), dim3(IB), 0, e->stream, data, a_off, a_len, b_off,
-                       b_len, npairs, (const uint32_t *)work, cap);
-    HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(fm_isect_kernel, dim3(grid), dim3(IB), 0, e->stream, data, a_off, a_len, b_off, b_len, counts,
-                       out_off, out, npairs, work, cap);
+    isx_launch(e->stream, grid, work, cap, data, a_off, a_len, b_off, b_len, npairs, counts, out_off, out);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(e->ev1, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, e->ev0, e->ev1));
     if (kernel_ms) *kernel_ms = ms;
-    if (over) {
-        uint32_t flag = 0;
-        HIP_TRY(hipMemcpy(&flag, work + 3, 4, hipMemcpyDeviceToHost));
-        *over = flag != 0;
-    }
+    uint32_t flags = 0;
+    HIP_TRY(hipMemcpy(&flags, work + ISX_FLAGS, 4, hipMemcpyDeviceToHost));
+    if (flags & ISX_GAVE_UP)
+        return fail(FM_ERR_INTERNAL, "intersection: a segment's wait for the earlier segments of its pair passed its limit");
+    if (over) *over = (flags & ISX_OVER) != 0;
     return FM_OK;
 }
 
@@ -4065,26 +3624,25 @@ int fm_intersect_pairs(fm_engine *e, const uint32_t *data, const uint64_t *a_off
     if (npairs == 0 || !a_len || !b_len)
         return isect_run(e, nullptr, 0, data, a_off, a_len, b_off, b_len, npairs, counts, out_off, out, kernel_ms,
                          nullptr);
-    // size the work array: the tiles of every pair (lengths read back)
+    // size the work array: the segments of every pair (lengths read back)
     HIP_TRY(hipSetDevice(e->device));
     std::vector<uint32_t> la(npairs), lb(npairs);
     HIP_TRY(hipMemcpy(la.data(), a_len, npairs * 4ull, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(lb.data(), b_len, npairs * 4ull, hipMemcpyDeviceToHost));
-    uint64_t tiles = 0;
-    for (uint32_t p = 0; p < npairs; p++) {
-        const uint64_t na = std::min(la[p], lb[p]), nb = std::max(la[p], lb[p]);
-        if (na) tiles += nb >= (uint64_t)ISKEW * na ? (na + IB - 1) / IB : (na + nb + XT - 1) / XT;
-    }
-    if (tiles >= (1ull << 31)) return fail(FM_ERR_INVALID_ARG, "too many intersection tiles");
-    const size_t need = isect_work_words(npairs, tiles) * 4;
+    uint64_t segs = 0;
+    for (uint32_t p = 0; p < npairs; p++) segs += isx_segments(la[p], lb[p]);
+    if (segs >= (1ull << 31)) return fail(FM_ERR_INVALID_ARG, "too many intersection segments");
+    const size_t need = isx_work_words(npairs, segs) * 4;
     if (need > e->isect_cap) {
         (void)hipFree(e->isect_buf);
         e->isect_buf = nullptr;
         e->isect_cap = 0;
         HIP_TRY(hipMalloc((void **)&e->isect_buf, need));
+        // status words carry a call epoch: a new array starts from zero
+        HIP_TRY(hipMemset(e->isect_buf, 0, need));
         e->isect_cap = need;
     }
-    return isect_run(e, (uint32_t *)e->isect_buf, (uint32_t)tiles, data, a_off, a_len, b_off, b_len, npairs, counts,
+    return isect_run(e, (uint32_t *)e->isect_buf, (uint32_t)segs, data, a_off, a_len, b_off, b_len, npairs, counts,
                      out_off, out, kernel_ms, nullptr);
 }
 
@@ -4097,13 +3655,14 @@ int fm_intersect_pairs_noalloc(fm_engine *e, const uint32_t *data, const uint64_
                     (uint32_t)FM_NOALLOC_MAX_PAIRS);
     uint32_t *work = isect_scratch_of(e->device);
     if (!work) return fail(FM_ERR_DEVICE, "no scratch address for device %d", e->device);
-    const uint32_t cap = (uint32_t)((NOALLOC_WORDS - isect_work_words(npairs, 0)) / 2);
+    const uint32_t cap = (uint32_t)((NOALLOC_WORDS - isx_work_words(npairs, 0)) / 2);
     bool over = false;
     int rc = isect_run(e, work, cap, data, a_off, a_len, b_off, b_len, npairs, counts, out_off, out, kernel_ms, &over);
     if (rc) return rc;
     if (over)
-        return fail(FM_ERR_INVALID_ARG, "the pairs need more than the %u tiles of the no-allocation scratch (%u bytes)",
-                    cap, (uint32_t)FM_NOALLOC_SCRATCH_BYTES);
+        return fail(FM_ERR_INVALID_ARG,
+                    "the pairs need more than the %u segments of the no-allocation scratch (%u bytes)", cap,
+                    (uint32_t)FM_NOALLOC_SCRATCH_BYTES);
     return FM_OK;
 }
 

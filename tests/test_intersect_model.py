@@ -1,234 +1,448 @@
-"""CPU model of the tiled intersection kernels (engine.hip:
-fm_isect_cut_kernel, fm_isect_kernel): the host tile plan (merge tiles of
-4864 merge-path elements, skew tiles of 256 elements of the shorter list when
-the longer one is >= 32x longer), each tile's cut, the tile's LDS image (the
-A slice from word 0 and the B slice from the next 256-word chunk, loaded a
-chunk of one list per register at clamped indices), the per-thread 19-element
-stretches found by binary search in the image, the branch-free steps that
-every thread runs to the full 19 (past its stretch only B steps, which match
-nothing), the match check against the next B element (staged one past the
-tile), and the output offset of each tile as the sum of the counts of the
-pair's earlier tiles (the running offset of the block that streams the pair).
-Checked against numpy on random and edge-case pairs, including matches
-straddling tile and thread boundaries."""
+"""CPU model of the segmented intersection kernels (isect.hip:
+fm_isx_plan_kernel, fm_isx_kernel), checked against numpy.
+
+Modelled as the device runs them: the segment counts and the plan's
+first-segment table (clamped to the work array, overflow flagged); a wave's
+pair lookup (64 samples per round); the two merge-path cuts of a segment
+searched by half waves with 32 samples per round; a merge segment's A part
+A[i0, i1) and B part B[j0, j1] (one B element past the cut); dense segments
+(A part values spanning < 16,384 integers): chunks loaded as two 16-byte
+loads per lane (lane l holds elements 4l..4l+3 and 256+4l..256+4l+3) through
+a buffer range check (zeros past a chunk's end, masked by index in partial
+chunks); each lane's group of four elements sets its bits with two ORs (the
+word of the group's first element and the next one) and, when the group
+spans further or its first element is out of range, one OR per element; the
+A bitmap, and the bitmap of the B elements inside the A part's range, in
+512-word regions whose words past the cleared range hold stale bits; the
+count as the popcount of their AND, and the matches written from the AND
+in 64-word rounds through a 512-word stage; sparse segments: one A chunk at
+a time sorted in LDS (clamped copies of its last element after its end), B
+windows searched there in 9 steps, chunk and window advanced by their last
+values, counted first and searched again to write; skew segments (the
+longer list >= 32x the shorter): the B range of a 512-element A chunk by two
+32-ary searches and a branch-free binary search per element; and the
+decoupled look-back that gives each segment the number of matches in its
+pair's earlier segments, under random interleavings of the waves."""
 import numpy as np
 import pytest
 
-IB, XVT, ISKEW = 256, 19, 32
-XT = IB * XVT
+W, K = 64, 8
+CH = K * W
+SEG = 8192
+BMW = 512
+BMSPAN = BMW * 32
+SKEW = 32
+U32 = 0xFFFFFFFF
 
 
-def merge_split(A, na, B, nb, diag):
-    lo, hi = max(0, diag - nb), min(diag, na)
-    while lo < hi:
-        mid = (lo + hi) >> 1
-        if A[mid] <= B[diag - 1 - mid]:
-            lo = mid + 1
-        else:
-            hi = mid
+def segments(la, lb, seg=SEG, ch=CH):
+    na, nb = min(la, lb), max(la, lb)
+    if na == 0:
+        return 0
+    return -(-na // ch) if nb >= SKEW * na else -(-(na + nb) // seg)
+
+
+def plan(lens, cap, seg=SEG, ch=CH):
+    """fm_isx_plan_kernel: seg_begin (each entry at most cap), the total
+    (0 when it exceeds cap) and the overflow flag."""
+    begin, acc = [], 0
+    for la, lb in lens:
+        begin.append(min(acc, cap))
+        acc += segments(la, lb, seg, ch)
+    begin.append(min(acc, cap))
+    over = acc > cap
+    return begin, (0 if over else acc), over
+
+
+def find_pair(seg_begin, npairs, g):
+    """64 samples per round; the largest p < npairs with seg_begin[p] <= g."""
+    lo, hi = 0, npairs
+    while hi - lo > 1:
+        step = -(-(hi - lo) // W)
+        t = sum(1 for k in range(W) if k * step < hi - lo and seg_begin[lo + k * step] <= g)
+        assert t >= 1
+        lo, hi = lo + (t - 1) * step, min(lo + t * step, hi)
     return lo
 
 
-def plan(pairs):
-    """tile_begin per pair and the pair of each tile (host plan)."""
-    begin, tile_pair = [], []
-    for p, (A, B) in enumerate(pairs):
-        begin.append(len(tile_pair))
-        na, nb = sorted((len(A), len(B)))
-        if na == 0:
-            continue
-        n = -(-na // IB) if nb >= ISKEW * na else -(-(na + nb) // XT)
-        tile_pair += [p] * n
-    begin.append(len(tile_pair))
-    return begin, tile_pair
-
-
-def tile_matches(A, B, k, cuts, te_minus_tb):
-    """Matches of tile k of a pair, in thread order (the block scan order)."""
-    na, nb = len(A), len(B)
-    if nb >= ISKEW * na:
-        first, last = k * IB, min(k * IB + IB, na) - 1
-        lo = int(np.searchsorted(B, A[first], "left"))
-        hi = int(np.searchsorted(B, A[last], "right"))
-        out = []
-        for x in A[first:last + 1]:
-            i = lo + int(np.searchsorted(B[lo:hi], x, "left"))
-            if i < hi and B[i] == x:
-                out.append(int(x))
-        return out
-    d0, d1 = k * XT, min(k * XT + XT, na + nb)
-    a0 = cuts[k]
-    a1 = cuts[k + 1] if k + 1 < te_minus_tb else na
-    b0, b1 = d0 - a0, d1 - a1
-    assert 0 <= a1 - a0 <= XT and 0 <= b1 - b0 <= XT
-    sA = A[a0:a1]
-    ta, tb = a1 - a0, b1 - b0
-    tbx = tb + (1 if b1 < nb else 0)
-    # the LDS image: register x of thread tid holds word x * IB + tid; the A
-    # slice from word 0, the B slice (+ the next B element) from word
-    # xa * IB; every load is a chunk of one list at a clamped index
-    xa = -(-ta // IB)
-    XR = XVT + 2
-    assert xa + -(-tbx // IB) <= XR
-    img = np.empty(XR * IB, dtype=np.int64)
-    for x in range(XR):
-        e = np.arange(IB)
-        if x < xa:
-            img[x * IB:(x + 1) * IB] = A[np.minimum(a0 + x * IB + e, na - 1)]
+def first_false(lo, hi, pred):
+    """One half wave of first_false2: 32 samples per round."""
+    rounds = 0
+    while lo < hi:
+        n, step = hi - lo, -(-(hi - lo) // 32)
+        probes = [pred(lo + k * step) for k in range(32) if k * step < n]
+        t = sum(probes)
+        assert probes == [True] * t + [False] * (len(probes) - t)  # the predicate is monotone
+        if t == 0:
+            hi = lo
         else:
-            img[x * IB:(x + 1) * IB] = B[np.minimum(b0 + (x - xa) * IB + e, nb - 1)]
-    assert np.array_equal(img[:ta], sA) and np.array_equal(img[xa * IB:xa * IB + tbx], B[b0:b0 + tbx])
-    sB_at = xa * IB
+            lo, hi = lo + (t - 1) * step + 1, min(lo + t * step, hi)
+        rounds += 1
+    return lo, rounds
+
+
+def merge_cut(A, B, d):
+    """A elements among the first d of merge(A, B), A first on ties."""
+    na, nb = len(A), len(B)
+    return first_false(max(0, d - nb), min(d, na), lambda i: int(A[i]) <= int(B[d - 1 - i]))
+
+
+def buffer_chunk(src, start, n):
+    """load_chunk_x4: CH words from src[start:], zeros at or past n, as
+    (lane, group, element) -> element index h*256 + 4*lane + e."""
+    flat = np.zeros(CH, np.uint64)
+    m = min(n, CH)
+    if m > 0:
+        flat[:m] = src[start:start + m]
+    return flat
+
+
+def set_bits(bm, flat, amin, span, n, check):
+    """set_bits: per lane and group of four consecutive elements, two ORs
+    at the group's first word and the next, single ORs for the rest."""
+    for h in range(2):
+        for l in range(W):
+            idx = [h * (CH // 2) + 4 * l + e for e in range(4)]
+            off = [(int(flat[i]) - amin) & U32 for i in idx]
+            ok = [(not check or o <= span) and i < n for o, i in zip(off, idx)]
+            wb = off[0] >> 5
+            lo = hi = 0
+            for e in range(4):
+                rel = (off[e] - (wb << 5)) & U32
+                near = ok[0] and rel < 64
+                bit = 1 << (rel & 31)
+                if ok[e] and near:
+                    if rel < 32:
+                        lo |= bit
+                    else:
+                        hi |= bit
+                elif ok[e]:
+                    bm[(off[e] >> 5) & (BMW - 1)] |= 1 << (off[e] & 31)
+            if ok[0]:
+                bm[wb & (BMW - 1)] |= lo
+                bm[(wb + 1) & (BMW - 1)] |= hi
+
+
+def clamped_chunk(src, start, n):
+    """load_chunk: CH words, clamped to the last of n >= 1."""
+    idx = np.minimum(np.arange(CH), n - 1)
+    return src[start + idx].astype(np.uint64)
+
+
+def dense_segment(pa, pb, amin, span, rng):
+    """Matches of a dense segment as dense_emit writes them."""
+    la, lb = len(pa), len(pb)
+    bmA = [int(x) for x in rng.integers(0, 1 << 32, BMW, dtype=np.uint64)]  # stale bits
+    bmB = [int(x) for x in rng.integers(0, 1 << 32, BMW, dtype=np.uint64)]
+    nw = (span >> 5) + 1 if la else 0
+    assert nw <= BMW
+    for w in range(nw):
+        bmA[w] = bmB[w] = 0
+    for c in range(-(-la // CH)):
+        set_bits(bmA, buffer_chunk(pa, c * CH, la - c * CH), amin, span, la - c * CH, False)
+    if la == 0:
+        return []
+    for c in range(-(-lb // CH)):
+        set_bits(bmB, buffer_chunk(pb, c * CH, lb - c * CH), amin, span, lb - c * CH, True)
+    common = [bmA[w] & bmB[w] for w in range(nw)]
+    # dense_emit: 64-word rounds, each lane's values at its word's prefix offset, through a CH-word stage
     out = []
-    for tid in range(IB):
-        dl = min(tid * XVT, ta + tb)
-        i = merge_split(img, ta, img[sB_at:], tb, dl)
-        j = dl - i
-        # every thread runs XVT branch-free steps: one whose stretch ends at
-        # the tile's end continues from (ta, tb) with B steps only
-        av, bv = img[i], img[sB_at + j]
-        for st in range(XVT):
-            take_a = i < ta and (j >= tb or av <= bv)
-            if take_a and j < tbx and bv == av:
-                out.append(int(av))
-            i += 1 if take_a else 0
-            j += 0 if take_a else 1
-            assert i <= ta and sB_at + j < XR * IB  # reads stay in the image
-            if not take_a and j > tb:
-                assert st >= min(XVT, ta + tb - dl)  # only past the stretch
-            av, bv = img[i], img[sB_at + j]
+    for w0 in range(0, nw, W):
+        words = [common[w] if w < nw else 0 for w in range(w0, w0 + W)]
+        counts = [bin(x).count("1") for x in words]
+        tot = sum(counts)
+        incl = np.cumsum(counts)
+        for lo in range(0, tot, CH):
+            stage = [None] * CH
+            for l in range(W):
+                pos = int(incl[l]) - counts[l]
+                b = words[l]
+                while b:
+                    if lo <= pos < lo + CH:
+                        stage[pos - lo] = amin + 32 * (w0 + l) + ((b & -b).bit_length() - 1)
+                    pos += 1
+                    b &= b - 1
+            cnt = min(CH, tot - lo)
+            assert None not in stage[:cnt]
+            out += stage[:cnt]
     return out
 
 
-def intersect_model(pairs):
-    begin, tile_pair = plan(pairs)
-    results = []
-    for p, (A, B) in enumerate(pairs):
+def sparse_segment(pa, pb):
+    la, lb = len(pa), len(pb)
+    ia = jb = 0
+    out = []
+    a_has, b_has = la > 0, lb > 0
+    sorted_chunk = amin = amax = bmax = b = None
+
+    def next_a():
+        ch = clamped_chunk(pa, ia, la - ia)
+        return ch, int(ch[0]), int(ch[-1])
+
+    def next_b():
+        ch = clamped_chunk(pb, jb, lb - jb)
+        return ch, int(ch[-1])
+
+    if a_has:
+        sorted_chunk, amin, amax = next_a()
+    if b_has:
+        b, bmax = next_b()
+    while a_has or b_has:
+        if a_has and b_has:
+            n = lb - jb
+            for k in range(CH):
+                x = int(b[k])
+                pos, h = 0, CH // 2
+                while h >= 1:
+                    if int(sorted_chunk[pos + h - 1]) < x:
+                        pos += h
+                    h >>= 1
+                assert pos < CH
+                if amin <= x <= amax and k < n and int(sorted_chunk[pos]) == x:
+                    out.append(x)
+        adv_a, adv_b = a_has, b_has
+        if a_has and b_has:
+            adv_b, adv_a = bmax <= amax, amax <= bmax
+        if adv_b:
+            jb += CH
+            b_has = jb < lb
+            if b_has:
+                b, bmax = next_b()
+        if adv_a:
+            ia += CH
+            a_has = ia < la
+            if a_has:
+                sorted_chunk, amin, amax = next_a()
+    return out
+
+
+def skew_segment(pa, B):
+    la, nb = len(pa), len(B)
+    a = clamped_chunk(pa, 0, la)
+    amin, amax = int(a[0]), int(a[-1])
+    lo, _ = first_false(0, nb, lambda j: int(B[j]) < amin)
+    hi, _ = first_false(0, nb, lambda j: int(B[j]) <= amax)
+    out = []
+    if lo < hi:
+        for k in range(CH):
+            base, n = lo, hi - lo
+            while n > 1:
+                h = n >> 1
+                assert base + h < hi
+                if int(B[base + h]) <= int(a[k]):
+                    base += h
+                n -= h
+            if k < la and int(B[base]) == int(a[k]):
+                out.append(int(a[k]))
+    return out
+
+
+def run_segment(A, B, s, rng, stats):
+    na, nb = len(A), len(B)
+    if nb >= SKEW * na:
+        first = s * CH
+        stats["skew"] += 1
+        return skew_segment(A[first:first + min(CH, na - first)], B)
+    d0, d1 = s * SEG, min(s * SEG + SEG, na + nb)
+    (i0, r0), (i1, r1) = merge_cut(A, B, d0), merge_cut(A, B, d1)
+    stats["rounds"] = max(stats["rounds"], r0, r1)
+    j0, j1 = d0 - i0, d1 - i1
+    jx = min(j1 + 1, nb)
+    pa, pb = A[i0:i1].astype(np.uint64), B[j0:jx].astype(np.uint64)
+    amin = int(pa[0]) if len(pa) else 0
+    span = int(pa[-1]) - amin if len(pa) else 0
+    if span < BMSPAN:
+        stats["dense"] += 1
+        m = dense_segment(pa, pb, amin, span, rng)
+    else:
+        stats["sparse"] += 1
+        m = sparse_segment(pa, pb)
+    return m
+
+
+def intersect_model(pairs, rng):
+    """Per pair: the matches of its segments concatenated in segment order,
+    each segment stored at the offset the look-back gives it."""
+    lens = [(len(a), len(b)) for a, b in pairs]
+    seg_begin, total, over = plan(lens, 1 << 30)
+    assert not over
+    stats = {"dense": 0, "sparse": 0, "skew": 0, "rounds": 0}
+    per_seg = []
+    for g in range(total):
+        p = find_pair(seg_begin, len(pairs), g)
+        assert seg_begin[p] <= g < seg_begin[p + 1]
+        A, B = pairs[p]
         if len(A) > len(B):
             A, B = B, A
-        ntile = begin[p + 1] - begin[p]
-        cuts = [merge_split(A, len(A), B, len(B), k * XT) for k in range(ntile)]
-        out, pre = [], 0
-        for k in range(ntile):
-            m = tile_matches(A, B, k, cuts, ntile)
-            assert len(out) == pre  # the running offset is the earlier tiles' total
-            out += m
-            pre += len(m)
-        results.append(np.array(out, dtype=np.uint32))
-    assert all(tile_pair[t] == p for p in range(len(pairs)) for t in range(begin[p], begin[p + 1]))
-    return results
+        per_seg.append((p, run_segment(A, B, g - seg_begin[p], rng, stats)))
+    prefix = look_back_model([len(m) for _, m in per_seg], seg_begin, rng)
+    results = []
+    for p, (A, B) in enumerate(pairs):
+        room = min(len(A), len(B))
+        out = np.full(room, U32, np.uint64)
+        count = 0
+        for g in range(seg_begin[p], seg_begin[p + 1]):
+            m = per_seg[g][1]
+            lim = min(len(m), room - prefix[g]) if prefix[g] < room else 0
+            out[prefix[g]:prefix[g] + lim] = m[:lim]
+            count = prefix[g] + len(m)
+        results.append(out[:count].astype(np.uint32))
+    return results, stats
 
 
-@pytest.mark.parametrize("seed", range(8))
-def test_tiled_intersection_model(seed):
-    rng = np.random.default_rng(seed)
-    shapes = [(0, 10, 50), (1, 1, 2), (5000, 5200, 20000), (3000, 3000, 3000), (4096, 4096, 8192),
-              (1, 9000, 9000), (8191, 8193, 18000), (300, 12000, 30000), (257, 9000, 9000), (20, 0, 5)]
+def look_back_model(counts, seg_begin, rng):
+    """Waves run segments concurrently in claim order; each publishes its
+    count, then reads up to 64 predecessors per round until it meets an
+    inclusive sum, then publishes its own. Random interleaving; returns every
+    segment's exclusive prefix within its pair."""
+    n = len(counts)
+    pair_first = {}
+    for p in range(len(seg_begin) - 1):
+        for g in range(seg_begin[p], seg_begin[p + 1]):
+            pair_first[g] = seg_begin[p]
+    status = [None] * n  # None, ("A", c) or ("I", sum)
+    prefix = [None] * n
+    # each wave: state 0 = counting, 1 = looking back, 2 = done
+    state = [0] * n
+    look = [None] * n  # (t, acc)
+    started = 0
+    active = []
+    while started < n or active:
+        if started < n and (not active or rng.random() < 0.4):
+            active.append(started)
+            started += 1
+        g = active[rng.integers(len(active))]
+        sb = pair_first[g]
+        if state[g] == 0:
+            if g == sb:
+                status[g] = ("I", counts[g])
+                prefix[g] = 0
+                state[g] = 2
+                active.remove(g)
+            else:
+                status[g] = ("A", counts[g])
+                state[g] = 1
+                look[g] = (g - 1, 0)
+        elif state[g] == 1:
+            t, acc = look[g]
+            window = [status[t - l] if l <= t - sb else ("I", 0) for l in range(W)]
+            incl = [i for i, s in enumerate(window) if s is not None and s[0] == "I"]
+            waiting = [i for i, s in enumerate(window) if s is None]
+            if incl:
+                f = incl[0]
+                if not [i for i in waiting if i <= f]:
+                    prefix[g] = acc + sum(window[i][1] for i in range(f + 1))
+                    status[g] = ("I", prefix[g] + counts[g])
+                    state[g] = 2
+                    active.remove(g)
+            elif not waiting:
+                look[g] = (t - W, acc + sum(s[1] for s in window))
+    for g in range(n):
+        sb = pair_first[g]
+        assert prefix[g] == sum(counts[sb:g])
+    return prefix
+
+
+def make_pairs(rng, shapes):
     pairs = []
     for na, nb, span in shapes:
-        A = np.unique(rng.integers(0, span, na)).astype(np.uint32)
-        B = np.unique(rng.integers(0, span, nb)).astype(np.uint32)
-        pairs.append((A, B) if seed % 2 else (B, A))
-    got = intersect_model(pairs)
+        A = np.unique(rng.integers(0, span, na, dtype=np.int64)).astype(np.uint32)
+        B = np.unique(rng.integers(0, span, nb, dtype=np.int64)).astype(np.uint32)
+        pairs.append((A, B))
+    return pairs
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_segmented_intersection_model(seed):
+    rng = np.random.default_rng(seed)
+    shapes = [(0, 10, 50), (1, 1, 2), (5000, 5200, 20000), (3000, 3000, 3000), (4096, 4096, 8192),
+              (1, 9000, 9000), (8191, 8193, 18000), (300, 12000, 30000), (257, 9000, 9000), (20, 0, 5),
+              (6000, 7000, 2_000_000), (2000, 2500, 10 ** 9), (700, 30000, 30000), (12000, 11000, 40000)]
+    pairs = make_pairs(rng, shapes)
+    if seed % 2:
+        pairs = [(b, a) for a, b in pairs]
+    got, stats = intersect_model(pairs, rng)
+    for (A, B), g in zip(pairs, got):
+        assert np.array_equal(g, np.intersect1d(A, B))
+    # all three segment kinds ran, and the cut searches took few rounds
+    assert stats["dense"] and stats["sparse"] and stats["skew"]
+    assert stats["rounds"] <= 4
+
+
+def test_edge_values_and_equal_lists():
+    rng = np.random.default_rng(5)
+    top = np.uint32(U32)
+    pairs = [(np.arange(9000, dtype=np.uint32), np.arange(9000, dtype=np.uint32)),
+             (np.array([0, 5, 7, top], np.uint32), np.array([0, 6, 7, top], np.uint32)),
+             (np.arange(U32 - 6000, U32, 2, dtype=np.int64).astype(np.uint32),
+              np.append(np.arange(U32 - 7000, U32, 3, dtype=np.int64), U32).astype(np.uint32)),
+             # a value 0 in B against an A part starting at 0 (zeros read past a chunk's end are masked)
+             (np.arange(0, 700, 1, dtype=np.uint32), np.arange(0, 1400, 2, dtype=np.uint32)),
+             # dense A part next to a sparse one in the same pair
+             (np.concatenate([np.arange(3000, dtype=np.uint32), np.arange(10 ** 8, 10 ** 8 + 3000 * 50000, 50000,
+                                                                         dtype=np.int64).astype(np.uint32)]),
+              np.arange(0, 10 ** 8 + 3000 * 50000, 25000, dtype=np.int64).astype(np.uint32))]
+    got, stats = intersect_model(pairs, rng)
     for (A, B), g in zip(pairs, got):
         assert np.array_equal(g, np.intersect1d(A, B))
 
 
-def test_tile_plan_counts():
-    A = np.arange(10, dtype=np.uint32)
-    pairs = [(A, np.arange(400, dtype=np.uint32)), (np.arange(5000, dtype=np.uint32), np.arange(5000, dtype=np.uint32)),
-             (np.zeros(0, np.uint32), A)]
-    begin, tile_pair = plan(pairs)
-    # 400 >= 32 * 10: one skew tile; 10000 merge-path elements: 3 tiles; empty: none
-    assert begin == [0, 1, 4, 4]
-    assert tile_pair == [0, 1, 1, 1]
-
-
-def size_class(nt):
-    """isect_size_class: 16 * (bit length - 1) + the four bits after the
-    leading one."""
-    b = int(nt).bit_length() - 1
-    frac = (nt >> (b - 4)) & 15 if b >= 4 else (nt << (4 - b)) & 15
-    return 16 * b + frac
-
-
-def test_size_classes_are_ordered():
-    # monotone in the tile count, exact below 32, within 7% of each other
-    # inside a class, and within 0..511 for every u32 count
-    prev = -1
-    for nt in list(range(1, 5000)) + [2**31 - 1, 2**31, 2**32 - 1]:
-        c = size_class(nt)
-        assert 0 <= c <= 511 and c >= prev
-        assert nt >= 32 or c > prev
-        prev = c
-    for c in range(0, 16 * 14):
-        members = [nt for nt in range(1, 1 << 14) if size_class(nt) == c]
-        if members:
-            assert max(members) < 1.07 * min(members)
-
-
-def test_equal_pair_counts_per_block_balance_the_blocks():
-    # the host's grid rule (every block takes the same number of pairs)
-    # with blocks taking pairs largest first in size-class order, on the
-    # benchmark's pair sizes: the busiest block ends within 5% of the mean
-    import heapq
+def test_benchmark_shape_segments_are_dense():
+    """The benchmark's lists (gaps 1..4) make every merge segment dense: the
+    A part of 8,192 path elements spans fewer than 16,384 values."""
     rng = np.random.default_rng(7)
-    nt = -(-(rng.integers(16384, 49152, 2048) + rng.integers(16384, 49152, 2048)) // XT)
-    resident = 1280
-    rounds = -(-len(nt) // resident)
-    grid = -(-len(nt) // rounds)
-    assert grid == 1024
-    order = sorted(range(len(nt)), key=lambda p: -size_class(int(nt[p])))
-    h = [(0, i) for i in range(grid)]
-    for p in order:
-        x, i = heapq.heappop(h)
-        heapq.heappush(h, (x + int(nt[p]), i))
-    assert max(x for x, _ in h) < 1.05 * nt.sum() / grid
+    stats_total = {"dense": 0, "sparse": 0}
+    for _ in range(3):
+        la, lb = (int(x) for x in rng.integers(16384, 49152, 2))
+        A = np.cumsum(rng.integers(1, 5, la)).astype(np.uint32)
+        B = np.cumsum(rng.integers(1, 5, lb)).astype(np.uint32)
+        got, stats = intersect_model([(A, B)], rng)
+        assert np.array_equal(got[0], np.intersect1d(A, B))
+        stats_total["dense"] += stats["dense"]
+        stats_total["sparse"] += stats["sparse"]
+    assert stats_total["sparse"] == 0 and stats_total["dense"] > 0
 
 
-def device_plan(lens, cap):
-    """fm_isect_plan_kernel: tile_begin by a scan of the tile counts (each
-    entry at most cap), the pairs with tiles ordered by the size class of
-    their tile count, largest first, and the overflow flag (no work) when
-    the tiles exceed cap."""
-    counts = []
-    for la, lb in lens:
-        na, nb = sorted((la, lb))
-        counts.append(0 if na == 0 else (-(-na // IB) if nb >= ISKEW * na else -(-(na + nb) // XT)))
-    begin, acc = [], 0
-    for c in counts:
-        begin.append(min(acc, cap))
-        acc += c
-    begin.append(min(acc, cap))
-    over = acc > cap
-    buckets = {}
-    for p, c in enumerate(counts):
-        if c:
-            buckets.setdefault(511 - size_class(c), []).append(p)
-    order = [p for b in sorted(buckets) for p in buckets[b]]
-    return begin, order, (0 if over else len(order)), over
+def test_segment_counts_and_plan():
+    lens = [(10, 400), (9000, 9000), (0, 10), (4096, 0), (1, 31)]
+    # 400 >= 32 * 10: one skew segment; 18,000 path elements: 3 segments; empty lists: none; 31 < 32: merge
+    assert [segments(a, b) for a, b in lens] == [1, 3, 0, 0, 1]
+    begin, total, over = plan(lens, 100)
+    assert begin == [0, 1, 4, 4, 4, 5] and total == 5 and not over
+    begin, total, over = plan(lens, 3)
+    assert over and total == 0 and all(b <= 3 for b in begin)
 
 
-def test_device_plan_equals_the_host_plan():
-    rng = np.random.default_rng(9)
-    pairs = []
+def test_find_pair_equals_searchsorted():
+    rng = np.random.default_rng(3)
+    for npairs in (1, 2, 63, 64, 65, 2048, 5000):
+        nseg = rng.integers(0, 5, npairs)
+        begin = np.concatenate([[0], np.cumsum(nseg)]).tolist()
+        for g in rng.integers(0, max(begin[-1], 1), 50):
+            if begin[-1] == 0:
+                break
+            p = find_pair(begin, npairs, int(g))
+            assert p == int(np.searchsorted(begin, g, side="right")) - 1
+            assert begin[p] <= g < begin[p + 1]
+
+
+def test_first_false_equals_bisect():
+    rng = np.random.default_rng(4)
     for _ in range(300):
-        na, nb = (int(x) for x in rng.integers(0, 20000, 2))
-        if rng.random() < 0.2:
-            nb = na * 40
-        pairs.append((np.zeros(na), np.zeros(nb)))
-    lens = [(len(a), len(b)) for a, b in pairs]
-    host_begin, host_tile_pair = plan(pairs)
-    begin, order, nwork, over = device_plan(lens, len(host_tile_pair))
-    assert begin == host_begin and not over and nwork == len(order)
-    # every pair with tiles once, largest tile count (by size class) first
-    assert sorted(order) == sorted(set(host_tile_pair))
-    classes = [size_class(int(begin[p + 1] - begin[p])) for p in order]
-    assert classes == sorted(classes, reverse=True)
+        lo = int(rng.integers(0, 1000))
+        hi = lo + int(rng.integers(0, 200000))
+        split = int(rng.integers(lo, hi + 1))
+        got, rounds = first_false(lo, hi, lambda i: i < split)
+        assert got == split
+        assert rounds <= 4 or hi - lo > 32 ** 4
 
 
-def test_device_plan_flags_too_many_tiles_and_stays_in_the_array():
-    lens = [(5000, 6000)] * 10
-    begin, order, nwork, over = device_plan(lens, 20)
-    assert over and nwork == 0
-    assert all(b <= 20 for b in begin)  # cut[t] / cut2[t] are only read for t < cap
+def test_merge_cuts_match_the_merge():
+    rng = np.random.default_rng(6)
+    for _ in range(50):
+        A = np.unique(rng.integers(0, 5000, int(rng.integers(0, 3000))))
+        B = np.unique(rng.integers(0, 5000, int(rng.integers(0, 3000))))
+        merged = sorted([(int(x), 0) for x in A] + [(int(x), 1) for x in B])  # A first on ties
+        for d in rng.integers(0, len(A) + len(B) + 1, 10):
+            i, _ = merge_cut(A, B, int(d))
+            assert i == sum(1 for _, src in merged[:int(d)] if src == 0)
