@@ -11,28 +11,30 @@
 //    every match is found in the segment that holds its A element;
 //  - skew segments (the longer list at least ISX_SKEW times the shorter):
 //    ISX_CHUNK elements of A, looked up in B.
-// One wave works on one segment at a time, taken from a global counter (so
-// segments are taken in pair and path order):
+// One wave works on one segment at a time, claimed from sharded counters
+// (fm_isx_kernel explains the order and why every wait ends):
 //  1. its pair by a 64-ary search of the pairs' first segments, and its cuts
 //     (the two merge-path splits, one per half wave, 32-ary: four rounds of
 //     loads for lists of 10^5 elements);
-//  2. dense segments (A part values spanning fewer than 16,384 integers):
-//     the A part becomes a bitmap in the wave's LDS (one atomic OR per
-//     element) and a B element x matches when x lies in the A part's range
-//     and its bit is set (one LDS read). The A part and then the B part are
-//     streamed in 512-element chunks, four chunks in flight per wave;
-//     sparser segments hold one A chunk at a time sorted in LDS and search
-//     each B element there (9 LDS reads), advancing A chunks and B windows by
-//     their last values as a merge does; skew segments find the B range of
-//     their A chunk (two 32-ary searches) and binary-search each A element
-//     in it. Matches are compacted in order (ballot + mbcnt) into the wave's
-//     LDS buffer;
+//  2. dense segments (A part values spanning fewer than BMSPAN = 32,768
+//     integers): the A part and then the B part are streamed in 512-element
+//     chunks, four in flight per wave (16-byte buffer loads), into two LDS
+//     bitmaps over the A part's range (A's elements; B's elements in that
+//     range); the count is the popcount of their AND and the matches are the
+//     values of its set bits. Sparser segments hold one A chunk at a time
+//     sorted in LDS and search each B element there (9 LDS reads), advancing
+//     A chunks and B windows by their last values as a merge does; skew
+//     segments find the B range of their A chunk (two 32-ary searches) and
+//     binary-search each A element in it;
 //  3. the number of matches in the pair's earlier segments comes from
-//     decoupled look-back over per-segment status words (the segment's
-//     count is published first, then the inclusive sum), and the matches are
-//     stored at that offset with coalesced stores. Count-only calls add each
+//     decoupled look-back over per-segment status words (the segment's count
+//     is published first, then the inclusive sum), and the matches are
+//     stored at that offset with coalesced stores; a dense segment's are
+//     written after the wave has built its next segment, so the look-back
+//     mostly finds the earlier counts ready. Count-only calls add each
 //     segment's count to the pair's count instead.
-// Every element of a merge segment is read from HBM once.
+// Every element of a dense merge segment is read from HBM once; a sparse
+// segment with output reads its lists twice (to count, then to write).
 #include "isect.hpp"
 
 #include <algorithm>
@@ -44,9 +46,10 @@ constexpr uint32_t W = 64;
 constexpr uint32_t K = 8;                // registers per lane of a chunk
 constexpr uint32_t CH = K * W;           // elements per chunk
 constexpr uint32_t P = 4;                // chunks in flight per wave (dense segments)
-constexpr uint32_t BMW = 512;            // LDS words per wave: bitmap or sorted A chunk
+constexpr uint32_t BMW = 1024;           // LDS words of a dense segment's bitmap
 constexpr uint32_t BMSPAN = BMW * 32;    // dense: the A part spans fewer values
-constexpr uint32_t WPB = 4;              // waves per block
+constexpr uint32_t WPB = 2;              // waves per block
+constexpr uint32_t RUN = 16;             // consecutive segments per claim-class run
 // A look-back wait gives up after this long (s_memrealtime runs at 100 MHz).
 constexpr unsigned long long WAIT_TICKS = 200000000ull;
 static_assert(ISX_CHUNK == CH, "a skew segment is one chunk");
@@ -86,6 +89,14 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
 }
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v) { return rdl(wave_incl_scan(v), W - 1); }
 
+// A load of a value that no kernel of the call writes while it reads it,
+// through the constant address space: with a uniform address it becomes a
+// scalar load (tracked apart from the vector loads in flight).
+template <class T>
+__device__ __forceinline__ T ldc(const T *p) {
+    return *(const __attribute__((address_space(4))) T *)p;
+}
+
 struct Pair {
     const uint32_t *A, *B;
     uint32_t na, nb;
@@ -93,7 +104,8 @@ struct Pair {
 
 __device__ __forceinline__ Pair pair_of(const uint32_t *data, const uint64_t *a_off, const uint32_t *a_len,
                                         const uint64_t *b_off, const uint32_t *b_len, uint32_t p) {
-    Pair r{data + a_off[p], data + b_off[p], a_len[p], b_len[p]};
+    p = __builtin_amdgcn_readfirstlane(p);
+    Pair r{data + ldc(a_off + p), data + ldc(b_off + p), ldc(a_len + p), ldc(b_len + p)};
     if (r.na > r.nb) {
         const uint32_t *t = r.A;
         r.A = r.B;
@@ -508,12 +520,15 @@ __global__ __launch_bounds__(1024) void fm_isx_plan_kernel(const uint32_t *a_len
     }
 }
 
-// Claims: segment g belongs to class g % ncls (ncls <= ISX_CLASSES), whose
+// Claims: the segments are dealt to ncls <= ISX_CLASSES classes in runs of
+// RUN consecutive segments (run r to class r % ncls), and each class's
 // counter hands out its segments in order; a wave claims from the class of
 // its block (blocks are dispatched to the XCDs round-robin, so each XCD
-// claims mostly from its own counter) and, once that class is exhausted,
-// from the others in turn. One counter for all segments serialised the
-// claims (about 70 per microsecond on one address).
+// claims mostly from its own counter, and neighbouring segments of a run,
+// whose merge-path cut searches read each other's boundary lines, share an
+// L2) and, once that class is exhausted, from the others in turn. One
+// counter for all segments serialised the claims (about 70 per microsecond
+// on one address).
 // With output (OUT) a dense segment's bitmaps stay in LDS while the wave
 // builds its next segment, and only then is its offset resolved and its
 // matches written, so the look-back finds the earlier segments' counts
@@ -537,7 +552,7 @@ __global__ __launch_bounds__(W *WPB) void fm_isx_kernel(const uint32_t *__restri
     constexpr uint32_t NBM = OUT ? 3 : 2;
     constexpr uint32_t WAVE_WORDS = NBM * BMW + CH;
     __shared__ uint32_t s_lds[WPB][WAVE_WORDS];
-    const uint32_t wv = threadIdx.x / W, l = lane();
+    const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x / W), l = lane();
     uint32_t *const lds = s_lds[wv];
     uint32_t *const cs = lds + NBM * BMW;
     const uint32_t total = __builtin_amdgcn_readfirstlane(ld_rlx(&work[ISX_TOTAL]));
@@ -562,19 +577,19 @@ __global__ __launch_bounds__(W *WPB) void fm_isx_kernel(const uint32_t *__restri
             uint32_t k = 0;
             if (l == 0) k = __hip_atomic_fetch_add(&work[ISX_CLAIM + 32 * cls], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             k = rdl(k, 0);
-            const unsigned long long gg = cls + (unsigned long long)ncls * k;
+            const unsigned long long gg = ((unsigned long long)(k / RUN) * ncls + cls) * RUN + k % RUN;
             if (gg < total) {
-                g = (uint32_t)gg;
+                g = __builtin_amdgcn_readfirstlane((uint32_t)gg);
                 break;
             }
             cls = cls + 1 == ncls ? 0u : cls + 1;  // this class is exhausted
             tried++;
         }
         if (g >= total) break;
-        const uint32_t p = find_pair(seg_begin, npairs, g);
-        const uint32_t sb = seg_begin[p], s = g - sb, last = g + 1 == seg_begin[p + 1];
+        const uint32_t p = __builtin_amdgcn_readfirstlane(find_pair(seg_begin, npairs, g));
+        const uint32_t sb = ldc(seg_begin + p), s = g - sb, last = g + 1 == ldc(seg_begin + p + 1);
         const Pair q = pair_of(data, a_off, a_len, b_off, b_len, p);
-        const uint64_t ooff = OUT ? out_off[p] : 0ull;
+        const uint64_t ooff = OUT ? ldc(out_off + p) : 0ull;
         if ((uint64_t)q.nb >= (uint64_t)ISX_SKEW * q.na) {
             // skew segment: CH elements of A, found in B and kept in registers
             const uint32_t first = s * CH;
@@ -607,7 +622,7 @@ __global__ __launch_bounds__(W *WPB) void fm_isx_kernel(const uint32_t *__restri
         const uint32_t j0 = (uint32_t)(d0 - i0), j1 = (uint32_t)(d1 - i1);
         const uint32_t jx = min(j1 + 1, q.nb);
         const uint32_t la = i1 - i0, lb = jx - j0;
-        const uint32_t amin = la ? A[i0] : 0u, span = la ? A[i1 - 1] - amin : 0u;
+        const uint32_t amin = la ? ldc(A + i0) : 0u, span = la ? ldc(A + i1 - 1) - amin : 0u;
         if (span < BMSPAN) {
             const uint32_t sa = OUT ? (pslot + 1) % 3 : 0u, sbm = OUT ? (pslot + 2) % 3 : 1u;
             const uint32_t c = dense_build<OUT>(A + i0, la, B + j0, lb, amin, span, lds + sa * BMW, lds + sbm * BMW);

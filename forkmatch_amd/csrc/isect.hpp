@@ -10,7 +10,7 @@
 namespace fm {
 
 // Merge-path elements per merge segment.
-constexpr uint32_t ISX_SEG = 8192;
+constexpr uint32_t ISX_SEG = 16384;
 // Elements of the shorter list per skew segment (a pair whose longer list is
 // at least ISX_SKEW times the shorter one).
 constexpr uint32_t ISX_CHUNK = 512;

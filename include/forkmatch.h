@@ -449,14 +449,15 @@ int fm_intersect_pairs(fm_engine *e, const uint32_t *data,
 
 /* No counterpart in the reference. For callers that cannot allocate device
  * memory: code under HIP graph capture, tight serving loops, harnesses that
- * forbid hipMalloc. fm_intersect_pairs with its work array (claim counter,
- * pair order, tile starts and cuts) in a fixed device array of the library
- * (FM_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends) instead of
- * engine memory; it calls no hipMalloc, hipFree or other allocating routine
- * (the engine, created beforehand, provides the device and stream). Limits:
- * FM_NOALLOC_MAX_PAIRS pairs, and the pairs' tiles (3,840 merge-path
- * elements each, or 256 elements of the shorter list of a pair whose longer
- * list is 32x longer) must fit the array: about 458k tiles at 65,536 pairs.
+ * forbid hipMalloc. fm_intersect_pairs with its work array (claim counters,
+ * the pairs' first segments, one status word per segment) in a fixed device
+ * array of the library (FM_NOALLOC_SCRATCH_BYTES per GPU, held until the
+ * process ends) instead of engine memory; it calls no hipMalloc, hipFree or
+ * other allocating routine (the engine, created beforehand, provides the
+ * device and stream). Limits: FM_NOALLOC_MAX_PAIRS pairs, and the pairs'
+ * segments (16,384 merge-path elements each, or 512 elements of the shorter
+ * list of a pair whose longer list is 32x longer) must fit the array: about
+ * 491k segments at 65,536 pairs, 524k at one pair.
  * Beyond either limit it returns FM_ERR_INVALID_ARG (counts are then zero).
  * Every call on one GPU uses that GPU's one array: calls that overlap on one
  * GPU (from several engines or threads) race. */

@@ -6,14 +6,14 @@ first-segment table (clamped to the work array, overflow flagged); a wave's
 pair lookup (64 samples per round); the two merge-path cuts of a segment
 searched by half waves with 32 samples per round; a merge segment's A part
 A[i0, i1) and B part B[j0, j1] (one B element past the cut); dense segments
-(A part values spanning < 16,384 integers): chunks loaded as two 16-byte
+(A part values spanning < 32,768 integers): chunks loaded as two 16-byte
 loads per lane (lane l holds elements 4l..4l+3 and 256+4l..256+4l+3) through
 a buffer range check (zeros past a chunk's end, masked by index in partial
 chunks); each lane's group of four elements sets its bits with two ORs (the
 word of the group's first element and the next one) and, when the group
 spans further or its first element is out of range, one OR per element; the
 A bitmap, and the bitmap of the B elements inside the A part's range, in
-512-word regions whose words past the cleared range hold stale bits; the
+1,024-word regions whose words past the cleared range hold stale bits; the
 count as the popcount of their AND, and the matches written from the AND
 in 64-word rounds through a 512-word stage; sparse segments: one A chunk at
 a time sorted in LDS (clamped copies of its last element after its end), B
@@ -28,8 +28,8 @@ import pytest
 
 W, K = 64, 8
 CH = K * W
-SEG = 8192
-BMW = 512
+SEG = 16384
+BMW = 1024
 BMSPAN = BMW * 32
 SKEW = 32
 U32 = 0xFFFFFFFF
@@ -357,7 +357,8 @@ def test_segmented_intersection_model(seed):
     rng = np.random.default_rng(seed)
     shapes = [(0, 10, 50), (1, 1, 2), (5000, 5200, 20000), (3000, 3000, 3000), (4096, 4096, 8192),
               (1, 9000, 9000), (8191, 8193, 18000), (300, 12000, 30000), (257, 9000, 9000), (20, 0, 5),
-              (6000, 7000, 2_000_000), (2000, 2500, 10 ** 9), (700, 30000, 30000), (12000, 11000, 40000)]
+              (6000, 7000, 2_000_000), (2000, 2500, 10 ** 9), (700, 30000, 30000), (12000, 11000, 40000),
+              (20000, 25000, 60000)]
     pairs = make_pairs(rng, shapes)
     if seed % 2:
         pairs = [(b, a) for a, b in pairs]
@@ -389,7 +390,7 @@ def test_edge_values_and_equal_lists():
 
 def test_benchmark_shape_segments_are_dense():
     """The benchmark's lists (gaps 1..4) make every merge segment dense: the
-    A part of 8,192 path elements spans fewer than 16,384 values."""
+    A part of 16,384 path elements spans fewer than 32,768 values."""
     rng = np.random.default_rng(7)
     stats_total = {"dense": 0, "sparse": 0}
     for _ in range(3):
@@ -405,10 +406,10 @@ def test_benchmark_shape_segments_are_dense():
 
 def test_segment_counts_and_plan():
     lens = [(10, 400), (9000, 9000), (0, 10), (4096, 0), (1, 31)]
-    # 400 >= 32 * 10: one skew segment; 18,000 path elements: 3 segments; empty lists: none; 31 < 32: merge
-    assert [segments(a, b) for a, b in lens] == [1, 3, 0, 0, 1]
+    # 400 >= 32 * 10: one skew segment; 18,000 path elements: 2 segments; empty lists: none; 31 < 32: merge
+    assert [segments(a, b) for a, b in lens] == [1, 2, 0, 0, 1]
     begin, total, over = plan(lens, 100)
-    assert begin == [0, 1, 4, 4, 4, 5] and total == 5 and not over
+    assert begin == [0, 1, 3, 3, 3, 4] and total == 4 and not over
     begin, total, over = plan(lens, 3)
     assert over and total == 0 and all(b <= 3 for b in begin)
 

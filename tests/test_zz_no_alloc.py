@@ -1,5 +1,6 @@
 """fm_intersect_pairs_noalloc (the entry point that allocates nothing: its
-work counter and pair order live in the library's fixed device array)
+claim counters, segment table and status words live in the library's
+fixed device array)
 against fm_intersect_pairs, through the C ABI: the same counts and the same
 intersections, bit for bit, on two calls in a row, the second with fewer
 and shorter pairs (so state left in the fixed array by the first call must
