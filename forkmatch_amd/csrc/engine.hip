@@ -3598,9 +3598,8 @@ static int isect_run(fm_engine *e, uint32_t *work, uint32_t cap, const uint32_t 
     if (over) *over = false;
     if (npairs == 0) return FM_OK;
     HIP_TRY(hipSetDevice(e->device));
-    const uint32_t grid = (uint32_t)std::max(1, isx_blocks_per_cu() * e->cu_count);
     HIP_TRY(hipEventRecord(e->ev0, e->stream));
-    isx_launch(e->stream, grid, work, cap, data, a_off, a_len, b_off, b_len, npairs, counts, out_off, out);
+    isx_launch(e->stream, (uint32_t)e->cu_count, work, cap, data, a_off, a_len, b_off, b_len, npairs, counts, out_off, out);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(e->ev1, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));

@@ -48,7 +48,13 @@ constexpr uint32_t CH = K * W;           // elements per chunk
 constexpr uint32_t P = 4;                // chunks in flight per wave (dense segments)
 constexpr uint32_t BMW = 1024;           // LDS words of a dense segment's bitmap
 constexpr uint32_t BMSPAN = BMW * 32;    // dense: the A part spans fewer values
-constexpr uint32_t WPB = 2;              // waves per block
+constexpr uint32_t WPB_COUNT = 2;        // waves per block, count only
+constexpr uint32_t NB = 2;               // builder waves per block with output (and one emitter)
+constexpr uint32_t NS = 4;               // handoff slots per block with output
+constexpr uint32_t NE = 1;               // emitter waves per block with output
+// Count-only blocks per CU: more concurrent streams read slower (measured:
+// 8 waves per CU beat 10, 12 and 16)
+constexpr int COUNT_BLOCKS_PER_CU = 4;
 constexpr uint32_t RUN = 16;             // consecutive segments per claim-class run
 // A look-back wait gives up after this long (s_memrealtime runs at 100 MHz).
 constexpr unsigned long long WAIT_TICKS = 200000000ull;
@@ -96,6 +102,11 @@ template <class T>
 __device__ __forceinline__ T ldc(const T *p) {
     return *(const __attribute__((address_space(4))) T *)p;
 }
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+// four words at a 4-byte-aligned address (global accesses of several dwords
+// need only dword alignment)
+typedef uint32_t u32x4_a4 __attribute__((ext_vector_type(4), aligned(4)));
 
 struct Pair {
     const uint32_t *A, *B;
@@ -184,17 +195,21 @@ __device__ __forceinline__ uint32_t emit(bool m, uint32_t x, uint32_t c, uint32_
     return c + (uint32_t)__popcll(mk);
 }
 
+// Cache policy of the streamed chunk loads: non-temporal (each list element
+// is read once; keeping them out of the way of the L2's other lines measured
+// 8% faster).
+constexpr int AUX_NT = 2;
+
 // One chunk of n elements from p for the dense path, two 16-byte loads per
 // lane through a buffer resource: lane l holds elements 4l..4l+3 in r[0..3]
 // and 256+4l..256+4l+3 in r[4..7]; elements at or past n read as 0 (the
 // buffer's range check, per dword; nothing is fetched for them).
 __device__ __forceinline__ void load_chunk_x4(const uint32_t *p, uint32_t n, uint32_t (&r)[K]) {
-    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
     const __amdgpu_buffer_rsrc_t rs =
         __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t *>(p), (short)0, (int)(min(n, CH) * 4), 0x00020000);
     const int voff = (int)(lane() * 16);
-    const u32x4 a = __builtin_amdgcn_raw_buffer_load_b128(rs, voff, 0, 0);
-    const u32x4 b = __builtin_amdgcn_raw_buffer_load_b128(rs, voff + (int)(CH / 2 * 4), 0, 0);
+    const u32x4 a = __builtin_amdgcn_raw_buffer_load_b128(rs, voff, 0, AUX_NT);
+    const u32x4 b = __builtin_amdgcn_raw_buffer_load_b128(rs, voff + (int)(CH / 2 * 4), 0, AUX_NT);
     r[0] = a.x, r[1] = a.y, r[2] = a.z, r[3] = a.w;
     r[4] = b.x, r[5] = b.y, r[6] = b.z, r[7] = b.w;
 }
@@ -247,14 +262,13 @@ __device__ __forceinline__ void set_bits(uint32_t *bm, const uint32_t (&r)[K], u
 
 // Dense merge segment: A part pa[0, la) with values in [amin, amin + span],
 // span < BMSPAN, and B part pb[0, lb). Builds the bitmap of the A part (bmA)
-// and of the B elements in its range (bmB), both span / 32 + 1 words, and
-// returns the number of common bits; with KEEP bmA ends as the common bits. The A chunks, then the B chunks, are
-// streamed P at a time: chunk c + P is loaded into the registers chunk c
-// leaves (chunks past the segment load nothing), so every load is issued P
-// chunks before its use and the load sequence is the same on every path.
-template <bool KEEP>
-__device__ __forceinline__ uint32_t dense_build(const uint32_t *pa, uint32_t la, const uint32_t *pb, uint32_t lb,
-                                                uint32_t amin, uint32_t span, uint32_t *bmA, uint32_t *bmB) {
+// and of the B elements in its range (bmB), both span / 32 + 1 words. The A
+// chunks, then the B chunks, are streamed P at a time: chunk c + P is loaded
+// into the registers chunk c leaves (chunks past the segment load nothing),
+// so every load is issued P chunks before its use and the load sequence is
+// the same on every path.
+__device__ __forceinline__ void dense_build(const uint32_t *pa, uint32_t la, const uint32_t *pb, uint32_t lb,
+                                            uint32_t amin, uint32_t span, uint32_t *bmA, uint32_t *bmB) {
     const uint32_t l = lane();
     const uint32_t nA = (la + CH - 1) / CH, nC = nA + (lb + CH - 1) / CH;
     const uint32_t nw = la ? (span >> 5) + 1 : 0u;
@@ -288,10 +302,16 @@ __device__ __forceinline__ uint32_t dense_build(const uint32_t *pa, uint32_t la,
             chunk(cc + P, r[s]);
         }
     }
+}
+
+// The number of common bits of the nw words of bmA and bmB; with WRITE the
+// common bits are also written to dst (which may be bmA).
+template <bool WRITE>
+__device__ __forceinline__ uint32_t common_bits(uint32_t *bmA, const uint32_t *bmB, uint32_t nw, uint32_t *dst) {
     uint32_t c = 0;
-    for (uint32_t w = l; w < nw; w += W) {
+    for (uint32_t w = lane(); w < nw; w += W) {
         const uint32_t x = bmA[w] & bmB[w];
-        if (KEEP) bmA[w] = x;
+        if (WRITE) dst[w] = x;
         c += (uint32_t)__popc(x);
     }
     return wave_sum(c);
@@ -320,10 +340,19 @@ __device__ __forceinline__ void dense_emit(const uint32_t *bm, uint32_t nw, uint
                 pos++;
                 b &= b - 1;
             }
-            const uint32_t cnt = min(CH, tot - lo);
-            for (uint32_t i = l; i < cnt; i += W) {
-                const uint32_t at = base + lo + i;
-                if (at < lim) o[at] = stage[i];
+            // the stage leaves in non-temporal 16-byte stores (whole groups
+            // of four), the group at its end or at lim dword by dword
+            const uint32_t cnt = min(CH, tot - lo), at0 = base + lo;
+            for (uint32_t i = 4 * l; i < cnt; i += 4 * W) {
+                const u32x4 v = *reinterpret_cast<const u32x4 *>(stage + i);
+                const uint32_t at = at0 + i;
+                if (i + 4 <= cnt && at + 4 <= lim) {
+                    __builtin_nontemporal_store(v, reinterpret_cast<u32x4_a4 *>(o + at));
+                } else {
+#pragma unroll
+                    for (uint32_t e = 0; e < 4; e++)
+                        if (i + e < cnt && at + e < lim) o[at + e] = v[e];
+                }
             }
         }
         base += tot;
@@ -473,6 +502,41 @@ __device__ __forceinline__ uint32_t resolve(unsigned long long *status, uint32_t
     return prefix;
 }
 
+// One look-back attempt for segment g (its count c published) that does not
+// wait: true with the count of the pair's segments before g in prefix (and
+// g's inclusive sum published) when every segment it needs has published,
+// false otherwise.
+__device__ __forceinline__ bool try_resolve(unsigned long long *status, uint32_t g, uint32_t sb, uint32_t epoch,
+                                            uint32_t c, uint32_t &prefix) {
+    const unsigned long long tag = (unsigned long long)epoch << 34;
+    const uint32_t l = lane();
+    prefix = 0;
+    if (g != sb) {
+        uint32_t t = g - 1;  // lane k reads segment t - k
+        while (true) {
+            const bool in = l <= t - sb;
+            const unsigned long long st =
+                in ? __hip_atomic_load(&status[t - l], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : (tag | ST_INCL);
+            const bool ready = (uint32_t)(st >> 34) == epoch && (st & (ST_AGG | ST_INCL)) != 0;
+            const unsigned long long incl = __ballot(ready && (st & ST_INCL)), waiting = __ballot(!ready);
+            if (incl) {
+                const uint32_t f = (uint32_t)__ffsll((long long)incl) - 1;  // the nearest inclusive sum
+                const unsigned long long upto = f == 63 ? ~0ull : ((2ull << f) - 1);
+                if (waiting & upto) return false;
+                prefix += wave_sum(l <= f ? (uint32_t)st : 0u);
+                break;
+            }
+            if (waiting) return false;
+            prefix += wave_sum((uint32_t)st);  // 64 published counts, none inclusive
+            t -= W;
+        }
+        if (l == 0)
+            __hip_atomic_store(&status[g], tag | ST_INCL | (uint32_t)(prefix + c), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return true;
+}
+
 // One block of 1,024 threads: every pair's first segment (a scan of the
 // segment counts), the counts cleared, the claim counters cleared, the
 // segment total (0 and the ISX_OVER flag when it exceeds cap), a new epoch.
@@ -520,64 +584,184 @@ __global__ __launch_bounds__(1024) void fm_isx_plan_kernel(const uint32_t *a_len
     }
 }
 
+// A dense segment handed from a builder wave to its block's emitter wave:
+// its common bits sit in a slot of BMW words.
+struct Handoff {
+    uint32_t g, sb, p, nw, amin, c, last, room;
+    uint64_t off;
+    unsigned long long since;  // when an emitter first found its look-back waiting (0: not yet)
+};
+constexpr uint32_t SLOT_FREE = 0, SLOT_FILLING = 1, SLOT_READY = 2, SLOT_EMITTING = 3;
+
+// A block's LDS. Count only: WPB_COUNT builder areas (two bitmaps, then CH
+// words for a sparse segment's sorted A chunk or a stage). With output: NB
+// builder areas, NS slots with their handoffs and states, the number of
+// builders that have finished, and a stage per emitter.
+template <bool OUT>
+struct BlockLds {
+    uint32_t builder[WPB_COUNT][2 * BMW + CH];
+};
+template <>
+struct BlockLds<true> {
+    uint32_t builder[NB][2 * BMW + CH];
+    uint32_t slot[NS][BMW];
+    Handoff handoff[NS];
+    uint32_t state[NS];
+    uint32_t done;
+    uint32_t stage[NE][CH];
+};
+
+// Ordering of the LDS handoffs between the waves of a block (LDS only).
+__device__ __forceinline__ void lds_release() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local"); }
+__device__ __forceinline__ void lds_acquire() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local"); }
+
+// A free slot taken for filling, or -1 when every slot is busy.
+__device__ __forceinline__ int take_slot(uint32_t *state) {
+    int s = -1;
+    if (lane() == 0) {
+#pragma unroll
+        for (uint32_t i = 0; i < NS; i++) {
+            uint32_t expect = SLOT_FREE;
+            if (__hip_atomic_compare_exchange_strong(&state[i], &expect, SLOT_FILLING, __ATOMIC_ACQUIRE,
+                                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
+                s = (int)i;
+                break;
+            }
+        }
+    }
+    return (int)rdl((uint32_t)s, 0);
+}
+
+// An emitter wave of a block (with output): goes round the slots, takes a
+// ready one (ready -> emitting), and when the look-back finds the segment's
+// offset without waiting writes its matches and, for a pair's last segment,
+// the pair's count, then frees the slot; otherwise it makes the slot ready
+// again for the next pass. A slot that waits longer than the look-back limit
+// is given up (the call reports it). Ends when every builder of the block has
+// finished and no slot is ready. Its stores never hold up a builder's loads
+// (each wave counts its own memory operations).
+__device__ void emitter(BlockLds<true> &sh, uint32_t *stage, unsigned long long *status, uint32_t epoch,
+                        uint32_t *work, uint32_t *counts, uint32_t *out) {
+    const uint32_t l = lane();
+    while (true) {
+        bool ready_any = false, progress = false;
+#pragma unroll
+        for (uint32_t i = 0; i < NS; i++) {
+            if (rdl(__hip_atomic_load(&sh.state[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP), 0) != SLOT_READY)
+                continue;
+            ready_any = true;
+            uint32_t took = 0;
+            if (l == 0) {
+                uint32_t expect = SLOT_READY;
+                took = __hip_atomic_compare_exchange_strong(&sh.state[i], &expect, SLOT_EMITTING, __ATOMIC_ACQUIRE,
+                                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+            if (!rdl(took, 0)) continue;
+            Handoff &h = sh.handoff[i];
+            const uint32_t g = __builtin_amdgcn_readfirstlane(h.g), sb = __builtin_amdgcn_readfirstlane(h.sb);
+            const uint32_t c = __builtin_amdgcn_readfirstlane(h.c);
+            uint32_t prefix = 0;
+            uint32_t next = SLOT_FREE;
+            if (!try_resolve(status, g, sb, epoch, c, prefix)) {
+                const unsigned long long now = __builtin_amdgcn_s_memrealtime(), since = h.since;
+                if (!since) {
+                    if (l == 0) h.since = now;
+                    next = SLOT_READY;
+                } else if (now - since <= WAIT_TICKS) {
+                    next = SLOT_READY;
+                } else if (l == 0) {
+                    atomicOr(&work[ISX_FLAGS], ISX_GAVE_UP);
+                }
+            } else {
+                const uint32_t room = __builtin_amdgcn_readfirstlane(h.room), nw = __builtin_amdgcn_readfirstlane(h.nw);
+                const uint32_t amin = __builtin_amdgcn_readfirstlane(h.amin), p = __builtin_amdgcn_readfirstlane(h.p);
+                const uint32_t last = __builtin_amdgcn_readfirstlane(h.last);
+                const uint64_t off = h.off;
+                if (c) dense_emit(sh.slot[i], nw, amin, stage, out + off + prefix, prefix < room ? room - prefix : 0u);
+                if (last && l == 0) counts[p] = prefix + c;
+                progress = true;
+            }
+            lds_release();
+            if (l == 0) __hip_atomic_store(&sh.state[i], next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+        if (progress) continue;
+        if (!ready_any &&
+            rdl(__hip_atomic_load(&sh.done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP), 0) == NB) {
+            lds_acquire();
+            bool busy = false;
+            for (uint32_t i = 0; i < NS; i++)
+                busy |= __hip_atomic_load(&sh.state[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == SLOT_READY;
+            if (!__ballot(busy)) break;
+            continue;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
 // Claims: the segments are dealt to ncls <= ISX_CLASSES classes in runs of
 // RUN consecutive segments (run r to class r % ncls), and each class's
-// counter hands out its segments in order; a wave claims from the class of
-// its block (blocks are dispatched to the XCDs round-robin, so each XCD
-// claims mostly from its own counter, and neighbouring segments of a run,
+// counter hands out its segments in order; a builder wave claims from the
+// class of its block (blocks are dispatched to the XCDs round-robin, so each
+// XCD claims mostly from its own counter, and neighbouring segments of a run,
 // whose merge-path cut searches read each other's boundary lines, share an
 // L2) and, once that class is exhausted, from the others in turn. One
 // counter for all segments serialised the claims (about 70 per microsecond
 // on one address).
-// With output (OUT) a dense segment's bitmaps stay in LDS while the wave
-// builds its next segment, and only then is its offset resolved and its
-// matches written, so the look-back finds the earlier segments' counts
-// published. A wave publishes the count of every segment it claims before it
-// waits on any other, and waits only on lower segments of the same pair.
-// Every wait ends: a claimed segment is published by a wave that is building
-// it; a wait on an unclaimed lower segment t of class j by a wave whose own
-// segment is of class i means counter_j <= t < own < counter_i, so such waits
-// form no cycle, and the waves of the class with the lowest counter (every
-// class has waves: ncls <= the grid's waves) wait only on claimed segments.
+// With output (OUT) a block has NB builder waves and NE emitter waves. A
+// builder hands a dense segment's common bits to the emitters through a free
+// slot and goes on to its next segment, so its loads are never held up by
+// stores of its own; with no free slot, and for sparse and skew segments, it
+// finds the offset and writes the matches itself.
+// A wave publishes the count of every segment it builds before it waits on
+// any other, waits only on lower segments of the same pair, and a builder
+// never waits for the emitter. Every wait ends: a claimed segment is
+// published by the builder that claimed it once it is built; a wait on an
+// unclaimed lower segment t of class j by a wave whose own segment is of
+// class i means counter_j <= t < own < counter_i, so such waits form no
+// cycle, and the builders of the class with the lowest counter (every class
+// has builders: ncls <= the grid's blocks) wait only on claimed segments.
 template <bool OUT>
-__global__ __launch_bounds__(W *WPB) void fm_isx_kernel(const uint32_t *__restrict__ data, const uint64_t *a_off,
-                                                         const uint32_t *a_len, const uint64_t *b_off,
-                                                         const uint32_t *b_len, uint32_t npairs, uint32_t *counts,
-                                                         const uint64_t *out_off, uint32_t *out, uint32_t *work,
-                                                         uint32_t ncls) {
-    // per wave: bitmaps of BMW words (with output three: the pending
-    // segment's common bits and the current segment's two; without, two),
-    // then CH words for a sparse segment's sorted A chunk or the stage of
-    // dense_emit
-    constexpr uint32_t NBM = OUT ? 3 : 2;
-    constexpr uint32_t WAVE_WORDS = NBM * BMW + CH;
-    __shared__ uint32_t s_lds[WPB][WAVE_WORDS];
+__global__ __launch_bounds__(W *(OUT ? NB + NE : WPB_COUNT)) void fm_isx_kernel(
+    const uint32_t *__restrict__ data, const uint64_t *a_off, const uint32_t *a_len, const uint64_t *b_off,
+    const uint32_t *b_len, uint32_t npairs, uint32_t *counts, const uint64_t *out_off, uint32_t *out, uint32_t *work,
+    uint32_t ncls) {
+    __shared__ BlockLds<OUT> sh;
     const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x / W), l = lane();
-    uint32_t *const lds = s_lds[wv];
-    uint32_t *const cs = lds + NBM * BMW;
     const uint32_t total = __builtin_amdgcn_readfirstlane(ld_rlx(&work[ISX_TOTAL]));
     const uint32_t epoch = __builtin_amdgcn_readfirstlane(ld_rlx(&work[ISX_EPOCH]));
     const uint32_t *const seg_begin = work + ISX_HDR;
     unsigned long long *const status = reinterpret_cast<unsigned long long *>(work + isx_status_word(npairs));
-    // the dense segment whose matches are not written yet (OUT): its common
-    // bits in bitmap pslot; the current segment uses the other two
-    bool pend = false;
-    uint32_t pg = 0, psb = 0, pp = 0, pnw = 0, pamin = 0, pc = 0, plast = 0, proom = 0, pslot = 0;
-    uint64_t poff = 0;
-    auto finish = [&]() {
-        const uint32_t prefix = resolve(status, pg, psb, epoch, pc, work);
-        dense_emit(lds + pslot * BMW, pnw, pamin, cs, out + poff + prefix, prefix < proom ? proom - prefix : 0u);
-        if (plast && l == 0) counts[pp] = prefix + pc;
-        pend = false;
-    };
+    if constexpr (OUT) {
+        if (threadIdx.x < NS) sh.state[threadIdx.x] = SLOT_FREE;
+        if (threadIdx.x == 0) sh.done = 0;
+        __syncthreads();
+        if (wv >= NB) {
+            emitter(sh, sh.stage[wv - NB], status, epoch, work, counts, out);
+            return;
+        }
+    }
+    uint32_t *const bmA = sh.builder[wv], *const bmB = bmA + BMW, *const cs = bmA + 2 * BMW;
     uint32_t cls = blockIdx.x % ncls, tried = 0;
     while (true) {
         uint32_t g = total;
         while (tried < ncls) {
+            // segment of the class's k-th claim
+            auto seg_of = [&](uint32_t k) { return ((unsigned long long)(k / RUN) * ncls + cls) * RUN + k % RUN; };
             uint32_t k = 0;
-            if (l == 0) k = __hip_atomic_fetch_add(&work[ISX_CLAIM + 32 * cls], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (l == 0) {
+                uint32_t *const ctr = &work[ISX_CLAIM + 32 * cls];
+                // after the home class, a class is read before it is claimed
+                // from: claims on exhausted classes by every wave at the end
+                // would queue on the counters' addresses
+                bool spent = false;
+                if (tried) {
+                    k = ld_rlx(ctr);
+                    spent = seg_of(k) >= total;
+                }
+                if (!spent) k = __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
             k = rdl(k, 0);
-            const unsigned long long gg = ((unsigned long long)(k / RUN) * ncls + cls) * RUN + k % RUN;
+            const unsigned long long gg = seg_of(k);
             if (gg < total) {
                 g = __builtin_amdgcn_readfirstlane((uint32_t)gg);
                 break;
@@ -601,7 +785,6 @@ __global__ __launch_bounds__(W *WPB) void fm_isx_kernel(const uint32_t *__restri
                 continue;
             }
             publish(status, g, sb, epoch, c);
-            if (pend) finish();
             const uint32_t prefix = resolve(status, g, sb, epoch, c, work);
             uint32_t *const o = out + ooff + prefix;
             const uint32_t lim = prefix < q.na ? q.na - prefix : 0u;
@@ -624,17 +807,30 @@ __global__ __launch_bounds__(W *WPB) void fm_isx_kernel(const uint32_t *__restri
         const uint32_t la = i1 - i0, lb = jx - j0;
         const uint32_t amin = la ? ldc(A + i0) : 0u, span = la ? ldc(A + i1 - 1) - amin : 0u;
         if (span < BMSPAN) {
-            const uint32_t sa = OUT ? (pslot + 1) % 3 : 0u, sbm = OUT ? (pslot + 2) % 3 : 1u;
-            const uint32_t c = dense_build<OUT>(A + i0, la, B + j0, lb, amin, span, lds + sa * BMW, lds + sbm * BMW);
-            if (!OUT) {
+            dense_build(A + i0, la, B + j0, lb, amin, span, bmA, bmB);
+            const uint32_t nw = la ? (span >> 5) + 1 : 0u;
+            if constexpr (!OUT) {
+                const uint32_t c = common_bits<false>(bmA, bmB, nw, nullptr);
                 if (l == 0 && c) __hip_atomic_fetch_add(&counts[p], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                continue;
+            } else {
+                const int slot = take_slot(sh.state);
+                const uint32_t c = common_bits<true>(bmA, bmB, nw, slot >= 0 ? sh.slot[slot] : bmA);
+                publish(status, g, sb, epoch, c);
+                if (!c && !last) {
+                    // nothing to write, and no pair count: the published count is all later segments need
+                    if (slot >= 0 && l == 0)
+                        __hip_atomic_store(&sh.state[slot], SLOT_FREE, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                } else if (slot >= 0) {
+                    if (l == 0) sh.handoff[slot] = Handoff{g, sb, p, nw, amin, c, last, q.na, ooff, 0ull};
+                    lds_release();
+                    if (l == 0)
+                        __hip_atomic_store(&sh.state[slot], SLOT_READY, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                } else {
+                    const uint32_t prefix = resolve(status, g, sb, epoch, c, work);
+                    if (c) dense_emit(bmA, nw, amin, cs, out + ooff + prefix, prefix < q.na ? q.na - prefix : 0u);
+                    if (last && l == 0) counts[p] = prefix + c;
+                }
             }
-            publish(status, g, sb, epoch, c);
-            if (pend) finish();
-            pend = true;
-            pg = g, psb = sb, pp = p, pnw = la ? (span >> 5) + 1 : 0u, pamin = amin, pc = c, plast = last;
-            proom = q.na, pslot = sa, poff = ooff;
             continue;
         }
         // sparse segment: counted first; its matches are found again and
@@ -645,34 +841,45 @@ __global__ __launch_bounds__(W *WPB) void fm_isx_kernel(const uint32_t *__restri
             continue;
         }
         publish(status, g, sb, epoch, c);
-        if (pend) finish();
         const uint32_t prefix = resolve(status, g, sb, epoch, c, work);
         if (c) sparse_segment<true>(A + i0, la, B + j0, lb, cs, out + ooff + prefix, prefix < q.na ? q.na - prefix : 0u);
         if (last && l == 0) counts[p] = prefix + c;
     }
-    if (OUT && pend) finish();
+    if constexpr (OUT) {
+        lds_release();
+        if (l == 0) __hip_atomic_fetch_add(&sh.done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+}
+
+template <bool OUT>
+int blocks_per_cu() {
+    int b = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, fm_isx_kernel<OUT>, W * (OUT ? NB + NE : WPB_COUNT), 0) !=
+            hipSuccess ||
+        b <= 0)
+        b = 1;
+    return b;
 }
 
 }  // namespace
 
-void isx_launch(hipStream_t stream, uint32_t grid, uint32_t *work, uint32_t cap, const uint32_t *data,
+void isx_launch(hipStream_t stream, uint32_t cus, uint32_t *work, uint32_t cap, const uint32_t *data,
                 const uint64_t *a_off, const uint32_t *a_len, const uint64_t *b_off, const uint32_t *b_len,
                 uint32_t npairs, uint32_t *counts, const uint64_t *out_off, uint32_t *out) {
     hipLaunchKernelGGL(fm_isx_plan_kernel, dim3(1), dim3(1024), 0, stream, a_len, b_len, npairs, counts, work, cap);
-    const uint32_t ncls = std::min<uint32_t>(ISX_CLASSES, std::max<uint32_t>(grid, 1u));
+    const uint32_t grid = std::max<uint32_t>(1u, (uint32_t)isx_blocks_per_cu(out_off != nullptr) * cus);
+    const uint32_t ncls = std::min<uint32_t>(ISX_CLASSES, grid);
     if (out_off)
-        hipLaunchKernelGGL(fm_isx_kernel<true>, dim3(grid), dim3(W * WPB), 0, stream, data, a_off, a_len, b_off,
+        hipLaunchKernelGGL(fm_isx_kernel<true>, dim3(grid), dim3(W * (NB + NE)), 0, stream, data, a_off, a_len, b_off,
                            b_len, npairs, counts, out_off, out, work, ncls);
     else
-        hipLaunchKernelGGL(fm_isx_kernel<false>, dim3(grid), dim3(W * WPB), 0, stream, data, a_off, a_len, b_off,
-                           b_len, npairs, counts, out_off, out, work, ncls);
+        hipLaunchKernelGGL(fm_isx_kernel<false>, dim3(grid), dim3(W * WPB_COUNT), 0, stream, data, a_off, a_len,
+                           b_off, b_len, npairs, counts, out_off, out, work, ncls);
 }
 
-int isx_blocks_per_cu() {
-    int b = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, fm_isx_kernel<true>, W * WPB, 0) != hipSuccess || b <= 0)
-        b = 2;
-    return b;
+int isx_blocks_per_cu(bool out) {
+    static const int with_out = blocks_per_cu<true>(), count_only = std::min(COUNT_BLOCKS_PER_CU, blocks_per_cu<false>());
+    return out ? with_out : count_only;
 }
 
 }  // namespace fm

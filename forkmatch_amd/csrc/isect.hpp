@@ -51,12 +51,13 @@ constexpr uint32_t ISX_GAVE_UP = 2u;   // a look-back wait passed its limit
 // Launches the plan and main kernels on `stream`; allocates nothing. `work`
 // holds isx_work_words(npairs, cap) words, zeroed before its first use (the
 // status words carry a call epoch kept in the header, so later calls need no
-// clearing). `grid` is the main kernel's block count (resident blocks).
-void isx_launch(hipStream_t stream, uint32_t grid, uint32_t *work, uint32_t cap, const uint32_t *data,
+// clearing). The main kernel runs the blocks that fit at once on `cus`
+// compute units.
+void isx_launch(hipStream_t stream, uint32_t cus, uint32_t *work, uint32_t cap, const uint32_t *data,
                 const uint64_t *a_off, const uint32_t *a_len, const uint64_t *b_off, const uint32_t *b_len,
                 uint32_t npairs, uint32_t *counts, const uint64_t *out_off, uint32_t *out);
 
-// Resident blocks per CU of the main kernel.
-int isx_blocks_per_cu();
+// Resident blocks per CU of the main kernel, with output or count only.
+int isx_blocks_per_cu(bool out);
 
 }  // namespace fm

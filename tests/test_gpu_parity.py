@@ -402,18 +402,29 @@ def test_intersection_microkernel(engine):
     lists, a_off, a_len, b_off, b_len = [], [], [], [], []
     pos = 0
     expected = []
-    # one-tile, many-tile (up to ~550 tiles: more than one window of staged
-    # cuts), skewed and empty pairs
-    sizes = [(0, 5), (1, 1), (3, 3000), (2500, 2600), (5000, 70), (64, 64), (4096, 10000), (1, 0),
-             (100000, 100000), (600, 40000), (30000, 25000), (1000000, 1100000)]
-    for na, nb in sizes:
-        span = max(20000, 3 * max(na, nb))
+    # one-segment and many-segment pairs (up to 129 merge segments of 16,384
+    # path elements), skew pairs (longer list >= 32x the shorter), empty
+    # pairs; dense segments (A part spanning < 32,768 values) at these spans,
+    # sparse ones over the wide spans of the last three
+    sizes = [(0, 5, 0), (1, 1, 0), (3, 3000, 0), (2500, 2600, 0), (5000, 70, 0), (64, 64, 0), (4096, 10000, 0),
+             (1, 0, 0), (100000, 100000, 0), (600, 40000, 0), (30000, 25000, 0), (1000000, 1100000, 0),
+             (20000, 30000, 10 ** 9), (5000, 6000, 10 ** 8), (700, 30000, 3 * 10 ** 6)]
+    for na, nb, wide in sizes:
+        span = wide or max(20000, 3 * max(na, nb))
         a = np.unique(rng.integers(0, span, na, dtype=np.uint32))
         b = np.unique(rng.integers(0, span, nb, dtype=np.uint32))
         a_off.append(pos); a_len.append(len(a)); lists.append(a); pos += len(a)
         b_off.append(pos); b_len.append(len(b)); lists.append(b); pos += len(b)
         expected.append(np.intersect1d(a, b))
-    # equal lists (every element matches: the most matches per stretch) and
+    # a pair whose first segments are dense and later ones sparse
+    a = np.concatenate([np.arange(40000, dtype=np.uint32),
+                        np.arange(10 ** 8, 10 ** 8 + 30000 * 50000, 50000, dtype=np.int64).astype(np.uint32)])
+    b = np.arange(0, 10 ** 8 + 30000 * 50000, 25000, dtype=np.int64).astype(np.uint32)
+    b = np.union1d(b, np.arange(0, 40000, 3, dtype=np.uint32))
+    a_off.append(pos); a_len.append(len(a)); lists.append(a); pos += len(a)
+    b_off.append(pos); b_len.append(len(b)); lists.append(b); pos += len(b)
+    expected.append(np.intersect1d(a, b))
+    # equal lists (every element matches: every bit of the AND set) and
     # lists that hold the largest and smallest u32 values
     top = np.uint32(0xFFFFFFFF)
     for a, b in [(np.arange(9000, dtype=np.uint32), np.arange(9000, dtype=np.uint32)),

@@ -144,9 +144,17 @@ def dense_segment(pa, pb, amin, span, rng):
         return []
     for c in range(-(-lb // CH)):
         set_bits(bmB, buffer_chunk(pb, c * CH, lb - c * CH), amin, span, lb - c * CH, True)
-    common = [bmA[w] & bmB[w] for w in range(nw)]
-    # dense_emit: 64-word rounds, each lane's values at its word's prefix offset, through a CH-word stage
-    out = []
+    return [bmA[w] & bmB[w] for w in range(nw)]
+
+
+def dense_emit(common, amin, o, lim):
+    """dense_emit: 64-word rounds, each lane's values at its word's prefix
+    offset in a CH-word stage, the stage flushed to o[base + lo, ...) in
+    groups of four words (one 16-byte store per lane) and, for the group at
+    the stage's end or at lim, word by word; nothing at or past lim is
+    written. Returns the number of values."""
+    nw = len(common)
+    base = 0
     for w0 in range(0, nw, W):
         words = [common[w] if w < nw else 0 for w in range(w0, w0 + W)]
         counts = [bin(x).count("1") for x in words]
@@ -162,10 +170,19 @@ def dense_segment(pa, pb, amin, span, rng):
                         stage[pos - lo] = amin + 32 * (w0 + l) + ((b & -b).bit_length() - 1)
                     pos += 1
                     b &= b - 1
-            cnt = min(CH, tot - lo)
+            cnt, at0 = min(CH, tot - lo), base + lo
             assert None not in stage[:cnt]
-            out += stage[:cnt]
-    return out
+            for l in range(W):
+                for i in range(4 * l, cnt, 4 * W):
+                    at = at0 + i
+                    if i + 4 <= cnt and at + 4 <= lim:
+                        o[at:at + 4] = stage[i:i + 4]
+                    else:
+                        for e in range(4):
+                            if i + e < cnt and at + e < lim:
+                                o[at + e] = stage[i + e]
+        base += tot
+    return base
 
 
 def sparse_segment(pa, pb):
@@ -253,11 +270,10 @@ def run_segment(A, B, s, rng, stats):
     span = int(pa[-1]) - amin if len(pa) else 0
     if span < BMSPAN:
         stats["dense"] += 1
-        m = dense_segment(pa, pb, amin, span, rng)
-    else:
-        stats["sparse"] += 1
-        m = sparse_segment(pa, pb)
-    return m
+        common = dense_segment(pa, pb, amin, span, rng)
+        return ("bits", common, amin)
+    stats["sparse"] += 1
+    return sparse_segment(pa, pb)
 
 
 def intersect_model(pairs, rng):
@@ -275,7 +291,10 @@ def intersect_model(pairs, rng):
         if len(A) > len(B):
             A, B = B, A
         per_seg.append((p, run_segment(A, B, g - seg_begin[p], rng, stats)))
-    prefix = look_back_model([len(m) for _, m in per_seg], seg_begin, rng)
+    def n_of(m):
+        return sum(bin(x).count("1") for x in m[1]) if isinstance(m, tuple) else len(m)
+
+    prefix = look_back_model([n_of(m) for _, m in per_seg], seg_begin, rng, restart=bool(rng.integers(2)))
     results = []
     for p, (A, B) in enumerate(pairs):
         room = min(len(A), len(B))
@@ -283,18 +302,28 @@ def intersect_model(pairs, rng):
         count = 0
         for g in range(seg_begin[p], seg_begin[p + 1]):
             m = per_seg[g][1]
-            lim = min(len(m), room - prefix[g]) if prefix[g] < room else 0
-            out[prefix[g]:prefix[g] + lim] = m[:lim]
-            count = prefix[g] + len(m)
+            lim = room - prefix[g] if prefix[g] < room else 0
+            if isinstance(m, tuple):
+                view = out[prefix[g]:]  # writes past lim would fail the room check below
+                written = np.full(len(view), U32, np.uint64)
+                n = dense_emit(m[1], m[2], written, lim)
+                assert n == n_of(m) and np.all(written[lim:] == U32)
+                view[:lim] = np.where(written[:lim] != U32, written[:lim], view[:lim])
+            else:
+                n = len(m)
+                out[prefix[g]:prefix[g] + min(n, lim)] = m[:min(n, lim)]
+            count = prefix[g] + n
         results.append(out[:count].astype(np.uint32))
     return results, stats
 
 
-def look_back_model(counts, seg_begin, rng):
+def look_back_model(counts, seg_begin, rng, restart=False):
     """Waves run segments concurrently in claim order; each publishes its
     count, then reads up to 64 predecessors per round until it meets an
     inclusive sum, then publishes its own. Random interleaving; returns every
-    segment's exclusive prefix within its pair."""
+    segment's exclusive prefix within its pair. With restart, an attempt
+    that meets an unpublished predecessor starts again from g - 1 (the
+    emitters' look-back that does not wait)."""
     n = len(counts)
     pair_first = {}
     for p in range(len(seg_begin) - 1):
@@ -335,8 +364,12 @@ def look_back_model(counts, seg_begin, rng):
                     status[g] = ("I", prefix[g] + counts[g])
                     state[g] = 2
                     active.remove(g)
+                elif restart:
+                    look[g] = (g - 1, 0)
             elif not waiting:
                 look[g] = (t - W, acc + sum(s[1] for s in window))
+            elif restart:
+                look[g] = (g - 1, 0)
     for g in range(n):
         sb = pair_first[g]
         assert prefix[g] == sum(counts[sb:g])
@@ -447,3 +480,148 @@ def test_merge_cuts_match_the_merge():
         for d in rng.integers(0, len(A) + len(B) + 1, 10):
             i, _ = merge_cut(A, B, int(d))
             assert i == sum(1 for _, src in merged[:int(d)] if src == 0)
+
+
+def claim_seg(k, cls, ncls, run):
+    """Segment of class cls's k-th claim: runs of `run` consecutive segments
+    dealt to the classes in turn (fm_isx_kernel's claim mapping)."""
+    return ((k // run) * ncls + cls) * run + k % run
+
+
+def block_protocol_model(counts, seg_begin, rng, nblocks=3, nb=2, ns=2, ne=1, ncls=2, run=2):
+    """Builder waves claim segments (home class = block % ncls; after the
+    home class a class's counter is read before it is claimed from), build
+    them for a random number of steps, publish the count (the inclusive sum
+    for a pair's first segment), then hand the segment to a free slot of
+    their block, skip it (a zero count that is not its pair's last), or find
+    its offset themselves (waiting). Emitter waves take ready slots and try
+    the look-back without waiting, making the slot ready again when it
+    cannot finish, and end when their block's builders have ended with no
+    slot ready. Every wave is scheduled at random; returns the offsets found
+    (None for skipped segments) and fails if the waves stop making progress."""
+    total = len(counts)
+    sb_of, last_of = [0] * total, [False] * total
+    for p in range(len(seg_begin) - 1):
+        for g in range(seg_begin[p], seg_begin[p + 1]):
+            sb_of[g], last_of[g] = seg_begin[p], g == seg_begin[p + 1] - 1
+    ctr = [0] * ncls
+    status = [None] * total
+    prefix = [None] * total
+    slots = [[None] * ns for _ in range(nblocks)]  # None (free) or ["ready" | "emitting", g]
+    done = [0] * nblocks
+    waves = []
+    for b in range(nblocks):
+        waves += [{"kind": "b", "block": b, "cls": b % ncls, "tried": 0, "state": "claim"} for _ in range(nb)]
+        waves += [{"kind": "e", "block": b, "state": "run"} for _ in range(ne)]
+
+    def attempt(g):
+        acc = 0
+        for t in range(g - 1, sb_of[g] - 1, -1):
+            if status[t] is None:
+                return None
+            if status[t][0] == "I":
+                return acc + status[t][1]
+            acc += status[t][1]
+        return acc
+
+    def record(g, pre):
+        assert prefix[g] is None
+        prefix[g] = pre
+        if g != sb_of[g]:
+            status[g] = ("I", pre + counts[g])
+
+    steps, limit = 0, 400 * (total + 1) * len(waves)
+    while True:
+        live = [w for w in waves if w["state"] != "end"]
+        if not live:
+            break
+        steps += 1
+        assert steps < limit, "no progress"
+        w = live[rng.integers(len(live))]
+        b = w["block"]
+        if w["kind"] == "b":
+            if w["state"] == "claim":
+                if w["tried"] >= ncls:
+                    w["state"] = "end"
+                    done[b] += 1
+                    continue
+                c = w["cls"]
+                spent = w["tried"] > 0 and claim_seg(ctr[c], c, ncls, run) >= total
+                g = total
+                if not spent:
+                    g = claim_seg(ctr[c], c, ncls, run)
+                    ctr[c] += 1
+                if g >= total:
+                    w["cls"], w["tried"] = (c + 1) % ncls, w["tried"] + 1
+                    continue
+                w.update(g=g, state="build", steps=int(rng.integers(1, 5)))
+            elif w["state"] == "build":
+                w["steps"] -= 1
+                if w["steps"]:
+                    continue
+                g = w["g"]
+                status[g] = ("I" if g == sb_of[g] else "A", counts[g])
+                if counts[g] == 0 and not last_of[g]:
+                    w["state"] = "claim"
+                    continue
+                free = [i for i in range(ns) if slots[b][i] is None]
+                if free:
+                    slots[b][free[0]] = ["ready", g]
+                    w["state"] = "claim"
+                else:
+                    w["state"] = "fallback"
+            elif w["state"] == "fallback":
+                pre = attempt(w["g"])
+                if pre is not None:
+                    record(w["g"], pre)
+                    w["state"] = "claim"
+        else:
+            ready = [i for i in range(ns) if slots[b][i] is not None and slots[b][i][0] == "ready"]
+            if not ready:
+                if done[b] == nb and all(s is None or s[0] != "ready" for s in slots[b]):
+                    w["state"] = "end"
+                continue
+            i = ready[rng.integers(len(ready))]
+            g = slots[b][i][1]
+            slots[b][i][0] = "emitting"
+            pre = attempt(g)
+            if pre is None:
+                slots[b][i][0] = "ready"
+            else:
+                record(g, pre)
+                slots[b][i] = None
+    assert all(s is None for blk in slots for s in blk)
+    return prefix
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_builder_emitter_protocol_finds_every_offset(seed):
+    rng = np.random.default_rng(100 + seed)
+    nseg = rng.integers(1, 7, 9)  # segments per pair
+    seg_begin = np.concatenate([[0], np.cumsum(nseg)]).tolist()
+    total = seg_begin[-1]
+    counts = [int(x) if rng.random() < 0.7 else 0 for x in rng.integers(0, 50, total)]
+    params = dict(nblocks=int(rng.integers(1, 4)), nb=int(rng.integers(1, 3)), ns=int(rng.integers(1, 3)),
+                  ne=int(rng.integers(1, 3)), run=int(rng.integers(1, 4)))
+    params["ncls"] = int(rng.integers(1, params["nblocks"] + 1))  # every class has a home block
+    prefix = block_protocol_model(counts, seg_begin, rng, **params)
+    for p in range(len(seg_begin) - 1):
+        for g in range(seg_begin[p], seg_begin[p + 1]):
+            if counts[g] == 0 and g != seg_begin[p + 1] - 1:
+                assert prefix[g] is None
+            else:
+                assert prefix[g] == sum(counts[seg_begin[p]:g])
+
+
+def test_claim_mapping_covers_every_segment_once_in_class_order():
+    for ncls, run, total in ((1, 16, 100), (8, 16, 9149), (3, 2, 50), (8, 1, 17)):
+        seen = []
+        for c in range(ncls):
+            segs = []
+            k = 0
+            while claim_seg(k, c, ncls, run) < total:
+                segs.append(claim_seg(k, c, ncls, run))
+                k += 1
+            assert segs == sorted(segs)
+            seen += segs
+        assert sorted(seen) == list(range(total))
