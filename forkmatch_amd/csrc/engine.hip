@@ -2250,8 +2250,8 @@ struct fm_engine {
     uint64_t epoch = 0;
     bool dirty = true;  // queues/pool need a full re-initialisation
     fm_server *server = nullptr;  // a server owns the engine while it lives
-    void *isect_buf = nullptr;    // intersection work counter and pair order (grow only)
-    size_t isect_cap = 0;
+    void *isect_buf = nullptr;    // intersection work array (regrown when too small, one layout each)
+    uint64_t isect_segs = 0, isect_pairs = 0;  // its segment and pair capacity
     FSlot *pfree_ring = nullptr;
     uint32_t pfcap = 0;
     uint64_t prealloc = 0;     // PREALLOC: arenas + graph may take this many device bytes
@@ -3561,7 +3561,10 @@ void fm_server_destroy(fm_server *s) {
 // the address hipGetSymbolAddress returns for the device, taken once per
 // device at engine creation, never the array's name.
 constexpr uint32_t NOALLOC_WORDS = FM_NOALLOC_SCRATCH_BYTES / 4;
-static_assert(isx_work_words(FM_NOALLOC_MAX_PAIRS, 0) < NOALLOC_WORDS, "the pair arrays fit the scratch");
+// its fixed geometry: room for the largest pair count's table, the rest status words
+constexpr uint64_t NOALLOC_PAIRS = (uint64_t)FM_NOALLOC_MAX_PAIRS + 1;
+constexpr uint32_t NOALLOC_SEGS = (uint32_t)((NOALLOC_WORDS - ISX_HDR - NOALLOC_PAIRS) / 2);
+static_assert(isx_work_words(NOALLOC_PAIRS, NOALLOC_SEGS) <= NOALLOC_WORDS, "the work array fits the scratch");
 static __device__ __align__(256) uint32_t scratch[NOALLOC_WORDS];
 constexpr int FM_MAX_DEVICES = 64;
 static std::mutex g_scratch_mu;
@@ -3587,8 +3590,9 @@ static uint32_t *isect_scratch_of(int device) {
 }
 
 // The plan and main kernels on the engine's stream with `work` holding
-// isx_work_words(npairs, cap) words; allocates nothing. When the pairs need
-// more than cap segments the kernels do no work and *over is set.
+// isx_work_words(npairs + 1, cap) words laid out for cap segments;
+// allocates nothing. When the pairs need more than cap segments the kernels
+// do no work and *over is set.
 static int isect_run(fm_engine *e, uint32_t *work, uint32_t cap, const uint32_t *data, const uint64_t *a_off,
                      const uint32_t *a_len, const uint64_t *b_off, const uint32_t *b_len, uint32_t npairs,
                      uint32_t *counts, const uint64_t *out_off, uint32_t *out, double *kernel_ms, bool *over) {
@@ -3631,18 +3635,21 @@ int fm_intersect_pairs(fm_engine *e, const uint32_t *data, const uint64_t *a_off
     uint64_t segs = 0;
     for (uint32_t p = 0; p < npairs; p++) segs += isx_segments(la[p], lb[p]);
     if (segs >= (1ull << 31)) return fail(FM_ERR_INVALID_ARG, "too many intersection segments");
-    const size_t need = isx_work_words(npairs, segs) * 4;
-    if (need > e->isect_cap) {
+    if (segs > e->isect_segs || npairs + 1ull > e->isect_pairs) {
+        // a new array, laid out for at least as many segments and pairs as
+        // the old one (one layout for the array's life)
+        const uint64_t nsegs = std::max<uint64_t>(segs, e->isect_segs), npr = std::max<uint64_t>(npairs + 1ull, e->isect_pairs);
+        const size_t bytes = isx_work_words(npr, nsegs) * 4;
         (void)hipFree(e->isect_buf);
         e->isect_buf = nullptr;
-        e->isect_cap = 0;
-        HIP_TRY(hipMalloc((void **)&e->isect_buf, need));
+        e->isect_segs = e->isect_pairs = 0;
+        HIP_TRY(hipMalloc((void **)&e->isect_buf, bytes));
         // status words carry a call epoch: a new array starts from zero
-        HIP_TRY(hipMemset(e->isect_buf, 0, need));
-        e->isect_cap = need;
+        HIP_TRY(hipMemset(e->isect_buf, 0, bytes));
+        e->isect_segs = nsegs, e->isect_pairs = npr;
     }
-    return isect_run(e, (uint32_t *)e->isect_buf, (uint32_t)segs, data, a_off, a_len, b_off, b_len, npairs, counts,
-                     out_off, out, kernel_ms, nullptr);
+    return isect_run(e, (uint32_t *)e->isect_buf, (uint32_t)e->isect_segs, data, a_off, a_len, b_off, b_len, npairs,
+                     counts, out_off, out, kernel_ms, nullptr);
 }
 
 int fm_intersect_pairs_noalloc(fm_engine *e, const uint32_t *data, const uint64_t *a_off, const uint32_t *a_len,
@@ -3654,7 +3661,7 @@ int fm_intersect_pairs_noalloc(fm_engine *e, const uint32_t *data, const uint64_
                     (uint32_t)FM_NOALLOC_MAX_PAIRS);
     uint32_t *work = isect_scratch_of(e->device);
     if (!work) return fail(FM_ERR_DEVICE, "no scratch address for device %d", e->device);
-    const uint32_t cap = (uint32_t)((NOALLOC_WORDS - isx_work_words(npairs, 0)) / 2);
+    const uint32_t cap = NOALLOC_SEGS;
     bool over = false;
     int rc = isect_run(e, work, cap, data, a_off, a_len, b_off, b_len, npairs, counts, out_off, out, kernel_ms, &over);
     if (rc) return rc;

@@ -538,48 +538,55 @@ __device__ __forceinline__ bool try_resolve(unsigned long long *status, uint32_t
 }
 
 // One block of 1,024 threads: every pair's first segment (a scan of the
-// segment counts), the counts cleared, the claim counters cleared, the
-// segment total (0 and the ISX_OVER flag when it exceeds cap), a new epoch.
+// segment counts; thread t takes a run of consecutive pairs, so its length
+// loads are issued together), the counts cleared, the claim counters
+// cleared, the segment total (0 and the ISX_OVER flag when it exceeds cap),
+// a new epoch. When the epoch wraps, the status words are cleared first, so
+// no word of a call 2^30 calls back reads as current.
 __global__ __launch_bounds__(1024) void fm_isx_plan_kernel(const uint32_t *a_len, const uint32_t *b_len,
                                                            uint32_t npairs, uint32_t *counts, uint32_t *work,
                                                            uint32_t cap) {
-    __shared__ unsigned long long s_part[1024 / W];
-    __shared__ unsigned long long s_carry;
+    constexpr uint32_t T = 1024, EPOCH_MASK = (1u << 30) - 1;
+    __shared__ unsigned long long s_part[T / W];
     const uint32_t tid = threadIdx.x, l = lane(), wid = tid / W;
-    uint32_t *const seg_begin = work + ISX_HDR;
-    if (tid == 0) s_carry = 0;
+    uint32_t *const seg_begin = work + isx_table_word(cap);
     if (tid < ISX_CLASSES) work[ISX_CLAIM + 32 * tid] = 0;
-    __syncthreads();
-    for (uint32_t p0 = 0; p0 < npairs; p0 += 1024) {
-        const uint32_t p = p0 + tid;
-        const unsigned long long ns = p < npairs ? isx_segments(a_len[p], b_len[p]) : 0ull;
-        if (p < npairs) counts[p] = 0;
-        unsigned long long v = ns;
+    const uint32_t run = (npairs + T - 1) / T;
+    const uint32_t r0 = min(tid * run, npairs), r1 = min(r0 + run, npairs);
+    unsigned long long mine = 0;
+    for (uint32_t p = r0; p < r1; p++) {
+        mine += isx_segments(a_len[p], b_len[p]);
+        counts[p] = 0;
+    }
+    unsigned long long v = mine;
 #pragma unroll
-        for (int o = 1; o < (int)W; o <<= 1) {
-            const unsigned long long t = __shfl_up(v, o, W);
-            if (l >= (uint32_t)o) v += t;
-        }
-        if (l == W - 1) s_part[wid] = v;
-        __syncthreads();
-        unsigned long long pre = s_carry, tot = s_carry;
-        for (uint32_t w = 0; w < 1024 / W; w++) {
-            if (w < wid) pre += s_part[w];
-            tot += s_part[w];
-        }
-        const unsigned long long begin = pre + v - ns;
-        if (p < npairs) seg_begin[p] = begin <= cap ? (uint32_t)begin : cap;
-        __syncthreads();
-        if (tid == 0) s_carry = tot;
-        __syncthreads();
+    for (int o = 1; o < (int)W; o <<= 1) {
+        const unsigned long long t = __shfl_up(v, o, W);
+        if (l >= (uint32_t)o) v += t;
+    }
+    if (l == W - 1) s_part[wid] = v;
+    const uint32_t old_epoch = work[ISX_EPOCH];
+    if (old_epoch == EPOCH_MASK) {
+        // the epoch wraps: clear the status words (every segment the array can hold)
+        unsigned long long *const status = reinterpret_cast<unsigned long long *>(work + ISX_HDR);
+        for (uint32_t g = tid; g < cap; g += T) status[g] = 0;
+    }
+    __syncthreads();
+    unsigned long long begin = v - mine, total = 0;
+    for (uint32_t w = 0; w < T / W; w++) {
+        if (w < wid) begin += s_part[w];
+        total += s_part[w];
+    }
+    for (uint32_t p = r0; p < r1; p++) {
+        seg_begin[p] = begin <= cap ? (uint32_t)begin : cap;
+        begin += isx_segments(a_len[p], b_len[p]);
     }
     if (tid == 0) {
-        const unsigned long long total = s_carry;
         const bool over = total > cap;
         seg_begin[npairs] = over ? cap : (uint32_t)total;
         work[ISX_TOTAL] = over ? 0u : (uint32_t)total;
         work[ISX_FLAGS] = over ? ISX_OVER : 0u;
-        const uint32_t ep = (work[ISX_EPOCH] + 1) & ((1u << 30) - 1);
+        const uint32_t ep = (old_epoch + 1) & EPOCH_MASK;
         work[ISX_EPOCH] = ep ? ep : 1u;
     }
 }
@@ -724,13 +731,13 @@ template <bool OUT>
 __global__ __launch_bounds__(W *(OUT ? NB + NE : WPB_COUNT)) void fm_isx_kernel(
     const uint32_t *__restrict__ data, const uint64_t *a_off, const uint32_t *a_len, const uint64_t *b_off,
     const uint32_t *b_len, uint32_t npairs, uint32_t *counts, const uint64_t *out_off, uint32_t *out, uint32_t *work,
-    uint32_t ncls) {
+    uint32_t cap, uint32_t ncls) {
     __shared__ BlockLds<OUT> sh;
     const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x / W), l = lane();
     const uint32_t total = __builtin_amdgcn_readfirstlane(ld_rlx(&work[ISX_TOTAL]));
     const uint32_t epoch = __builtin_amdgcn_readfirstlane(ld_rlx(&work[ISX_EPOCH]));
-    const uint32_t *const seg_begin = work + ISX_HDR;
-    unsigned long long *const status = reinterpret_cast<unsigned long long *>(work + isx_status_word(npairs));
+    const uint32_t *const seg_begin = work + isx_table_word(cap);
+    unsigned long long *const status = reinterpret_cast<unsigned long long *>(work + ISX_HDR);
     if constexpr (OUT) {
         if (threadIdx.x < NS) sh.state[threadIdx.x] = SLOT_FREE;
         if (threadIdx.x == 0) sh.done = 0;
@@ -871,10 +878,10 @@ void isx_launch(hipStream_t stream, uint32_t cus, uint32_t *work, uint32_t cap, 
     const uint32_t ncls = std::min<uint32_t>(ISX_CLASSES, grid);
     if (out_off)
         hipLaunchKernelGGL(fm_isx_kernel<true>, dim3(grid), dim3(W * (NB + NE)), 0, stream, data, a_off, a_len, b_off,
-                           b_len, npairs, counts, out_off, out, work, ncls);
+                           b_len, npairs, counts, out_off, out, work, cap, ncls);
     else
         hipLaunchKernelGGL(fm_isx_kernel<false>, dim3(grid), dim3(W * WPB_COUNT), 0, stream, data, a_off, a_len,
-                           b_off, b_len, npairs, counts, out_off, out, work, ncls);
+                           b_off, b_len, npairs, counts, out_off, out, work, cap, ncls);
 }
 
 int isx_blocks_per_cu(bool out) {

@@ -456,8 +456,8 @@ int fm_intersect_pairs(fm_engine *e, const uint32_t *data,
  * other allocating routine (the engine, created beforehand, provides the
  * device and stream). Limits: FM_NOALLOC_MAX_PAIRS pairs, and the pairs'
  * segments (16,384 merge-path elements each, or 512 elements of the shorter
- * list of a pair whose longer list is 32x longer) must fit the array: about
- * 491k segments at 65,536 pairs, 524k at one pair.
+ * list of a pair whose longer list is 32x longer) must fit the array: 491,359
+ * segments.
  * Beyond either limit it returns FM_ERR_INVALID_ARG (counts are then zero).
  * Every call on one GPU uses that GPU's one array: calls that overlap on one
  * GPU (from several engines or threads) race. */

@@ -397,6 +397,32 @@ def test_exhausted_bank_pool_fails_cleanly():
         assert e.match(g, q2, table=False, timeout_ms=TIMEOUT_MS).count(0) > 0
 
 
+def test_intersection_calls_of_different_pair_counts(engine):
+    """One engine's work array serves calls with many pairs, then few, then
+    many again (each call's first-segment table and status words reuse the
+    array): every call's counts and intersections equal numpy's."""
+    rng = np.random.default_rng(12)
+    for npairs, max_len in [(3000, 3000), (7, 60000), (2500, 6000), (1, 200000), (3000, 3000), (40, 20000)]:
+        lists, a_off, a_len, b_off, b_len, expected = [], [], [], [], [], []
+        pos = 0
+        for _ in range(npairs):
+            na, nb = (int(x) for x in rng.integers(0, max_len, 2))
+            span = 3 * max(na, nb, 1)
+            a = np.unique(rng.integers(0, span, na, dtype=np.uint32))
+            b = np.unique(rng.integers(0, span, nb, dtype=np.uint32))
+            a_off.append(pos); a_len.append(len(a)); lists.append(a); pos += len(a)
+            b_off.append(pos); b_len.append(len(b)); lists.append(b); pos += len(b)
+            expected.append(np.intersect1d(a, b))
+        data = np.concatenate(lists).astype(np.uint32)
+        out_off = np.cumsum([0] + [min(x, y) for x, y in zip(a_len, b_len)]).astype(np.uint64)
+        res = fm.intersect_pairs(engine, data, np.array(a_off, np.uint64), np.array(a_len, np.uint32),
+                                 np.array(b_off, np.uint64), np.array(b_len, np.uint32), out_off[:-1],
+                                 int(out_off[-1]))
+        for i, ex in enumerate(expected):
+            assert res["counts"][i] == len(ex), (npairs, i)
+            assert np.array_equal(res["out"][out_off[i]: out_off[i] + len(ex)], ex), (npairs, i)
+
+
 def test_intersection_microkernel(engine):
     rng = np.random.default_rng(11)
     lists, a_off, a_len, b_off, b_len = [], [], [], [], []

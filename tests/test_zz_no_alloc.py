@@ -1,5 +1,5 @@
 """fm_intersect_pairs_noalloc (the entry point that allocates nothing: its
-claim counters, segment table and status words live in the library's
+claim counters, status words and segment table live in the library's
 fixed device array)
 against fm_intersect_pairs, through the C ABI: the same counts and the same
 intersections, bit for bit, on two calls in a row, the second with fewer
