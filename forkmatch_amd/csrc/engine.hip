@@ -134,10 +134,6 @@ struct LevelQ {
 
 // Idle-wave count in shards on separate lines: waves going idle and busy
 // update one shard each, readers sum them.
-constexpr uint32_t HUNGRY_SHARDS = 8;
-struct HungryShard {
-    alignas(128) int32_t n;
-};
 
 constexpr unsigned long long SRV_ACTIVE_ONE = 1ull << 32;
 constexpr unsigned long long SRV_CLOSING = 1ull << 63;
@@ -158,7 +154,10 @@ struct Ctl {
     // than pending, which every task end updates.
     alignas(128) uint32_t done;
     alignas(128) unsigned long long root_cursor;
-    HungryShard hungry[HUNGRY_SHARDS];  // waves that found no work, by workgroup id % HUNGRY_SHARDS
+    // Waves that found no work. It changes only when a wave runs out of work
+    // or finds some again, so one word takes the atomics; busy waves read it
+    // at their work-splitting checks.
+    alignas(128) int32_t idle_waves;
     alignas(128) uint32_t abort;
     uint32_t error;
     unsigned long long t0;  // wall clock of the first wave (deadline origin)
@@ -332,8 +331,10 @@ __device__ __forceinline__ bool spin_ok(const Ctl *c, uint32_t &iter, uint32_t s
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
+// Lane src's value on every lane (src is the same on every lane): a scalar
+// read of the lane's register, no LDS round trip.
 __device__ __forceinline__ uint32_t bcast(uint32_t v, int src) {
-    return (uint32_t)__shfl((int)v, src, WAVE);
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, src);
 }
 
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
@@ -1037,9 +1038,39 @@ __device__ __forceinline__ void flush_run(const KParams &P, WaveOut &o, uint32_t
 // LEAF_U driver-row loads in flight. Every candidate is still loaded and
 // checked one by one. Splits and the first/last parent ranges follow
 // process_task.
+// Work splitting: a busy wave donates part of its task while idle waves
+// outnumber the waiting work (queued tasks and unclaimed root chunks): with
+// thousands of waves idle, every busy wave would otherwise cut its work into
+// pieces at every check. The queue and root counters, which every push, pop
+// and root claim updates, are read only while some wave is idle: polled by
+// every busy wave at every check they slow those updates down (one-query
+// launches took up to 1.8x longer). The counters are read at the check:
+// deciding on values read one check earlier let more waves donate for the
+// same idle waves (up to 1.8x the splits) and cost more than the round trip.
+// A donated range is staged again by the wave that takes it (parent rows
+// into LDS, one row lookup per parent), so candidates are donated only from
+// long enough ranges: 2,048 candidates in inner levels, 4,096 in count-only
+// leaf batches. On the plain R-MAT 2^20 workload (32 queries) the thresholds
+// of 512 took 283 ms of kernel time per step against 263 ms.
+constexpr uint32_t SPLIT_MIN_CANDIDATES = 32 * WAVE;
+constexpr uint32_t SPLIT_MIN_CANDIDATES_LEAF = 64 * WAVE;
+__device__ __forceinline__ bool split_wanted(const KParams &P) {
+    uint32_t want = 0;
+    const int32_t idle = lane_id() == 0 ? ld_rlx(&P.ctl->idle_waves) : 0;
+    if (lane_id() == 0 && idle > 0) {
+        const unsigned long long rc = ld_rlx(&P.ctl->root_cursor);
+        const long long roots_left = rc < P.total_root_chunks ? (long long)(P.total_root_chunks - rc) : 0;
+        want = (long long)idle > (long long)max(0, ld_rlx(&P.ctl->queued)) + roots_left ? 1u : 0u;
+    }
+    return bcast(want, 0) != 0;
+}
+
 constexpr uint32_t LEAF_TILE = 4;
 constexpr uint32_t LEAF_MAX = LEAF_TILE * WAVE;  // parents per batch
-static_assert(LEAF_MAX * 2 < POOL_WORDS, "leaf batch arrays fit the pool");
+constexpr uint32_t LEAF_U = 8;                   // candidates per lane per pass
+// pool: 0 (word 3, the sum before parent 0) | cum[LEAF_MAX] | dstart[LEAF_MAX] | parent rows
+constexpr uint32_t LEAF_HEAD = 4;
+static_assert(LEAF_HEAD + 2 * LEAF_MAX + MAXN <= POOL_WORDS, "leaf batch arrays fit the pool");
 
 __device__ __forceinline__ void process_leaf_task(const KParams &P, const TaskDesc &t, Lds &s, unsigned long long *stats,
                                   unsigned long long deadline) {
@@ -1051,11 +1082,10 @@ __device__ __forceinline__ void process_leaf_task(const KParams &P, const TaskDe
     const uint32_t n = pl.n, d = t.depth, W = d + 1, D = d + 1;
     const uint32_t req = pl.req[pl.req_begin[D]];
     const uint32_t lvl_self = n - 2 - d;
-    // pool: cum[LEAF_MAX] | dstart[LEAF_MAX] | parent rows
-    uint32_t *const cum = s.pool;
-    uint32_t *const dstart = s.pool + LEAF_MAX;
-    uint32_t *const par = s.pool + 2 * LEAF_MAX;
-    const uint32_t NB = min(LEAF_TILE * P.tile_parents, (POOL_WORDS - 2 * LEAF_MAX) / W);
+    uint32_t *const cum = s.pool + LEAF_HEAD;  // 16-byte aligned; cum[-1] == 0
+    uint32_t *const dstart = cum + LEAF_MAX;
+    uint32_t *const par = dstart + LEAF_MAX;
+    const uint32_t NB = min(LEAF_TILE * P.tile_parents, (POOL_WORDS - LEAF_HEAD - 2 * LEAF_MAX) / W);
     const uint32_t naut = pl.naut > 1 ? pl.naut : 1u;
     unsigned long long cnt_acc = 0;
     uint32_t cnt_local = t.count;    // shrinks when the upper parents are donated
@@ -1091,49 +1121,19 @@ __device__ __forceinline__ void process_leaf_task(const KParams &P, const TaskDe
             cum[p] = total + incl;
             total += bcast(incl, WAVE - 1);
         }
+        if (lane == 0) cum[-1] = 0;
         uint32_t T = total;
         __syncthreads();
-        // parent of candidate tt: the number of entries with cum <= tt, in
-        // four rounds of four independent LDS reads
-        auto locate = [&](uint32_t from, uint32_t &pi_out, uint32_t &x_out) {
-            const uint32_t tt = min(from + lane, T - 1);
-            uint32_t lo64 = 0;
+        // ends of the batch's four 64-parent groups (the same for every lane)
+        uint32_t gend[LEAF_TILE];
 #pragma unroll
-            for (uint32_t j = 0; j < 4; j++) lo64 += cum[64 * j + 63] <= tt ? 64u : 0u;
-            uint32_t lo = lo64;
-#pragma unroll
-            for (uint32_t j = 0; j < 4; j++) lo += cum[lo64 + 16 * j + 15] <= tt ? 16u : 0u;
-            uint32_t lo4 = lo;
-#pragma unroll
-            for (uint32_t j = 0; j < 4; j++) lo4 += cum[lo + 4 * j + 3] <= tt ? 4u : 0u;
-            const uint4 c4 = *reinterpret_cast<const uint4 *>(&cum[lo4]);
-            uint32_t excl = lo4 ? cum[lo4 - 1] : 0u;
-            lo = lo4;
-            if (c4.x <= tt) { lo++; excl = c4.x; }
-            if (c4.y <= tt) { lo++; excl = c4.y; }
-            if (c4.z <= tt) { lo++; excl = c4.z; }
-            if (c4.w <= tt) { lo++; excl = c4.w; }
-            pi_out = lo;
-            x_out = g.col[dstart[lo] + (tt - excl)];
-        };
-        constexpr uint32_t LEAF_U = 8;
+        for (uint32_t j = 0; j < LEAF_TILE; j++) gend[j] = cum[64 * j + 63];
         uint32_t hits = 0;
         for (uint32_t base = 0, it = 0; base < T; base += LEAF_U * WAVE, it++) {
             // work splitting while waves are idle (as in process_task), every
-            // 1,024 candidates
-            if ((it & 1u) == 0) {
-                uint32_t want = 0;
-                if (lane == 0) {
-                    long long idle_waves = 0;
-                    for (uint32_t h = 0; h < HUNGRY_SHARDS; h++) idle_waves += ld_rlx(&P.ctl->hungry[h].n);
-                    if (idle_waves > 0) {
-                        const unsigned long long rc = ld_rlx(&P.ctl->root_cursor);
-                        const long long roots_left =
-                            rc < P.total_root_chunks ? (long long)(P.total_root_chunks - rc) : 0;
-                        want = idle_waves > max(0, ld_rlx(&P.ctl->queued)) + roots_left ? 1u : 0u;
-                    }
-                }
-                if (bcast(want, 0)) {
+            // 2,048 candidates
+            if ((it & 3u) == 0) {
+                if (split_wanted(P)) {
                     const uint32_t rem_par = cnt_local > b0 + nb ? cnt_local - (b0 + nb) : 0;
                     if (rem_par >= 2 * WAVE) {
                         // the upper half of the parents after this batch
@@ -1143,7 +1143,7 @@ __device__ __forceinline__ void process_leaf_task(const KParams &P, const TaskDe
                         cnt_local = first;
                         last_end = ALL_CANDIDATES;
                         if (lane == 0) stats[ST_SPLIT]++;
-                    } else if (T - base >= 8 * WAVE) {
+                    } else if (T - base >= SPLIT_MIN_CANDIDATES_LEAF) {
                         // the upper half of this batch's remaining candidates
                         // and the parents after them
                         const uint32_t mid = base + ((T - base) / (2 * WAVE)) * WAVE;
@@ -1164,16 +1164,74 @@ __device__ __forceinline__ void process_leaf_task(const KParams &P, const TaskDe
                     }
                 }
             }
-            uint32_t xs[LEAF_U], ps[LEAF_U];
-#pragma unroll
-            for (uint32_t u = 0; u < LEAF_U; u++) locate(base + u * WAVE, ps[u], xs[u]);
+            // The lane's LEAF_U candidates (base + u * 64 + lane) advance in
+            // lockstep: each round issues its LDS reads for all of them
+            // before one wait, with no branch between rounds. The parent of
+            // candidate tt is the number of entries with cum <= tt: 64-parent
+            // group (registers), 16-parent group, 4-parent group, parent.
+            uint32_t tt[LEAF_U], lo[LEAF_U], r4[LEAF_U][4];
 #pragma unroll
             for (uint32_t u = 0; u < LEAF_U; u++) {
-                const uint32_t *const prow = &par[ps[u] * W];
-                bool ok = base + u * WAVE + lane < T;
-                for (uint32_t a = 0; a < W; a++) ok &= prow[a] != xs[u];
-                hits += ok ? 1u : 0u;
+                tt[u] = min(base + u * WAVE + lane, T - 1);
+                uint32_t l = 0;
+#pragma unroll
+                for (uint32_t j = 0; j < LEAF_TILE; j++) l += gend[j] <= tt[u] ? 64u : 0u;
+                lo[u] = l;
             }
+#pragma unroll
+            for (uint32_t u = 0; u < LEAF_U; u++)
+#pragma unroll
+                for (uint32_t j = 0; j < 4; j++) r4[u][j] = cum[lo[u] + 16 * j + 15];
+#pragma unroll
+            for (uint32_t u = 0; u < LEAF_U; u++)
+#pragma unroll
+                for (uint32_t j = 0; j < 4; j++) lo[u] += r4[u][j] <= tt[u] ? 16u : 0u;
+#pragma unroll
+            for (uint32_t u = 0; u < LEAF_U; u++)
+#pragma unroll
+                for (uint32_t j = 0; j < 4; j++) r4[u][j] = cum[lo[u] + 4 * j + 3];
+#pragma unroll
+            for (uint32_t u = 0; u < LEAF_U; u++)
+#pragma unroll
+                for (uint32_t j = 0; j < 4; j++) lo[u] += r4[u][j] <= tt[u] ? 4u : 0u;
+            uint32_t ex[LEAF_U];
+#pragma unroll
+            for (uint32_t u = 0; u < LEAF_U; u++) {
+                const uint4 c4 = *reinterpret_cast<const uint4 *>(&cum[lo[u]]);
+                r4[u][0] = c4.x;
+                r4[u][1] = c4.y;
+                r4[u][2] = c4.z;
+                r4[u][3] = c4.w;
+                ex[u] = cum[(int)lo[u] - 1];  // cum[-1] == 0
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < LEAF_U; u++)
+#pragma unroll
+                for (uint32_t j = 0; j < 4; j++) {
+                    const bool le = r4[u][j] <= tt[u];
+                    lo[u] += le ? 1u : 0u;
+                    ex[u] = le ? r4[u][j] : ex[u];
+                }
+            uint32_t xs[LEAF_U];
+#pragma unroll
+            for (uint32_t u = 0; u < LEAF_U; u++) xs[u] = dstart[lo[u]] + (tt[u] - ex[u]);
+#pragma unroll
+            for (uint32_t u = 0; u < LEAF_U; u++) xs[u] = g.col[xs[u]];
+            bool ok[LEAF_U];
+#pragma unroll
+            for (uint32_t u = 0; u < LEAF_U; u++) {
+                ok[u] = base + u * WAVE + lane < T;
+                lo[u] *= W;  // the parent's row in LDS
+            }
+            for (uint32_t a = 0; a < W; a++) {
+                uint32_t pv[LEAF_U];
+#pragma unroll
+                for (uint32_t u = 0; u < LEAF_U; u++) pv[u] = par[lo[u] + a];
+#pragma unroll
+                for (uint32_t u = 0; u < LEAF_U; u++) ok[u] &= pv[u] != xs[u];
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < LEAF_U; u++) hits += ok[u] ? 1u : 0u;
         }
         cnt_acc += wave_sum_u64(hits) * naut;
         if (lane == 0 && cnt_acc >= (1ull << 20)) {
@@ -1417,22 +1475,7 @@ __device__ __forceinline__ void process_task(const KParams &P, const TaskDesc &t
         // remaining candidates together with the parents after them (a
         // hub parent's long driver row is split across waves).
         auto maybe_split = [&](uint32_t base) __attribute__((always_inline)) {
-            uint32_t want = 0;
-            // donate only while idle waves outnumber the waiting work
-            // (queued tasks and unclaimed root chunks): with thousands
-            // of waves idle, every busy wave would otherwise cut its
-            // work into pieces at every check
-            if (lane == 0) {
-                long long idle_waves = 0;
-                for (uint32_t h = 0; h < HUNGRY_SHARDS; h++) idle_waves += ld_rlx(&P.ctl->hungry[h].n);
-                if (idle_waves > 0) {
-                    const unsigned long long rc = ld_rlx(&P.ctl->root_cursor);
-                    const long long roots_left =
-                        rc < P.total_root_chunks ? (long long)(P.total_root_chunks - rc) : 0;
-                    want = idle_waves > max(0, ld_rlx(&P.ctl->queued)) + roots_left ? 1u : 0u;
-                }
-            }
-            want = bcast(want, 0);
+            const uint32_t want = split_wanted(P) ? 1u : 0u;
             run_rows = want ? min(P.task_rows, WAVE) : P.task_rows;
             if (want) {
                 // a candidate split may already have ended the task inside this batch
@@ -1445,7 +1488,7 @@ __device__ __forceinline__ void process_task(const KParams &P, const TaskDesc &t
                     cnt_local = first;
                     last_end = ALL_CANDIDATES;
                     if (lane == 0) stats[ST_SPLIT]++;
-                } else if (T - base >= 8 * WAVE) {
+                } else if (T - base >= SPLIT_MIN_CANDIDATES) {
                     const uint32_t mid = base + ((T - base) / (2 * WAVE)) * WAVE;
                     uint32_t lo = 0, hi = nb - 1;
                     while (lo < hi) {
@@ -1939,7 +1982,7 @@ __device__ __forceinline__ void match_waves(const KParams &P) {
         }
         if (flag == 1) {
             if (idle == 0 && lane == 0) {
-                atom_add(&P.ctl->hungry[blockIdx.x % HUNGRY_SHARDS].n, 1);
+                atom_add(&P.ctl->idle_waves, 1);
                 idle_since = wall_clock64();
             }
             // exponential back-off keeps idle waves off the queue counters
@@ -1948,7 +1991,7 @@ __device__ __forceinline__ void match_waves(const KParams &P) {
             idle++;
             continue;
         }
-        if (idle && lane == 0) atom_add(&P.ctl->hungry[blockIdx.x % HUNGRY_SHARDS].n, -1);
+        if (idle && lane == 0) atom_add(&P.ctl->idle_waves, -1);
         idle = 0;
         const TaskDesc t = s_task;
         __syncthreads();
@@ -1994,10 +2037,15 @@ __device__ __forceinline__ void match_waves(const KParams &P) {
     if (lane < ST_N && stats[lane]) atom_add(&P.ctl->stats[blockIdx.x % STAT_SHARDS][lane], stats[lane]);
 }
 
-// One query (PAIRED: two candidates per lane), and serving (one per lane).
+// One query (PAIRED: two candidates per lane), and serving (one per lane,
+// compiled for 3 waves per SIMD).
 template <bool PAIRED>
 __global__ __launch_bounds__(WAVE) void fm_match_kernel(KParams P) {
     match_waves<PAIRED>(P);
+}
+template <>
+__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3, 3))) void fm_match_kernel<false>(KParams P) {
+    match_waves<false>(P);
 }
 
 // Batches: two candidates per lane, compiled for 3 waves per SIMD (12 per

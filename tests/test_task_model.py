@@ -19,7 +19,7 @@ def u32(x):
     return x
 
 
-def process(task, lens, rng, queue, seen, p_split, p_yield, counted=None):
+def process(task, lens, rng, queue, seen, p_split, p_yield, counted=None, min_cand=32 * WAVE):
     first, count, skip, last_end = task
     cnt_local, last = count, last_end
     b0 = 0
@@ -76,7 +76,7 @@ def process(task, lens, rng, queue, seen, p_split, p_yield, counted=None):
                     f = b0 + nb + keep
                     queue.append((first + f, u32(rem_par - keep), 0, last))
                     cnt_local, last = f, ALL
-                elif T - base >= 8 * WAVE:
+                elif T - base >= min_cand:
                     mid = base + ((T - base) // (2 * WAVE)) * WAVE
                     lo = parent_of(mid)
                     excl = cum[lo - 1] if lo else 0
@@ -104,8 +104,12 @@ def process(task, lens, rng, queue, seen, p_split, p_yield, counted=None):
         b0 += WAVE
 
 
+# min_cand: the fewest remaining candidates a candidate split donates from
+# (SPLIT_MIN_CANDIDATES in the engine; any value of at least 2 x 64 keeps the
+# split point inside the remaining range)
+@pytest.mark.parametrize("min_cand", [2 * WAVE, 32 * WAVE])
 @pytest.mark.parametrize("seed", range(40))
-def test_every_candidate_exactly_once(seed):
+def test_every_candidate_exactly_once(seed, min_cand):
     rng = random.Random(seed)
     nparents = rng.choice([1, 5, 63, 64, 65, 300, 1024])
     lens = [rng.choice([0, 1, 3, 10, 64, 65, 200, 700, 5000]) for _ in range(nparents)]
@@ -119,7 +123,7 @@ def test_every_candidate_exactly_once(seed):
         assert guard < 1_000_000
         t = queue.pop(rng.randrange(len(queue)))
         assert t[1] > 0
-        process(t, lens, rng, queue, seen, p_split, p_yield, counted)
+        process(t, lens, rng, queue, seen, p_split, p_yield, counted, min_cand)
     want = {(p, c) for p in range(nparents) for c in range(lens[p])}
     assert seen == want
     # the algorithmic counters: every parent's row words and lookups are
