@@ -383,6 +383,38 @@ __device__ __forceinline__ uint32_t window_word(const Window20 &w, uint32_t j) {
     const uint4 &q = w.q[j >> 2];
     return (j & 3) == 0 ? q.x : (j & 3) == 1 ? q.y : (j & 3) == 2 ? q.z : q.w;
 }
+// Bits j of the window's words equal to x / at most x: one compare and one
+// select per word, no per-word range test (the caller masks the range once).
+// The bits of one 16-byte load are combined before the next load's, so few
+// intermediate values are live at once.
+__device__ __forceinline__ uint32_t window_eq_bits(const Window20 &w, uint32_t x) {
+    uint32_t m = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < 5; i++) {
+        const uint4 q = w.q[i];
+        const uint32_t g = (q.x == x ? 1u : 0u) | (q.y == x ? 2u : 0u) | (q.z == x ? 4u : 0u) | (q.w == x ? 8u : 0u);
+        m |= g << (4 * i);
+    }
+    return m;
+}
+__device__ __forceinline__ uint32_t window_le_bits(const Window20 &w, uint32_t x) {
+    uint32_t m = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < 5; i++) {
+        const uint4 q = w.q[i];
+        const uint32_t g = (q.x <= x ? 1u : 0u) | (q.y <= x ? 2u : 0u) | (q.z <= x ? 4u : 0u) | (q.w <= x ? 8u : 0u);
+        m |= g << (4 * i);
+    }
+    return m;
+}
+// Window positions [r, r + n) for n <= 16, r <= 3.
+__device__ __forceinline__ uint32_t range_bits(uint32_t r, uint32_t n) { return ((1u << n) - 1u) << r; }
+
+// The number of skip entries of a row of more than 16 words: the blocks
+// that start inside it, (st >> 4) + 1 .. (st + len - 1) >> 4.
+__device__ __forceinline__ uint32_t skip_entries(uint32_t st, uint32_t len) {
+    return ((st + len - 1) >> SKIP_SHIFT) - (st >> SKIP_SHIFT);
+}
 
 // Membership of x in the sorted, strictly increasing row col[st, st + len).
 // A row of at most 16 words is one window. A longer row first counts, in one
@@ -411,21 +443,14 @@ __device__ __forceinline__ bool row_contains(const uint32_t *col, const uint32_t
             }
         }
         const Window20 w = load_window(skip, blo);
-        const uint32_t s0 = blo & ~3u;
-        uint32_t c = 0;
-#pragma unroll
-        for (uint32_t j = 0; j < 20; j++) c += (s0 + j - blo < bn && window_word(w, j) <= x) ? 1u : 0u;
+        const uint32_t c = __popc(window_le_bits(w, x) & range_bits(blo & 3u, bn));
         const uint32_t bucket = blo - 1 + c;
         lo = max(st, bucket << SKIP_SHIFT);
         end = min(end, (bucket + 1) << SKIP_SHIFT);
     }
     // x, if present, lies in [lo, end), at most 16 positions
     const Window20 w = load_window(col, lo);
-    const uint32_t w0 = lo & ~3u;
-    bool hit = false;
-#pragma unroll
-    for (uint32_t j = 0; j < 20; j++) hit |= (w0 + j - lo < end - lo) && window_word(w, j) == x;
-    return hit;
+    return (window_eq_bits(w, x) & range_bits(lo & 3u, end - lo)) != 0;
 }
 
 // Membership of xa in row a and of xb in row b by the steps of
@@ -442,11 +467,11 @@ __device__ __forceinline__ void row_contains2(const uint32_t *col, const uint32_
         uint32_t bloa = 0, bna = 0, blob = 0, bnb = 0;
         if (sa) {
             bloa = (sta >> SKIP_SHIFT) + 1;
-            bna = ((enda - 1) >> SKIP_SHIFT) - (sta >> SKIP_SHIFT);
+            bna = skip_entries(sta, lena);
         }
         if (sb) {
             blob = (stb >> SKIP_SHIFT) + 1;
-            bnb = ((endb - 1) >> SKIP_SHIFT) - (stb >> SKIP_SHIFT);
+            bnb = skip_entries(stb, lenb);
         }
         while (bna > 16 || bnb > 16) {
             const uint32_t hfa = bna >> 1, hfb = bnb >> 1;
@@ -470,13 +495,8 @@ __device__ __forceinline__ void row_contains2(const uint32_t *col, const uint32_
             }
         }
         const Window20 wa = load_window(skip, bloa), wb = load_window(skip, blob);
-        const uint32_t sa0 = bloa & ~3u, sb0 = blob & ~3u;
-        uint32_t ca = 0, cb = 0;
-#pragma unroll
-        for (uint32_t j = 0; j < 20; j++) {
-            ca += (sa0 + j - bloa < bna && window_word(wa, j) <= xa) ? 1u : 0u;
-            cb += (sb0 + j - blob < bnb && window_word(wb, j) <= xb) ? 1u : 0u;
-        }
+        const uint32_t ca = __popc(window_le_bits(wa, xa) & range_bits(bloa & 3u, bna));
+        const uint32_t cb = __popc(window_le_bits(wb, xb) & range_bits(blob & 3u, bnb));
         if (sa) {
             const uint32_t bucket = bloa - 1 + ca;
             loa = max(sta, bucket << SKIP_SHIFT);
@@ -489,15 +509,8 @@ __device__ __forceinline__ void row_contains2(const uint32_t *col, const uint32_
         }
     }
     const Window20 wa = load_window(col, loa), wb = load_window(col, lob);
-    const uint32_t a0 = loa & ~3u, b0 = lob & ~3u;
-    bool fa = false, fb = false;
-#pragma unroll
-    for (uint32_t j = 0; j < 20; j++) {
-        fa |= (a0 + j - loa < enda - loa) && window_word(wa, j) == xa;
-        fb |= (b0 + j - lob < endb - lob) && window_word(wb, j) == xb;
-    }
-    ha = fa;
-    hb = fb;
+    ha = (window_eq_bits(wa, xa) & range_bits(loa & 3u, enda - loa)) != 0;
+    hb = (window_eq_bits(wb, xb) & range_bits(lob & 3u, endb - lob)) != 0;
 }
 
 __device__ __forceinline__ void row_of(const DGraph &g, uint32_t req, uint32_t v, uint32_t &st, uint32_t &len) {
@@ -2037,15 +2050,10 @@ __device__ __forceinline__ void match_waves(const KParams &P) {
     if (lane < ST_N && stats[lane]) atom_add(&P.ctl->stats[blockIdx.x % STAT_SHARDS][lane], stats[lane]);
 }
 
-// One query (PAIRED: two candidates per lane), and serving (one per lane,
-// compiled for 3 waves per SIMD).
+// One query (PAIRED: two candidates per lane), and serving (one per lane).
 template <bool PAIRED>
 __global__ __launch_bounds__(WAVE) void fm_match_kernel(KParams P) {
     match_waves<PAIRED>(P);
-}
-template <>
-__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3, 3))) void fm_match_kernel<false>(KParams P) {
-    match_waves<false>(P);
 }
 
 // Batches: two candidates per lane, compiled for 3 waves per SIMD (12 per

@@ -11,7 +11,9 @@ between the row's values, rows starting at any alignment), and every word
 the probe reads must lie inside the column array or the skip index plus
 their 64-word padding. The paired probe (row_contains2, two candidates per
 lane) is modelled step for step and must give the single probe's answers,
-with its dummy reads (index 0 of either array) in bounds too."""
+with its dummy reads (index 0 of either array) in bounds too. The window
+tests are bit masks, as in the engine: a bit per window word (equal to x, or
+at most x), masked once by the range's bits, which must fit the 20 words."""
 import numpy as np
 import pytest
 
@@ -25,6 +27,20 @@ def window(arr, lo, reads, name):
     for j in range(20):
         reads.append((name, w0 + j))
     return w0, [int(arr[w0 + j]) for j in range(20)]
+
+
+def eq_bits(w, x):
+    return sum(1 << j for j in range(20) if w[j] == x)
+
+
+def le_bits(w, x):
+    return sum(1 << j for j in range(20) if w[j] <= x)
+
+
+def range_bits(r, n):
+    """engine.hip range_bits: window positions [r, r + n), n <= 16, r <= 3."""
+    assert 0 <= r <= 3 and 0 <= n <= 16
+    return ((1 << n) - 1) << r
 
 
 def probe(col, skip, st, n, x, reads):
@@ -43,12 +59,13 @@ def probe(col, skip, st, n, x, reads):
             else:
                 bn = half
         s0, w = window(skip, blo, reads, "skip")
-        c = sum(1 for j in range(20) if 0 <= s0 + j - blo < bn and w[j] <= x)
+        assert blo - s0 == (blo & 3)
+        c = bin(le_bits(w, x) & range_bits(blo & 3, bn)).count("1")
         bucket = blo - 1 + c
         lo = max(st, bucket << SKIP_SHIFT)
         end = min(end, (bucket + 1) << SKIP_SHIFT)
     w0, w = window(col, lo, reads, "col")
-    return any(0 <= w0 + j - lo < end - lo and w[j] == x for j in range(20))
+    return (eq_bits(w, x) & range_bits(lo & 3, end - lo)) != 0
 
 
 def _arrays(rng, lens):
@@ -139,8 +156,8 @@ def probe2(col, skip, sta, na, xa, stb, nb, xb, reads):
                     bnb = hfb
         sa0, wa = window(skip, bloa, reads, "skip")
         sb0, wb = window(skip, blob, reads, "skip")
-        ca = sum(1 for j in range(20) if 0 <= sa0 + j - bloa < bna and wa[j] <= xa)
-        cb = sum(1 for j in range(20) if 0 <= sb0 + j - blob < bnb and wb[j] <= xb)
+        ca = bin(le_bits(wa, xa) & range_bits(bloa & 3, bna)).count("1")
+        cb = bin(le_bits(wb, xb) & range_bits(blob & 3, bnb)).count("1")
         if sa:
             bucket = bloa - 1 + ca
             loa, enda = max(sta, bucket << SKIP_SHIFT), min(enda, (bucket + 1) << SKIP_SHIFT)
@@ -149,8 +166,8 @@ def probe2(col, skip, sta, na, xa, stb, nb, xb, reads):
             lob, endb = max(stb, bucket << SKIP_SHIFT), min(endb, (bucket + 1) << SKIP_SHIFT)
     a0, wa = window(col, loa, reads, "col")
     b0, wb = window(col, lob, reads, "col")
-    ha = any(0 <= a0 + j - loa < enda - loa and wa[j] == xa for j in range(20))
-    hb = any(0 <= b0 + j - lob < endb - lob and wb[j] == xb for j in range(20))
+    ha = (eq_bits(wa, xa) & range_bits(loa & 3, enda - loa)) != 0
+    hb = (eq_bits(wb, xb) & range_bits(lob & 3, endb - lob)) != 0
     return ha, hb
 
 
