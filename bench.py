@@ -622,6 +622,14 @@ def run_batched(g, seed: int, nq: int, budget: int, steps: int, warmup: int, sav
                      "alg_bytes_per_launch": sum(balg), "kernel": "fm_match_kernel_batch",
                      "kernel_ms_per_launch": dom_ms},
     }
+    # HBM bytes of the batched launch, measured with rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE on this query set and calibrated against a copy
+    tpath = ROOT / "profiles" / "fm_match_kernel_rmat20_batched_traffic.json"
+    if tpath.exists():
+        tr = json.loads(tpath.read_text())
+        if tr.get("queries") == len(queries) and tr.get("embeddings_per_launch") == embeddings:
+            out["roofline"]["traffic"] = tr["hbm_bytes_per_launch"]
+            out["roofline"]["traffic_source"] = str(tpath.relative_to(ROOT))
     engine.close()
     return out, chosen
 
