@@ -916,6 +916,27 @@ def main():
                          "kernel": "fm_match_kernel", "kernel_ms_per_step": dom_ms},
         }
         workload = f"rmat24_stress_{len(queries)}q_dense10"
+        # Where the step's work goes (untimed): the same capped launch counted
+        # without tables and with the engine's counters, summed over the
+        # queries: partial embeddings per depth (depth 9 = embeddings found
+        # before the cap stopped each query), candidates examined, membership
+        # probes and neighbour-row words read.
+        rb.close()
+        rb = None
+        ri = engine.match_batch(g, queries, table=False, instrument=True, max_count=cap, timeout_ms=300000,
+                                allow_errors=True)
+        qo = [ri.query(i) for i in range(len(queries))]
+        depth = max(len(o.partials) for o in qo)
+        parts = [sum(o.partials[d] for o in qo if d < len(o.partials)) for d in range(depth)]
+        result["work_breakdown"] = {
+            "partials_per_depth": parts,
+            "candidates": sum(o.candidates for o in qo), "probes": sum(o.probes for o in qo),
+            "row_words": sum(o.alg_row_bytes for o in qo) // 4,
+            "kernel_ms": ri.stats().kernel_ms,
+            "note": "count mode, counters on, same cap; partials at depth d are the valid partial embeddings "
+                    "of d + 1 vertices, the last entry the embeddings found"}
+        ri.close()
+        log(f"stress breakdown: partials per depth {parts}")
 
     if not args.load_queries:
         result["intersect_roofline"] = intersect_bench(engine)

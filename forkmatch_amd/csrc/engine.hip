@@ -762,6 +762,14 @@ __device__ __forceinline__ bool claim_root(const KParams &P, TaskDesc &t, unsign
 
 struct WaveTab {
     uint32_t q, page, fill, fill0, cap;
+    // 1 + the query whose table this wave found full (no page for it): in a
+    // batch launch pages are never returned to the arena, so later rows of
+    // that query are dropped without touching its page counters. After its
+    // table fills, a query with a row cap keeps counting until its count
+    // cap stops it (on dense R-MAT 2^24 patterns, 10^10 embeddings past a
+    // 2.5x10^7-row cap); each emit retrying a page claim and flagging the
+    // table there took the 64-query stress launch from ~1 s to 75 s.
+    uint32_t full;
 };
 
 // Lane 0 only.
@@ -933,7 +941,8 @@ __device__ __forceinline__ void emit_rows(const KParams &P, WaveTab &wt, uint32_
     if (lane == 0) {
         const uint32_t cap = (uint32_t)(P.page_words / n);
         if (wt.page != NO_PAGE && wt.q != q) tab_release(P, wt);
-        if (wt.page == NO_PAGE) tab_acquire(P, wt, q, cap, true, stats);
+        const bool known_full = wt.full == q + 1;
+        if (wt.page == NO_PAGE && !known_full) tab_acquire(P, wt, q, cap, true, stats);
         if (wt.page != NO_PAGE) {
             pA = wt.page;
             fA = wt.fill;
@@ -948,8 +957,10 @@ __device__ __forceinline__ void emit_rows(const KParams &P, WaveTab &wt, uint32_
                 }
             }
         }
-        if (pA == NO_PAGE || (nA < cnt && pB == NO_PAGE))
+        if (!known_full && (pA == NO_PAGE || (nA < cnt && pB == NO_PAGE))) {
             __hip_atomic_fetch_or(&P.qs[q].flags, FM_RESULT_TRUNCATED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            wt.full = q + 1;
+        }
     }
     pA = bcast(pA, 0);
     fA = bcast(fA, 0);
@@ -1064,8 +1075,12 @@ __device__ __forceinline__ void flush_run(const KParams &P, WaveOut &o, uint32_t
 // into LDS, one row lookup per parent), so candidates are donated only from
 // long enough ranges: 2,048 candidates in inner levels, 4,096 in count-only
 // leaf batches. On the plain R-MAT 2^20 workload (32 queries) the thresholds
-// of 512 took 283 ms of kernel time per step against 263 ms.
+// of 512 took 283 ms of kernel time per step against 263 ms. Serving keeps
+// 512: a query served while few others run needs its hub rows spread over
+// the idle waves (at 200 requests/s on R-MAT 2^22, p50 latency 10.7 ms
+// against 7.6 ms with the larger threshold).
 constexpr uint32_t SPLIT_MIN_CANDIDATES = 32 * WAVE;
+constexpr uint32_t SPLIT_MIN_CANDIDATES_SERVING = 8 * WAVE;
 constexpr uint32_t SPLIT_MIN_CANDIDATES_LEAF = 64 * WAVE;
 __device__ __forceinline__ bool split_wanted(const KParams &P) {
     uint32_t want = 0;
@@ -1076,6 +1091,16 @@ __device__ __forceinline__ bool split_wanted(const KParams &P) {
         want = (long long)idle > (long long)max(0, ld_rlx(&P.ctl->queued)) + roots_left ? 1u : 0u;
     }
     return bcast(want, 0) != 0;
+}
+
+// Lane 0 adds a wave's count to its query's every flush_at embeddings (and
+// at task end); the count cap is checked there. With a cap, the wave
+// flushes after about 1/4096 of it: every wave counting 2^20 before its
+// check let 3,072 waves count ~35x past a 3.9x10^7 cap on dense R-MAT 2^24
+// patterns before the query stopped.
+__device__ __forceinline__ unsigned long long count_flush_at(const DQuery &Q) {
+    const unsigned long long cap = Q.max_count;
+    return cap ? min(1ull << 20, max(cap >> 12, 4096ull)) : (1ull << 20);
 }
 
 constexpr uint32_t LEAF_TILE = 4;
@@ -1101,6 +1126,7 @@ __device__ __forceinline__ void process_leaf_task(const KParams &P, const TaskDe
     const uint32_t NB = min(LEAF_TILE * P.tile_parents, (POOL_WORDS - LEAF_HEAD - 2 * LEAF_MAX) / W);
     const uint32_t naut = pl.naut > 1 ? pl.naut : 1u;
     unsigned long long cnt_acc = 0;
+    const unsigned long long flush_at = count_flush_at(Q);
     uint32_t cnt_local = t.count;    // shrinks when the upper parents are donated
     uint32_t last_end = t.last_end;  // end of the last parent's driver range
     for (uint32_t b0 = 0; b0 < cnt_local; b0 += NB) {
@@ -1247,7 +1273,7 @@ __device__ __forceinline__ void process_leaf_task(const KParams &P, const TaskDe
             for (uint32_t u = 0; u < LEAF_U; u++) hits += ok[u] ? 1u : 0u;
         }
         cnt_acc += wave_sum_u64(hits) * naut;
-        if (lane == 0 && cnt_acc >= (1ull << 20)) {
+        if (lane == 0 && cnt_acc >= flush_at) {
             const unsigned long long now_cnt = atom_add(&Q.count, cnt_acc) + cnt_acc;
             cnt_acc = 0;
             if (Q.max_count && now_cnt > Q.max_count) query_fail(P, qi, FM_ERR_COUNT_LIMIT);
@@ -1361,6 +1387,7 @@ __device__ __forceinline__ void process_task(const KParams &P, const TaskDesc &t
     uint32_t run_rows = P.task_rows;
 
     unsigned long long cnt_acc = 0;
+    const unsigned long long flush_at = count_flush_at(Q);
     uint32_t cnt_local = t.count;       // shrinks when the upper parents are donated
     uint32_t last_end = t.last_end;     // end of the last parent's driver range
     for (uint32_t b0 = 0; b0 < cnt_local; b0 += NB) {
@@ -1501,7 +1528,7 @@ __device__ __forceinline__ void process_task(const KParams &P, const TaskDesc &t
                     cnt_local = first;
                     last_end = ALL_CANDIDATES;
                     if (lane == 0) stats[ST_SPLIT]++;
-                } else if (T - base >= SPLIT_MIN_CANDIDATES) {
+                } else if (T - base >= (PAIRED ? SPLIT_MIN_CANDIDATES : SPLIT_MIN_CANDIDATES_SERVING)) {
                     const uint32_t mid = base + ((T - base) / (2 * WAVE)) * WAVE;
                     uint32_t lo = 0, hi = nb - 1;
                     while (lo < hi) {
@@ -1732,7 +1759,7 @@ __device__ __forceinline__ void process_task(const KParams &P, const TaskDesc &t
             }
         }
         account(T);
-        if (final_level && lane == 0 && cnt_acc >= (1ull << 20)) {
+        if (final_level && lane == 0 && cnt_acc >= flush_at) {
             unsigned long long now_cnt = atom_add(&Q.count, cnt_acc) + cnt_acc;
             cnt_acc = 0;
             if (Q.max_count && now_cnt > Q.max_count) query_fail(P, qi, FM_ERR_COUNT_LIMIT);
@@ -1879,7 +1906,7 @@ __device__ __forceinline__ void match_waves(const KParams &P) {
     WaveOut o{NO_BANK, 0, 0};
     uint32_t idle = 0;
     unsigned long long idle_since = 0;  // lane 0: when this wave ran out of work
-    if (lane == 0) wt = WaveTab{0, NO_PAGE, 0, 0, 0};
+    if (lane == 0) wt = WaveTab{0, NO_PAGE, 0, 0, 0, 0};
     // per-wave counters in LDS (lane 0 updates them), flushed at exit
     __shared__ unsigned long long stats[ST_N];
     if (lane == 0)
