@@ -2941,7 +2941,7 @@ int fm_match_batch(fm_engine *e, const fm_graph *g, const fm_query *const *qs, u
     fm_match_opts o{};
     if (opts) o = *opts;
     if (o.timeout_ms == 0) o.timeout_ms = 120000;
-    if (o.task_rows == 0) o.task_rows = nq == 1 ? SubMaxMulSpill : FM_BATCH_TASK_ROWS;
+    if (o.task_rows == 0) o.task_rows = nq == 1 ? FM_QUERY_TASK_ROWS : FM_BATCH_TASK_ROWS;
     if (o.tile_parents > WAVE)
         return fail(FM_ERR_INVALID_ARG, "tile_parents %u is above %u", o.tile_parents, WAVE);
     if (o.tile_parents == 0) o.tile_parents = WAVE;

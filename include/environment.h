@@ -45,8 +45,19 @@
  * its remaining parents or candidates. Measured on R-MAT 2^20 with 16
  * queries of 6 vertices, one launch each: 1024-parent tasks take 100 ms per
  * step, 64-parent tasks 215 ms (per-task queue and staging overhead, and
- * siblings of one parent kept together share their neighbour rows in L2). */
+ * siblings of one parent kept together share their neighbour rows in L2).
+ * The serving loop uses it; one-query launches use FM_QUERY_TASK_ROWS. */
 #define SubMaxMulSpill 1024
+
+/* The same for a launch that carries one query. On R-MAT 2^20 with the 32
+ * plain queries of 6-8 vertices (16 counted without tables), one launch
+ * each: 1024-parent tasks 259 ms of kernel time per step, 2048-parent tasks
+ * 243 ms, 4096-parent tasks 245 ms; the count-only trees gain 18-27% (every
+ * task costs several atomics on the queue, pending and bank words that all
+ * waves share). */
+#ifndef FM_QUERY_TASK_ROWS
+#define FM_QUERY_TASK_ROWS 2048
+#endif
 
 /* The same for launches that carry several queries: there the queries
  * themselves keep the waves busy. Measured on 1,024 queries of 4-12 vertices

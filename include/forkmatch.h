@@ -286,8 +286,8 @@ typedef struct fm_match_opts {
                                  an explicit value starts all at once and is
                                  capped at the resident waves: 8 per CU for
                                  one query, 12 for several on gfx950) */
-    uint32_t task_rows;       /* parents per queued task (0: SubMaxMulSpill for one
-                                 query, FM_BATCH_TASK_ROWS for several) */
+    uint32_t task_rows;       /* parents per queued task (0: FM_QUERY_TASK_ROWS for
+                                 one query, FM_BATCH_TASK_ROWS for several) */
     uint32_t root_chunk;      /* roots per claim (0: TooLargeThreshold; one
                                  query: ~8 chunks per starting wave, 64 to
                                  TooLargeThreshold roots) */
