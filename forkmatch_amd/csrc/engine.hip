@@ -1080,6 +1080,13 @@ __device__ __forceinline__ void flush_run(const KParams &P, WaveOut &o, uint32_t
 // the idle waves (at 200 requests/s on R-MAT 2^22, p50 latency 10.7 ms
 // against 7.6 ms with the larger threshold).
 constexpr uint32_t SPLIT_MIN_CANDIDATES = 32 * WAVE;
+// While waves are idle, a busy wave queues its children in tasks of this
+// many parents instead of task_rows, so they spread sooner. Every task costs
+// atomics on words all waves share: on the plain R-MAT 2^20 workload (32
+// queries) tasks of 64 parents took 243 ms of kernel time per step, 256
+// took 225 ms, 1,024 took 226 ms. Serving keeps 64, for the latency of a
+// query served while few others run.
+constexpr uint32_t IDLE_TASK_ROWS = 4 * WAVE;
 constexpr uint32_t SPLIT_MIN_CANDIDATES_SERVING = 8 * WAVE;
 constexpr uint32_t SPLIT_MIN_CANDIDATES_LEAF = 64 * WAVE;
 __device__ __forceinline__ bool split_wanted(const KParams &P) {
@@ -1516,7 +1523,7 @@ __device__ __forceinline__ void process_task(const KParams &P, const TaskDesc &t
         // hub parent's long driver row is split across waves).
         auto maybe_split = [&](uint32_t base) __attribute__((always_inline)) {
             const uint32_t want = split_wanted(P) ? 1u : 0u;
-            run_rows = want ? min(P.task_rows, WAVE) : P.task_rows;
+            run_rows = want ? min(P.task_rows, PAIRED ? IDLE_TASK_ROWS : WAVE) : P.task_rows;
             if (want) {
                 // a candidate split may already have ended the task inside this batch
                 const uint32_t rem_par = cnt_local > b0 + nb ? cnt_local - (b0 + nb) : 0;
